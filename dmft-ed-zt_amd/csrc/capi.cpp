@@ -1,0 +1,905 @@
+// C-ABI layer of the MI355X ED impurity solver: context, device memory,
+// per-spin table cache and the Lanczos / ground-state / Green's-function
+// drivers.  Entry points are declared in include/edgpu.h.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/edgpu.h"
+#include "edgpu_internal.h"
+
+namespace edg {
+int tridiag_lowest(int n, const double *d, const double *e, double *lambda, double *z);
+}
+
+using namespace edg;
+
+namespace {
+
+constexpr long long kPad = 64;  // element alignment of stored vectors
+
+long long round_up(long long n, long long m) { return (n + m - 1) / m * m; }
+
+long long binom(int n, int k)
+{
+    if (k < 0 || k > n) return 0;
+    long long r = 1;
+    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+    return r;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct TabEntry {
+    SpinTab t;
+    std::vector<void *> allocs;
+    unsigned long long version = 0;
+};
+
+// Device scalars of one Lanczos run.
+struct Scalars {
+    double *alpha, *beta, *scale, *misc;  // misc[0] = |v_init|, misc[1..3] RQ sums
+    int *flags;                            // flags[0] = stop, flags[1] = nstep
+};
+
+}  // namespace
+
+struct edg_ctx {
+    edg_params P{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    unsigned long long budget = 0;
+    int cols = 4;
+    bool acc64 = false;
+
+    std::vector<double> em, ek, vmk;
+    unsigned long long version = 0;
+    std::map<std::pair<int, int>, TabEntry> tabs;
+    DevBuf dparams;  // em | ek | vmk on device
+
+    double *partial = nullptr;
+    unsigned *ticket = nullptr;
+    Scalars sc{};
+    void *scal_mem = nullptr;
+
+    DevBuf basis, ring, gs[2], stage, tmp, coefbuf, gbuf, omegabuf, mapbuf;
+    int gs_slot = 0;
+
+    bool has_gs = false;
+    int gs_nu = 0, gs_nd = 0, gs_isector = 0, gs_nstep = 0;
+    long long gs_dim = 0;
+    double e0 = 0, e0_ritz = 0, residual = 0;
+
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    long long n_hv = 0, n_steps = 0;
+    bool have_gs_ev = false, have_gf_ev = false;
+};
+
+namespace {
+
+int fail(edg_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(c, call)                                                                 \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(c, EDG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int ensure(edg_ctx *c, DevBuf &b, size_t bytes)
+{
+    if (b.bytes >= bytes) return 0;
+    if (b.p) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipFree(b.p));
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    if (hipMalloc(&b.p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, EDG_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    b.bytes = bytes;
+    return 0;
+}
+
+size_t elem_size(const edg_ctx *c) { return c->P.precision == EDG_FP32 ? 4 : 8; }
+
+int check_sector(edg_ctx *c, int nu, int nd)
+{
+    const int ns = c->P.norb + c->P.nbath;
+    if (nu < 0 || nd < 0 || nu > ns || nd > ns)
+        return fail(c, EDG_ERR_ARG, "invalid sector (" + std::to_string(nu) + "," + std::to_string(nd) +
+                                        ") for nsite=" + std::to_string(ns));
+    return 0;
+}
+
+// Install em/ek/vmk; bump the version (invalidating tables) when they change.
+int set_params(edg_ctx *c, const double *em, const double *ek, const double *vmk)
+{
+    const int norb = c->P.norb, nbath = c->P.nbath;
+    if (!em || (!ek && nbath) || (!vmk && nbath)) return fail(c, EDG_ERR_ARG, "em/ek/vmk must not be NULL");
+    std::vector<double> a(em, em + 2 * norb), b, v;
+    if (nbath) {
+        b.assign(ek, ek + 2 * nbath);
+        v.assign(vmk, vmk + 2 * norb * nbath);
+    }
+    if (c->version && a == c->em && b == c->ek && v == c->vmk) return 0;
+    c->em = a;
+    c->ek = b;
+    c->vmk = v;
+    ++c->version;
+    const size_t n = a.size() + b.size() + v.size();
+    if (int rc = ensure(c, c->dparams, n * sizeof(double))) return rc;
+    std::vector<double> all;
+    all.insert(all.end(), a.begin(), a.end());
+    all.insert(all.end(), b.begin(), b.end());
+    all.insert(all.end(), v.begin(), v.end());
+    HIPCHK(c, hipMemcpyAsync(c->dparams.p, all.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int get_tab(edg_ctx *c, int spin, int ne, const SpinTab **out)
+{
+    auto key = std::make_pair(spin, ne);
+    TabEntry &te = c->tabs[key];
+    const int norb = c->P.norb, nbath = c->P.nbath, ns = norb + nbath;
+    if (te.allocs.empty()) {
+        SpinTab &t = te.t;
+        t.n = (int)binom(ns, ne);
+        t.K = norb * nbath;
+        auto al = [&](void **p, size_t bytes) -> int {
+            if (hipMalloc(p, bytes < 16 ? 16 : bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                return fail(c, EDG_ERR_NOMEM, "table allocation failed");
+            }
+            te.allocs.push_back(*p);
+            return 0;
+        };
+        const size_t n = t.n, K = t.K;
+        int rc = 0;
+        rc |= al((void **)&t.pattern, n * 4);
+        rc |= al((void **)&t.col, K * n * 2);
+        rc |= al((void **)&t.valf, K * n * 4);
+        rc |= al((void **)&t.vald, K * n * 8);
+        rc |= al((void **)&t.cnt, n);
+        rc |= al((void **)&t.energy, n * 8);
+        rc |= al((void **)&t.mask, n);
+        if (norb >= 2) rc |= al((void **)&t.orb, (size_t)norb * norb * n * 4);
+        if (rc) return rc;
+    }
+    if (te.version != c->version) {
+        const double *dp = (const double *)c->dparams.p;
+        const double *em_s = dp + (spin - 1) * norb;
+        const double *ek_s = dp + 2 * norb + (spin - 1) * nbath;
+        const double *v_s = dp + 2 * norb + 2 * nbath + (size_t)(spin - 1) * nbath * norb;
+        const double UpmJ = c->P.Up - c->P.J;
+        if (launch_spin_table(c->stream, ns, norb, nbath, ne, em_s, ek_s, v_s, c->P.mu, UpmJ, te.t))
+            return fail(c, EDG_ERR_HIP, "spin table kernel launch failed");
+        te.version = c->version;
+    }
+    *out = &te.t;
+    return 0;
+}
+
+int make_op(edg_ctx *c, int nu, int nd, HvOp *op)
+{
+    if (int rc = check_sector(c, nu, nd)) return rc;
+    const SpinTab *tu, *td;
+    if (int rc = get_tab(c, 1, nu, &tu)) return rc;
+    if (int rc = get_tab(c, 2, nd, &td)) return rc;
+    op->nup = tu->n;
+    op->ndn = td->n;
+    op->norb = c->P.norb;
+    op->up = *tu;
+    op->dn = *td;
+    op->U = c->P.U;
+    op->Up = c->P.Up;
+    op->Jp = c->P.Jp;
+    return 0;
+}
+
+HvArgs base_args(edg_ctx *c)
+{
+    HvArgs a{};
+    a.partial = c->partial;
+    a.ticket = c->ticket;
+    return a;
+}
+
+// Run up to maxn Lanczos steps (lanczos_solver.f90:92-173) on vectors held
+// either as a stored basis (slot j at base + j*stride, maxn+1 slots) or a
+// ring of 3 slots.  Slot 0 holds the unnormalized start vector.  On return
+// the host arrays a, b (maxn entries, zero-padded) and scales s_j (if not
+// null) are filled; *vinit_norm receives |v_init|.
+template <typename T>
+int run_lanczos(edg_ctx *c, const HvOp &op, T *base, long long stride, bool stored, int maxn,
+                int *nstep, double *a, double *b, double *scales, double *vinit_norm)
+{
+    const long long D = (long long)op.nup * op.ndn;
+    const Scalars &S = c->sc;
+    auto slot = [&](int j) -> T * { return base + (stored ? (long long)j : (long long)(j % 3)) * stride; };
+    HIPCHK(c, hipMemsetAsync(S.alpha, 0, sizeof(double) * maxn, c->stream));
+    HIPCHK(c, hipMemsetAsync(S.beta, 0, sizeof(double) * (maxn + 1), c->stream));
+    HIPCHK(c, hipMemsetAsync(S.scale, 0, sizeof(double) * (maxn + 1), c->stream));
+    int fl[2] = {0, maxn};
+    HIPCHK(c, hipMemcpyAsync(S.flags, fl, sizeof(fl), hipMemcpyHostToDevice, c->stream));
+    if (launch_norm<T>(c->stream, slot(0), D, c->partial, c->ticket, &S.misc[0], &S.scale[0], nullptr))
+        return fail(c, EDG_ERR_HIP, "norm launch failed");
+    for (int j = 0; j < maxn; ++j) {
+        HvArgs ha = base_args(c);
+        ha.x = slot(j);
+        ha.y = slot(j + 1);
+        ha.p = j > 0 ? slot(j - 1) : nullptr;
+        ha.sx = &S.scale[j];
+        ha.sp = j > 0 ? &S.scale[j - 1] : nullptr;
+        ha.beta = &S.beta[j];
+        ha.result = &S.alpha[j];
+        ha.stop = S.flags;
+        if (launch_hv<T>(c->stream, op, HV_LANCZOS, c->cols, c->acc64, ha))
+            return fail(c, EDG_ERR_HIP, "H.v launch failed");
+        ++c->n_hv;
+        ++c->n_steps;
+        if (j == maxn - 1) break;
+        if (launch_axpy_norm<T>(c->stream, slot(j + 1), slot(j), D, &S.alpha[j], &S.scale[j],
+                                c->partial, c->ticket, &S.beta[j + 1], &S.scale[j + 1], S.flags,
+                                S.flags + 1, j))
+            return fail(c, EDG_ERR_HIP, "axpy launch failed");
+    }
+    std::vector<double> ha(maxn), hb(maxn + 1), hs(maxn + 1);
+    int hf[2];
+    double nv;
+    HIPCHK(c, hipMemcpyAsync(ha.data(), S.alpha, sizeof(double) * maxn, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(hb.data(), S.beta, sizeof(double) * (maxn + 1), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(hs.data(), S.scale, sizeof(double) * (maxn + 1), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(hf, S.flags, sizeof(hf), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&nv, &S.misc[0], sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int ns = hf[1];
+    *nstep = ns;
+    for (int k = 0; k < maxn; ++k) {
+        a[k] = k < ns ? ha[k] : 0.0;
+        b[k] = (k >= 1 && k <= ns) ? hb[k] : 0.0;  // b(nstep+1) is kept when the chain broke down
+    }
+    if (ns < maxn && ns >= 1) b[ns] = hb[ns];
+    b[0] = 0.0;
+    if (scales)
+        for (int k = 0; k <= maxn; ++k) scales[k] = hs[k];
+    if (vinit_norm) *vinit_norm = nv;
+    return 0;
+}
+
+template <typename T>
+int upload(edg_ctx *c, T *dst, const double *src, long long n)
+{
+    if (int rc = ensure(c, c->stage, n * sizeof(double))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->stage.p, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (launch_from_f64<T>(c->stream, dst, (const double *)c->stage.p, n))
+        return fail(c, EDG_ERR_HIP, "convert launch failed");
+    return 0;
+}
+
+template <typename T>
+int download(edg_ctx *c, double *dst, const T *src, long long n)
+{
+    if (int rc = ensure(c, c->stage, n * sizeof(double))) return rc;
+    if (launch_to_f64<T>(c->stream, (double *)c->stage.p, src, n))
+        return fail(c, EDG_ERR_HIP, "convert launch failed");
+    HIPCHK(c, hipMemcpyAsync(dst, c->stage.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int admit(edg_ctx *c, size_t bytes)
+{
+    if (c->budget && bytes > c->budget)
+        return fail(c, EDG_ERR_ADMISSION, "request needs " + std::to_string(bytes) +
+                                              " device bytes, budget is " + std::to_string(c->budget));
+    return 0;
+}
+
+// diag_lanczos (lanczos_solver.f90:20-90) on one sector with the stored
+// Krylov basis: Lanczos, lowest Ritz pair, eigenvector = sum_j z_j q_j,
+// then E0 as the FP64 Rayleigh quotient of the stored vector.
+template <typename T>
+int diag_sector(edg_ctx *c, int nu, int nd, unsigned long long seed, T *gs_out, double *e0_ritz,
+                double *e0, double *residual, int *nstep_out)
+{
+    HvOp op;
+    if (int rc = make_op(c, nu, nd, &op)) return rc;
+    const int maxn = c->P.maxnstep;
+    const long long D = (long long)op.nup * op.ndn, Dp = round_up(D, kPad);
+    const size_t need = (size_t)(maxn + 1) * Dp * sizeof(T);
+    if (int rc = admit(c, need + 2 * Dp * sizeof(T))) return rc;
+    if (int rc = ensure(c, c->basis, need)) return rc;
+    T *base = (T *)c->basis.p;
+    if (launch_start<T>(c->stream, base, D, seed)) return fail(c, EDG_ERR_HIP, "start launch failed");
+    int nstep;
+    std::vector<double> a(maxn), b(maxn), s(maxn + 1);
+    if (int rc = run_lanczos<T>(c, op, base, Dp, true, maxn, &nstep, a.data(), b.data(), s.data(), nullptr))
+        return rc;
+    std::vector<double> z(nstep);
+    double lam;
+    if (tridiag_lowest(nstep, a.data(), b.data() + 1, &lam, z.data()))
+        return fail(c, EDG_ERR_SOLVER, "tridiagonal eigensolve failed");
+    std::vector<double> coef(nstep);
+    for (int j = 0; j < nstep; ++j) coef[j] = z[j] * s[j];
+    if (int rc = ensure(c, c->coefbuf, sizeof(double) * 4 * (maxn + 8))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->coefbuf.p, coef.data(), sizeof(double) * nstep, hipMemcpyHostToDevice,
+                             c->stream));
+    double *misc = c->sc.misc;
+    if (launch_combine<T>(c->stream, gs_out, base, Dp, nstep, (const double *)c->coefbuf.p, D,
+                          c->partial, c->ticket, &misc[4], &misc[5]))
+        return fail(c, EDG_ERR_HIP, "combine launch failed");
+    if (launch_scale<T>(c->stream, gs_out, D, &misc[5])) return fail(c, EDG_ERR_HIP, "scale launch failed");
+    HvArgs ha = base_args(c);
+    ha.x = gs_out;
+    ha.result = &misc[1];
+    if (launch_hv<T>(c->stream, op, HV_RQ, c->cols, true, ha)) return fail(c, EDG_ERR_HIP, "RQ launch failed");
+    ++c->n_hv;
+    double sums[3];
+    HIPCHK(c, hipMemcpyAsync(sums, &misc[1], sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double xx = sums[2], xHx = sums[0], HH = sums[1];
+    const double rq = xHx / xx;
+    *e0_ritz = lam;
+    *e0 = rq;
+    *residual = (HH - 2.0 * rq * xHx + rq * rq * xx) / xx;
+    if (*residual < 0) *residual = 0.0;
+    *nstep_out = nstep;
+    return 0;
+}
+
+template <typename T>
+int ground_state_t(edg_ctx *c, int nsector, const int *sectors, int *isector, double *e0)
+{
+    c->has_gs = false;
+    const size_t es = sizeof(T);
+    double best = HUGE_VAL;
+    int bi = 0;
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    for (int i = 0; i < nsector; ++i) {
+        const int nu = sectors[2 * i], nd = sectors[2 * i + 1];
+        if (int rc = check_sector(c, nu, nd)) return rc;
+        const long long D = binom(c->P.norb + c->P.nbath, nu) * binom(c->P.norb + c->P.nbath, nd);
+        const int cand = 1 - c->gs_slot;
+        if (int rc = ensure(c, c->gs[cand], round_up(D, kPad) * es)) return rc;
+        double er, e, res;
+        int ns;
+        if (int rc = diag_sector<T>(c, nu, nd, c->P.seed + i, (T *)c->gs[cand].p, &er, &e, &res, &ns))
+            return rc;
+        if (e < best) {  // strict minimum: first lowest wins (ground_state.f90:62)
+            best = e;
+            bi = i + 1;
+            c->gs_slot = cand;
+            c->gs_nu = nu;
+            c->gs_nd = nd;
+            c->gs_dim = D;
+            c->e0 = e;
+            c->e0_ritz = er;
+            c->residual = res;
+            c->gs_nstep = ns;
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    c->have_gs_ev = true;
+    c->has_gs = true;
+    c->gs_isector = bi;
+    *isector = bi;
+    *e0 = best;
+    return 0;
+}
+
+// One green_diag chain (dmft_green.f90:202-257, one of G+ / G-): v_init =
+// c(+)|gs> in the neighbouring sector, Lanczos, and the coefficient arrays
+// with b(1) = |v_init| (:219,:249).
+template <typename T>
+int chain(edg_ctx *c, int iorb, int ispin, int pm, double *aa, double *bb, int *nstep)
+{
+    const int ns_site = c->P.norb + c->P.nbath, maxn = c->P.maxnstep_green;
+    const int nu = c->gs_nu + (ispin == 1 ? (pm == 1 ? 1 : -1) : 0);
+    const int nd = c->gs_nd + (ispin == 2 ? (pm == 1 ? 1 : -1) : 0);
+    for (int k = 0; k < maxn; ++k) aa[k] = bb[k] = 0.0;
+    *nstep = 0;
+    if (nu < 0 || nd < 0 || nu > ns_site || nd > ns_site) return 0;  // no such sector: no weight
+    HvOp op, op0;
+    if (int rc = make_op(c, nu, nd, &op)) return rc;
+    if (int rc = make_op(c, c->gs_nu, c->gs_nd, &op0)) return rc;
+    const long long D = (long long)op.nup * op.ndn, Dp = round_up(D, kPad);
+    if (int rc = admit(c, 3 * Dp * sizeof(T))) return rc;
+    if (int rc = ensure(c, c->ring, 3 * Dp * sizeof(T))) return rc;
+    if (int rc = ensure(c, c->mapbuf, sizeof(uint32_t) * 8192)) return rc;
+    T *base = (T *)c->ring.p;
+    const SpinTab &tout = ispin == 1 ? op.up : op.dn;
+    const int ne_in = ispin == 1 ? c->gs_nu : c->gs_nd;
+    if (launch_cmap(c->stream, tout.pattern, tout.n, ns_site, ne_in, iorb - 1, pm, (uint32_t *)c->mapbuf.p))
+        return fail(c, EDG_ERR_HIP, "cmap launch failed");
+    const int godd = ispin == 2 ? (c->gs_nu & 1) : 0;
+    if (launch_apply_c<T>(c->stream, (const T *)c->gs[c->gs_slot].p, base, op.nup, op.ndn, op0.nup, ispin,
+                          (const uint32_t *)c->mapbuf.p, godd))
+        return fail(c, EDG_ERR_HIP, "apply_c launch failed");
+    double nv;
+    if (int rc = run_lanczos<T>(c, op, base, Dp, false, maxn, nstep, aa, bb, nullptr, &nv)) return rc;
+    bb[0] = nv;
+    return 0;
+}
+
+template <typename T>
+int cluster_green_t(edg_ctx *c, int nw, const double *omega, double *G_cl, double *coeffs)
+{
+    if (!c->has_gs) return fail(c, EDG_ERR_STATE, "no ground state held: call edg_ground_state first");
+    const int norb = c->P.norb, nspin = c->P.nspin, maxn = c->P.maxnstep_green;
+    const bool even = !(nspin == 2 || c->gs_nu != c->gs_nd);
+    if (int rc = ensure(c, c->omegabuf, sizeof(double) * nw)) return rc;
+    if (int rc = ensure(c, c->gbuf, sizeof(double) * 2 * nw * 2)) return rc;
+    if (int rc = ensure(c, c->coefbuf, sizeof(double) * 4 * (maxn + 8))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->omegabuf.p, omega, sizeof(double) * nw, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    std::vector<double> g(2 * nw * 2);
+    std::vector<double> cf(4 * maxn);
+    for (int iorb = 1; iorb <= norb; ++iorb) {
+        double *co[2][4];
+        std::vector<double> keep[2];
+        for (int ispin = 1; ispin <= 2; ++ispin) {
+            const bool run = ispin == 1 || !even || c->P.both_spins;
+            keep[ispin - 1].assign(4 * maxn, 0.0);
+            for (int q = 0; q < 4; ++q) co[ispin - 1][q] = keep[ispin - 1].data() + q * maxn;
+            if (!run) {
+                keep[1] = keep[0];  // the reference writes spin-1 coefficients again (:102-109)
+                continue;
+            }
+            int np, nm;
+            if (int rc = chain<T>(c, iorb, ispin, 1, co[ispin - 1][0], co[ispin - 1][1], &np)) return rc;
+            if (int rc = chain<T>(c, iorb, ispin, 2, co[ispin - 1][2], co[ispin - 1][3], &nm)) return rc;
+            HIPCHK(c, hipMemcpyAsync(c->coefbuf.p, keep[ispin - 1].data(), sizeof(double) * 4 * maxn,
+                                     hipMemcpyHostToDevice, c->stream));
+            const double *dc = (const double *)c->coefbuf.p;
+            double *dg = (double *)c->gbuf.p + (size_t)(ispin - 1) * 2 * nw;
+            if (launch_cf(c->stream, nw, (const double *)c->omegabuf.p, c->e0, np, dc, dc + maxn, nm,
+                          dc + 2 * maxn, dc + 3 * maxn, dg, 0))
+                return fail(c, EDG_ERR_HIP, "continued fraction launch failed");
+            HIPCHK(c, hipMemcpyAsync(g.data() + (size_t)(ispin - 1) * 2 * nw, dg, sizeof(double) * 2 * nw,
+                                     hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        // G_cl assembly (dmft_green.f90:111-118)
+        for (int iw = 0; iw < nw; ++iw) {
+            const double r1 = g[2 * iw], i1 = g[2 * iw + 1];
+            const double r2 = g[2 * nw + 2 * iw], i2 = g[2 * nw + 2 * iw + 1];
+            double *o1 = G_cl + 2 * ((size_t)(iorb - 1) * nw + iw);
+            if (nspin == 1 && even) {
+                o1[0] = r1;
+                o1[1] = i1;
+            } else if (nspin == 1) {
+                o1[0] = 0.5 * (r1 + r2);
+                o1[1] = 0.5 * (i1 + i2);
+            } else {
+                o1[0] = r1;
+                o1[1] = i1;
+                double *o2 = G_cl + 2 * ((size_t)norb * nw + (size_t)(iorb - 1) * nw + iw);
+                o2[0] = r2;
+                o2[1] = i2;
+            }
+        }
+        if (coeffs) {
+            for (int ispin = 1; ispin <= 2; ++ispin)
+                for (int q = 0; q < 4; ++q)
+                    std::memcpy(coeffs + (((size_t)(iorb - 1) * 2 + (ispin - 1)) * 4 + q) * maxn,
+                                keep[ispin - 1].data() + q * maxn, sizeof(double) * maxn);
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_gf_ev = true;
+    return 0;
+}
+
+template <typename T>
+int hx_t(edg_ctx *c, int nu, int nd, const double *x, double *y, int py)
+{
+    HvOp op;
+    if (int rc = make_op(c, nu, nd, &op)) return rc;
+    const long long D = (long long)op.nup * op.ndn, Dp = round_up(D, kPad);
+    if (int rc = ensure(c, c->tmp, 2 * Dp * sizeof(T))) return rc;
+    T *dx = (T *)c->tmp.p, *dy = dx + Dp;
+    if (int rc = upload<T>(c, dx, x, D)) return rc;
+    if (py)
+        if (int rc = upload<T>(c, dy, y, D)) return rc;
+    HvArgs a = base_args(c);
+    a.x = dx;
+    a.y = dy;
+    a.accumulate = py;
+    if (launch_hv<T>(c->stream, op, HV_PLAIN, c->cols, c->acc64, a)) return fail(c, EDG_ERR_HIP, "H.v launch failed");
+    ++c->n_hv;
+    return download<T>(c, y, dy, D);
+}
+
+template <typename T>
+int lanczos_t(edg_ctx *c, int nu, int nd, const double *v_init, int *nstep, double *a, double *b)
+{
+    HvOp op;
+    if (int rc = make_op(c, nu, nd, &op)) return rc;
+    const long long D = (long long)op.nup * op.ndn, Dp = round_up(D, kPad);
+    if (int rc = admit(c, 3 * Dp * sizeof(T))) return rc;
+    if (int rc = ensure(c, c->ring, 3 * Dp * sizeof(T))) return rc;
+    if (int rc = upload<T>(c, (T *)c->ring.p, v_init, D)) return rc;
+    return run_lanczos<T>(c, op, (T *)c->ring.p, Dp, false, c->P.maxnstep, nstep, a, b, nullptr, nullptr);
+}
+
+template <typename T>
+int apply_c_t(edg_ctx *c, int nu, int nd, const double *vec, int pm, int iorb, int ispin, double *out)
+{
+    const int ns = c->P.norb + c->P.nbath;
+    const int nu2 = nu + (ispin == 1 ? (pm == 1 ? 1 : -1) : 0);
+    const int nd2 = nd + (ispin == 2 ? (pm == 1 ? 1 : -1) : 0);
+    if (int rc = check_sector(c, nu2, nd2)) return rc;
+    HvOp op0, op1;
+    if (int rc = make_op(c, nu, nd, &op0)) return rc;
+    if (int rc = make_op(c, nu2, nd2, &op1)) return rc;
+    const long long D0 = (long long)op0.nup * op0.ndn, D1 = (long long)op1.nup * op1.ndn;
+    const long long P0 = round_up(D0, kPad), P1 = round_up(D1, kPad);
+    if (int rc = ensure(c, c->tmp, (P0 + P1) * sizeof(T))) return rc;
+    if (int rc = ensure(c, c->mapbuf, sizeof(uint32_t) * 8192)) return rc;
+    T *din = (T *)c->tmp.p, *dout = din + P0;
+    if (int rc = upload<T>(c, din, vec, D0)) return rc;
+    const SpinTab &tout = ispin == 1 ? op1.up : op1.dn;
+    if (launch_cmap(c->stream, tout.pattern, tout.n, ns, ispin == 1 ? nu : nd, iorb - 1, pm,
+                    (uint32_t *)c->mapbuf.p))
+        return fail(c, EDG_ERR_HIP, "cmap launch failed");
+    if (launch_apply_c<T>(c->stream, din, dout, op1.nup, op1.ndn, op0.nup, ispin, (const uint32_t *)c->mapbuf.p,
+                          ispin == 2 ? (nu & 1) : 0))
+        return fail(c, EDG_ERR_HIP, "apply_c launch failed");
+    return download<T>(c, out, dout, D1);
+}
+
+template <typename T>
+int time_hv_t(edg_ctx *c, int nu, int nd, int warmup, int reps, double *ms, double *bytes)
+{
+    HvOp op;
+    if (int rc = make_op(c, nu, nd, &op)) return rc;
+    const long long D = (long long)op.nup * op.ndn, Dp = round_up(D, kPad);
+    if (int rc = ensure(c, c->tmp, 2 * Dp * sizeof(T))) return rc;
+    T *dx = (T *)c->tmp.p, *dy = dx + Dp;
+    c->have_gs_ev = false;
+    if (launch_start<T>(c->stream, dx, D, 12345)) return fail(c, EDG_ERR_HIP, "start launch failed");
+    HvArgs a = base_args(c);
+    a.x = dx;
+    a.y = dy;
+    for (int i = 0; i < warmup; ++i)
+        if (launch_hv<T>(c->stream, op, HV_PLAIN, c->cols, c->acc64, a)) return fail(c, EDG_ERR_HIP, "H.v launch failed");
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    for (int i = 0; i < reps; ++i)
+        if (launch_hv<T>(c->stream, op, HV_PLAIN, c->cols, c->acc64, a)) return fail(c, EDG_ERR_HIP, "H.v launch failed");
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev[1]));
+    float t = 0;
+    HIPCHK(c, hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
+    *ms = (double)t / reps;
+    *bytes = 2.0 * sizeof(T) * (double)D;
+    return 0;
+}
+
+}  // namespace
+
+// ============================================================== C ABI
+
+extern "C" {
+
+void edg_default_params(edg_params *p)
+{
+    std::memset(p, 0, sizeof(*p));
+    p->norb = 1;
+    p->nbath = 5;
+    p->nspin = 1;
+    p->U = 1.0;
+    p->J = 0.3;
+    p->Up = -1.0;
+    p->Jp = -1.0;
+    p->mu = 0.5;
+    p->maxnstep = 80;
+    p->diag_solver = 1;
+    p->precision = EDG_FP64;
+    p->both_spins = 0;
+    p->seed = 1;
+}
+
+int edg_device_count(int *n)
+{
+    int k = 0;
+    if (hipGetDeviceCount(&k) != hipSuccess) {
+        (void)hipGetLastError();
+        k = 0;
+    }
+    *n = k;
+    return 0;
+}
+
+int edg_create(const edg_params *p, int device, edg_ctx **out)
+{
+    if (!p || !out) return EDG_ERR_ARG;
+    *out = nullptr;
+    if (p->diag_solver != 1) return EDG_ERR_ARG;  // ARPACK path (arpack_solver.f90) not provided
+    if (p->norb < 1 || p->nbath < 0 || p->norb + p->nbath > 15 || p->norb > kMaxNorb ||
+        p->norb * p->nbath > kMaxHops || p->maxnstep < 1 || p->maxnstep > 1000 ||
+        p->maxnstep_green < 0 || p->maxnstep_green > 1000 ||
+        (p->nspin != 1 && p->nspin != 2) || (p->precision != EDG_FP32 && p->precision != EDG_FP64))
+        return EDG_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+        (void)hipGetLastError();
+        return EDG_ERR_HIP;
+    }
+    edg_ctx *c = new edg_ctx();
+    c->P = *p;
+    if (c->P.Up < 0) c->P.Up = c->P.U - 2.0 * c->P.J;  // dmft_params.f90:90-92
+    if (c->P.Jp < 0) c->P.Jp = c->P.J;                  // dmft_params.f90:94-96
+    if (c->P.maxnstep_green <= 0) c->P.maxnstep_green = c->P.maxnstep;
+    c->device = device;
+    c->cols = p->precision == EDG_FP32 ? 4 : 2;
+    auto bad = [&]() {
+        (void)hipGetLastError();
+        delete c;
+        return EDG_ERR_HIP;
+    };
+    if (hipSetDevice(device) != hipSuccess) return bad();
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bad();
+    if (hipMalloc((void **)&c->partial, sizeof(double) * 16 * 8192 + 64) != hipSuccess) return bad();
+    if (hipMalloc((void **)&c->ticket, 256) != hipSuccess) return bad();
+    if (hipMemset(c->ticket, 0, 256) != hipSuccess) return bad();
+    const int mx = (p->maxnstep > p->maxnstep_green ? p->maxnstep : p->maxnstep_green) + 8;
+    if (hipMalloc(&c->scal_mem, sizeof(double) * (3 * mx + 40) + 64) != hipSuccess) return bad();
+    double *s = (double *)c->scal_mem;
+    c->sc.alpha = s;
+    c->sc.beta = s + mx;
+    c->sc.scale = s + 2 * mx;
+    c->sc.misc = s + 3 * mx;
+    c->sc.flags = (int *)(s + 3 * mx + 32);
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) return bad();
+    *out = c;
+    return EDG_OK;
+}
+
+int edg_destroy(edg_ctx *c)
+{
+    if (!c) return EDG_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &kv : c->tabs)
+        for (void *p : kv.second.allocs) (void)hipFree(p);
+    DevBuf *bufs[] = {&c->dparams, &c->basis, &c->ring, &c->gs[0], &c->gs[1], &c->stage, &c->tmp,
+                      &c->coefbuf, &c->gbuf, &c->omegabuf, &c->mapbuf};
+    for (DevBuf *b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    if (c->partial) (void)hipFree(c->partial);
+    if (c->ticket) (void)hipFree(c->ticket);
+    if (c->scal_mem) (void)hipFree(c->scal_mem);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return EDG_OK;
+}
+
+const char *edg_last_error(const edg_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int edg_set_memory_budget(edg_ctx *c, unsigned long long bytes)
+{
+    if (!c) return EDG_ERR_ARG;
+    c->budget = bytes;
+    return EDG_OK;
+}
+
+int edg_set_tuning(edg_ctx *c, int cols_per_block, int fp64_accumulate)
+{
+    if (!c || (cols_per_block != 1 && cols_per_block != 2 && cols_per_block != 4 && cols_per_block != 8))
+        return EDG_ERR_ARG;
+    c->cols = cols_per_block;
+    c->acc64 = fp64_accumulate != 0;
+    return EDG_OK;
+}
+
+int edg_spin_table(edg_ctx *c, const double *em, const double *ek, const double *vmk, int ispin,
+                   int ne, int *n, int *kmax, unsigned *pattern, int *hop_col, double *hop_val,
+                   int *hop_cnt, double *energy)
+{
+    if (!c) return EDG_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (ispin != 1 && ispin != 2) return fail(c, EDG_ERR_ARG, "ispin must be 1 or 2");
+    if (int rc = check_sector(c, ne, 0)) return rc;
+    if (int rc = set_params(c, em, ek, vmk)) return rc;
+    const SpinTab *t;
+    if (int rc = get_tab(c, ispin, ne, &t)) return rc;
+    *n = t->n;
+    *kmax = t->K;
+    const size_t N = t->n, K = t->K;
+    if (pattern) HIPCHK(c, hipMemcpyAsync(pattern, t->pattern, N * 4, hipMemcpyDeviceToHost, c->stream));
+    std::vector<uint16_t> col(K * N);
+    std::vector<uint8_t> cnt(N);
+    if (hop_col && K) HIPCHK(c, hipMemcpyAsync(col.data(), t->col, K * N * 2, hipMemcpyDeviceToHost, c->stream));
+    if (hop_val && K) HIPCHK(c, hipMemcpyAsync(hop_val, t->vald, K * N * 8, hipMemcpyDeviceToHost, c->stream));
+    if (hop_cnt) HIPCHK(c, hipMemcpyAsync(cnt.data(), t->cnt, N, hipMemcpyDeviceToHost, c->stream));
+    if (energy) HIPCHK(c, hipMemcpyAsync(energy, t->energy, N * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (hop_col)
+        for (size_t i = 0; i < K * N; ++i) hop_col[i] = col[i] == kNoCol ? -1 : (int)col[i];
+    if (hop_cnt)
+        for (size_t i = 0; i < N; ++i) hop_cnt[i] = cnt[i];
+    return EDG_OK;
+}
+
+
+int edg_hx(edg_ctx *c, const double *em, const double *ek, const double *vmk, int ne_up, int ne_dn,
+           const double *x, double *y, int py)
+{
+    if (!c || !x || !y) return EDG_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (int rc = set_params(c, em, ek, vmk)) return rc;
+    return c->P.precision == EDG_FP32 ? hx_t<float>(c, ne_up, ne_dn, x, y, py)
+                                      : hx_t<double>(c, ne_up, ne_dn, x, y, py);
+}
+
+
+int edg_lanczos_iteration(edg_ctx *c, const double *em, const double *ek, const double *vmk,
+                          int ne_up, int ne_dn, const double *v_init, int *nstep, double *a, double *b)
+{
+    if (!c || !v_init || !nstep || !a || !b) return EDG_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (int rc = set_params(c, em, ek, vmk)) return rc;
+    return c->P.precision == EDG_FP32 ? lanczos_t<float>(c, ne_up, ne_dn, v_init, nstep, a, b)
+                                      : lanczos_t<double>(c, ne_up, ne_dn, v_init, nstep, a, b);
+}
+
+int edg_ground_state(edg_ctx *c, const double *em, const double *ek, const double *vmk, int nsector,
+                     const int *sectors, int *isector, double *e0)
+{
+    if (!c || nsector < 1 || !sectors || !isector || !e0) return EDG_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (int rc = set_params(c, em, ek, vmk)) return rc;
+    c->n_hv = c->n_steps = 0;
+    return c->P.precision == EDG_FP32 ? ground_state_t<float>(c, nsector, sectors, isector, e0)
+                                      : ground_state_t<double>(c, nsector, sectors, isector, e0);
+}
+
+int edg_gs_info(edg_ctx *c, int *ne_up, int *ne_dn, long long *dim, double *e0_ritz, double *e0,
+                double *residual, int *nstep)
+{
+    if (!c) return EDG_ERR_ARG;
+    if (!c->has_gs) return fail(c, EDG_ERR_STATE, "no ground state held");
+    if (ne_up) *ne_up = c->gs_nu;
+    if (ne_dn) *ne_dn = c->gs_nd;
+    if (dim) *dim = c->gs_dim;
+    if (e0_ritz) *e0_ritz = c->e0_ritz;
+    if (e0) *e0 = c->e0;
+    if (residual) *residual = c->residual;
+    if (nstep) *nstep = c->gs_nstep;
+    return EDG_OK;
+}
+
+int edg_get_gs_vector(edg_ctx *c, double *vec, long long dim)
+{
+    if (!c || !vec) return EDG_ERR_ARG;
+    if (!c->has_gs) return fail(c, EDG_ERR_STATE, "no ground state held");
+    if (dim != c->gs_dim) return fail(c, EDG_ERR_ARG, "dimension mismatch");
+    (void)hipSetDevice(c->device);
+    return c->P.precision == EDG_FP32 ? download<float>(c, vec, (const float *)c->gs[c->gs_slot].p, dim)
+                                      : download<double>(c, vec, (const double *)c->gs[c->gs_slot].p, dim);
+}
+
+int edg_set_gs(edg_ctx *c, int ne_up, int ne_dn, double e0, const double *vec, long long dim)
+{
+    if (!c || !vec) return EDG_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (int rc = check_sector(c, ne_up, ne_dn)) return rc;
+    const int ns = c->P.norb + c->P.nbath;
+    if (dim != binom(ns, ne_up) * binom(ns, ne_dn)) return fail(c, EDG_ERR_ARG, "dimension mismatch");
+    const size_t es = elem_size(c);
+    if (int rc = ensure(c, c->gs[c->gs_slot], round_up(dim, kPad) * es)) return rc;
+    int rc = c->P.precision == EDG_FP32 ? upload<float>(c, (float *)c->gs[c->gs_slot].p, vec, dim)
+                                        : upload<double>(c, (double *)c->gs[c->gs_slot].p, vec, dim);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->has_gs = true;
+    c->gs_nu = ne_up;
+    c->gs_nd = ne_dn;
+    c->gs_dim = dim;
+    c->e0 = c->e0_ritz = e0;
+    c->residual = 0.0;
+    c->gs_nstep = 0;
+    c->gs_isector = 1;
+    return EDG_OK;
+}
+
+int edg_cluster_green(edg_ctx *c, const double *em, const double *ek, const double *vmk, int nw,
+                      const double *omega, double *G_cl, double *coeffs)
+{
+    if (!c || nw < 1 || !omega || !G_cl) return EDG_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (int rc = set_params(c, em, ek, vmk)) return rc;
+    return c->P.precision == EDG_FP32 ? cluster_green_t<float>(c, nw, omega, G_cl, coeffs)
+                                      : cluster_green_t<double>(c, nw, omega, G_cl, coeffs);
+}
+
+int edg_observables(edg_ctx *c, double *occ, double *docc)
+{
+    if (!c || !occ || !docc) return EDG_ERR_ARG;
+    if (!c->has_gs) return fail(c, EDG_ERR_STATE, "no ground state held");
+    (void)hipSetDevice(c->device);
+    HvOp op;
+    if (int rc = make_op(c, c->gs_nu, c->gs_nd, &op)) return rc;
+    double *out = c->sc.misc + 6;
+    int rc = c->P.precision == EDG_FP32
+                 ? launch_observables<float>(c->stream, (const float *)c->gs[c->gs_slot].p, op, nullptr,
+                                             c->partial, c->ticket, out)
+                 : launch_observables<double>(c->stream, (const double *)c->gs[c->gs_slot].p, op, nullptr,
+                                              c->partial, c->ticket, out);
+    if (rc) return fail(c, EDG_ERR_HIP, "observables launch failed");
+    const int norb = c->P.norb;
+    std::vector<double> h(3 * norb);
+    HIPCHK(c, hipMemcpyAsync(h.data(), out, sizeof(double) * 3 * norb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int m = 0; m < norb; ++m) {
+        occ[m] = h[m];
+        occ[norb + m] = h[norb + m];
+        docc[m] = h[2 * norb + m];
+    }
+    return EDG_OK;
+}
+
+
+int edg_apply_c(edg_ctx *c, int ne_up, int ne_dn, const double *vec, int pm, int iorb, int ispin,
+                double *vec_out)
+{
+    if (!c || !vec || !vec_out || (pm != 1 && pm != 2) || (ispin != 1 && ispin != 2) || iorb < 1 ||
+        iorb > (c ? c->P.norb : 0))
+        return c ? fail(c, EDG_ERR_ARG, "bad apply_c arguments") : EDG_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (c->em.empty()) {  // apply_c needs only the basis: install zero parameters
+        std::vector<double> z(2 * c->P.norb * (c->P.nbath + 1) + 2 * c->P.nbath, 0.0);
+        if (int rc = set_params(c, z.data(), z.data(), z.data())) return rc;
+    }
+    return c->P.precision == EDG_FP32 ? apply_c_t<float>(c, ne_up, ne_dn, vec, pm, iorb, ispin, vec_out)
+                                      : apply_c_t<double>(c, ne_up, ne_dn, vec, pm, iorb, ispin, vec_out);
+}
+
+
+int edg_time_hv(edg_ctx *c, const double *em, const double *ek, const double *vmk, int ne_up, int ne_dn,
+                int warmup, int reps, double *ms_per_launch, double *bytes_per_launch)
+{
+    if (!c || reps < 1 || warmup < 0 || !ms_per_launch || !bytes_per_launch) return EDG_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (int rc = set_params(c, em, ek, vmk)) return rc;
+    return c->P.precision == EDG_FP32 ? time_hv_t<float>(c, ne_up, ne_dn, warmup, reps, ms_per_launch, bytes_per_launch)
+                                      : time_hv_t<double>(c, ne_up, ne_dn, warmup, reps, ms_per_launch, bytes_per_launch);
+}
+
+int edg_last_solve_stats(edg_ctx *c, double *phase_ms, long long *n_hv, long long *n_steps)
+{
+    if (!c || !phase_ms) return EDG_ERR_ARG;
+    phase_ms[0] = phase_ms[1] = 0.0;
+    float t;
+    if (c->have_gs_ev && hipEventElapsedTime(&t, c->ev[0], c->ev[1]) == hipSuccess) phase_ms[0] = t;
+    if (c->have_gf_ev && hipEventElapsedTime(&t, c->ev[2], c->ev[3]) == hipSuccess) phase_ms[1] = t;
+    (void)hipGetLastError();
+    if (n_hv) *n_hv = c->n_hv;
+    if (n_steps) *n_steps = c->n_steps;
+    return EDG_OK;
+}
+
+}  // extern "C"

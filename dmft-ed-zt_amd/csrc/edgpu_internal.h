@@ -1,0 +1,92 @@
+// Internal declarations shared by the HIP kernels and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace edg {
+
+constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
+constexpr int kMaxHops = 64;      // per-spin hop slots per config (norb*nbath)
+constexpr int kMaxNorb = 5;       // observables accumulate 3*norb sums
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+constexpr uint16_t kNoCol = 0xFFFFu;
+
+// Device arrays describing one spin half (spin sigma, N_sigma electrons) of
+// a sector under one set of Hamiltonian parameters.
+struct SpinTab {
+    int n = 0;               // C(nsite, N)
+    int K = 0;               // ELL width = norb*nbath
+    uint32_t *pattern = nullptr;   // [n]
+    uint16_t *col = nullptr;       // [K][n] target rank
+    float *valf = nullptr;         // [K][n] matrix element (FP32 copy)
+    double *vald = nullptr;        // [K][n] matrix element (FP64)
+    uint8_t *cnt = nullptr;        // [n]
+    double *energy = nullptr;      // [n]
+    uint8_t *mask = nullptr;       // [n] occupied impurity orbitals
+    uint32_t *orb = nullptr;       // [norb*norb][n] orbital moves dst<-src
+};
+
+// Everything the H.v kernel needs besides the vectors.
+struct HvOp {
+    int nup, ndn, norb;
+    SpinTab up, dn;
+    double U, Up, Jp;
+};
+
+enum HvMode { HV_PLAIN = 0, HV_LANCZOS = 1, HV_RQ = 2 };
+
+struct HvArgs {
+    const void *x;        // input vector (T)
+    void *y;              // output vector (T)
+    const void *p;        // previous Lanczos vector (T) or null
+    const double *sx;     // scale of x (Lanczos) or null
+    const double *sp;     // scale of p
+    const double *beta;   // beta_j
+    int accumulate;       // HV_PLAIN: y += Hx
+    double *partial;      // per-block partial sums
+    unsigned *ticket;     // last-block counter (returns to 0)
+    double *result;       // finalized sums
+    const int *stop;      // skip when *stop != 0
+};
+
+// ---- launchers (kernels.hip) ----
+int launch_spin_table(hipStream_t s, int nsite, int norb, int nbath, int ne, const double *em_s,
+                      const double *ek_s, const double *v_s, double mu, double UpmJ,
+                      const SpinTab &t);
+template <typename T>
+int launch_hv(hipStream_t s, const HvOp &op, int mode, int cols, bool acc64, const HvArgs &a);
+template <typename T>
+int launch_norm(hipStream_t s, const T *x, long long n, double *partial, unsigned *ticket,
+                double *norm_out, double *scale_out, const int *stop);
+template <typename T>
+int launch_axpy_norm(hipStream_t s, T *r, const T *u, long long n, const double *alpha,
+                     const double *su, double *partial, unsigned *ticket, double *beta_out,
+                     double *scale_out, int *stop, int *nstep, int step_index);
+template <typename T>
+int launch_start(hipStream_t s, T *x, long long n, unsigned long long seed);
+template <typename T>
+int launch_combine(hipStream_t s, T *out, const T *basis, long long stride, int nvec,
+                   const double *coef, long long n, double *partial, unsigned *ticket,
+                   double *norm_out, double *scale_out);
+template <typename T>
+int launch_scale(hipStream_t s, T *x, long long n, const double *scale);
+template <typename T>
+int launch_from_f64(hipStream_t s, T *dst, const double *src, long long n);
+template <typename T>
+int launch_to_f64(hipStream_t s, double *dst, const T *src, long long n);
+int launch_cmap(hipStream_t s, const uint32_t *pattern_out, int n_out, int nsite, int ne_in,
+                int bit, int pm, uint32_t *map);
+template <typename T>
+int launch_apply_c(hipStream_t s, const T *in, T *out, int nup_out, int ndn_out, int nup_in,
+                   int spin, const uint32_t *map, int global_odd);
+template <typename T>
+int launch_observables(hipStream_t s, const T *gs, const HvOp &op, const double *scale,
+                       double *partial, unsigned *ticket, double *out);
+int launch_cf(hipStream_t s, int nw, const double *omega, double e0, int np, const double *ap,
+              const double *bp, int nm, const double *am, const double *bm, double *g,
+              int accumulate);
+
+// Number of per-block partial slots the reductions may need.
+long long max_blocks_needed(long long n);
+
+}  // namespace edg

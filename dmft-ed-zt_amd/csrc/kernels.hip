@@ -1,0 +1,812 @@
+// HIP kernels of the ED impurity solver for gfx950 (MI355X).
+//
+// Sector vectors are stored as an nup x ndn column-major matrix X (up rank
+// fastest), the row order of ed_basis.f90:132-135.  The Hamiltonian of
+// impurity_hamiltonian.f90:77-247 splits as
+//     H = D(up,dn) + H_up (x) 1 + 1 (x) H_dn + K
+// where H_up / H_dn are the per-spin impurity-bath hoppings (same matrix for
+// every column / row), D is diagonal (one-spin energies plus the U / U'
+// cross terms) and K the Kanamori spin-flip and pair-hopping terms (norb>=2).
+// One workgroup owns C adjacent columns: it stages them in LDS with the C
+// values of a row interleaved, so an up-hop is one wide LDS read serving all
+// C columns, while down-hops read whole neighbour columns from global memory
+// with coalesced wave64 loads.  Reductions are FP64 and deterministic: each
+// block writes a partial, the last block to finish sums them in a fixed order.
+#include "edgpu_internal.h"
+
+namespace edg {
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum NS per-thread values over the block; result valid in thread 0.
+template <int NS>
+__device__ __forceinline__ void block_sums(double (&v)[NS], double (&out)[NS])
+{
+    __shared__ double red[NS][kBlock / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        double t = wave_sum(v[s]);
+        if (lane == 0) red[s][w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < kBlock / 64; ++k) t += red[s][k];
+            out[s] = t;
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ double load_agent(const double *p)
+{
+    unsigned long long b = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)b);
+}
+
+// Publish this block's partial sums; the last block to arrive reduces all
+// partials in a fixed order into tot (thread 0) and returns true.
+template <int NS>
+__device__ bool reduce_last(double (&mine)[NS], double *partial, unsigned *ticket,
+                            double (&tot)[NS])
+{
+    __shared__ int s_last;
+    double blk[NS];
+    block_sums<NS>(mine, blk);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) partial[(long long)blockIdx.x * NS + s] = blk[s];
+        __threadfence();
+        unsigned t = atomicAdd(ticket, 1u);
+        s_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    __threadfence();
+    double v[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) v[s] = 0.0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) v[s] += load_agent(&partial[(long long)b * NS + s]);
+    block_sums<NS>(v, tot);
+    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+__device__ __forceinline__ uint32_t binom_tab(const uint32_t (*B)[16], int n, int k)
+{
+    return (k < 0 || k > n) ? 0u : B[n][k];
+}
+
+// ------------------------------------------------------ per-spin tables
+
+// One thread per configuration rank r of N electrons on nsite sites.
+// Rank order = increasing bit-pattern order (ed_basis.f90:94-112), i.e. the
+// combinatorial number system r = sum_k C(p_k, k) over set bits p_1<..<p_N.
+__global__ __launch_bounds__(kBlock) void k_spin_table(int nsite, int norb, int nbath, int ne,
+                                                       const double *em_s, const double *ek_s,
+                                                       const double *v_s, double mu, double UpmJ,
+                                                       SpinTab t)
+{
+    __shared__ uint32_t B[16][16];
+    for (int e = threadIdx.x; e < 256; e += kBlock) {
+        int n = e >> 4, k = e & 15;
+        uint32_t c = 0;
+        if (k <= n) {
+            c = 1;
+            for (int i = 1; i <= k; ++i) c = c * (uint32_t)(n - k + i) / (uint32_t)i;
+        }
+        B[n][k] = c;
+    }
+    __syncthreads();
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= t.n) return;
+    // unrank
+    uint32_t p = 0, rem = (uint32_t)r;
+    for (int k = ne; k >= 1; --k) {
+        int c = k - 1;
+        while (c + 1 < nsite && B[c + 1][k] <= rem) ++c;
+        p |= 1u << c;
+        rem -= B[c][k];
+    }
+    t.pattern[r] = p;
+    auto rank_of = [&](uint32_t q) {
+        uint32_t s = 0;
+        int k = 0;
+        for (int b = 0; b < nsite; ++b)
+            if ((q >> b) & 1u) { ++k; s += binom_tab(B, b, k); }
+        return s;
+    };
+    // hybridization hops in reference loop order (iorb outer, bath inner)
+    int slot = 0;
+    for (int m = 0; m < norb; ++m) {
+        for (int l = 0; l < nbath; ++l) {
+            const double v = v_s[l * norb + m];
+            if (fabs(v) < 1.0e-16) continue;
+            const int i = m, j = norb + l;
+            const uint32_t bi = (p >> i) & 1u, bj = (p >> j) & 1u;
+            if (bi == bj) continue;
+            const uint32_t q = p ^ ((1u << i) | (1u << j));
+            const uint32_t between = p & ((1u << j) - 1u) & ~((2u << i) - 1u);
+            const double val = (__popc(between) & 1) ? -v : v;
+            t.col[(long long)slot * t.n + r] = (uint16_t)rank_of(q);
+            t.vald[(long long)slot * t.n + r] = val;
+            t.valf[(long long)slot * t.n + r] = (float)val;
+            ++slot;
+        }
+    }
+    t.cnt[r] = (uint8_t)slot;
+    for (int k = slot; k < t.K; ++k) {
+        t.col[(long long)k * t.n + r] = kNoCol;
+        t.vald[(long long)k * t.n + r] = 0.0;
+        t.valf[(long long)k * t.n + r] = 0.0f;
+    }
+    // one-spin diagonal energy
+    double e = 0.0;
+    for (int m = 0; m < norb; ++m)
+        if ((p >> m) & 1u) e += em_s[m] - mu;
+    for (int m = 0; m < norb; ++m)
+        for (int m2 = m + 1; m2 < norb; ++m2)
+            if (((p >> m) & 1u) && ((p >> m2) & 1u)) e += UpmJ;
+    for (int l = 0; l < nbath; ++l)
+        if ((p >> (norb + l)) & 1u) e += ek_s[l];
+    t.energy[r] = e;
+    t.mask[r] = (uint8_t)(p & ((1u << norb) - 1u));
+    // orbital moves dst <- src (Kanamori terms), sign of modes strictly between
+    if (norb >= 2) {
+        for (int dst = 0; dst < norb; ++dst)
+            for (int src = 0; src < norb; ++src) {
+                uint32_t code = kInvalid;
+                if (dst != src && ((p >> src) & 1u) && !((p >> dst) & 1u)) {
+                    const int lo = dst < src ? dst : src, hi = dst < src ? src : dst;
+                    const uint32_t q = p ^ ((1u << src) | (1u << dst));
+                    const uint32_t between = p & ((1u << hi) - 1u) & ~((2u << lo) - 1u);
+                    code = rank_of(q) | ((uint32_t)(__popc(between) & 1) << 31);
+                }
+                t.orb[(long long)(dst * norb + src) * t.n + r] = code;
+            }
+    }
+}
+
+int launch_spin_table(hipStream_t s, int nsite, int norb, int nbath, int ne, const double *em_s,
+                      const double *ek_s, const double *v_s, double mu, double UpmJ,
+                      const SpinTab &t)
+{
+    const int grid = (t.n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_spin_table, dim3(grid), dim3(kBlock), 0, s, nsite, norb, nbath, ne, em_s,
+                       ek_s, v_s, mu, UpmJ, t);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ------------------------------------------------------------------ H.v
+
+template <typename T, int C>
+struct alignas(sizeof(T) * C) Row {
+    T v[C];
+};
+
+template <typename ACC>
+__device__ __forceinline__ ACC hopval(const SpinTab &t, long long e);
+template <>
+__device__ __forceinline__ float hopval<float>(const SpinTab &t, long long e) { return t.valf[e]; }
+template <>
+__device__ __forceinline__ double hopval<double>(const SpinTab &t, long long e) { return t.vald[e]; }
+
+__device__ __forceinline__ double cross_energy(int mu, int md, double U, double Up)
+{
+    const int both = __popc(mu & md);
+    return U * both + Up * (double)(__popc(mu) * __popc(md) - both);
+}
+
+template <typename T, typename ACC, int C, int MODE, bool KAN>
+__global__ __launch_bounds__(kBlock) void k_hv(HvOp op, HvArgs a)
+{
+    if (a.stop && *a.stop) return;
+    extern __shared__ __align__(64) unsigned char smem[];
+    T *xs = reinterpret_cast<T *>(smem);
+    __shared__ int s_cnt[C];
+    __shared__ int s_col[C][kMaxHops];
+    __shared__ ACC s_val[C][kMaxHops];
+    __shared__ double s_ed[C];
+    __shared__ int s_md[C];
+    __shared__ uint32_t s_orb[KAN ? C : 1][KAN ? kMaxNorb * kMaxNorb : 1];
+
+    const int nup = op.nup, ndn = op.ndn;
+    const long long ld = nup;
+    const int idn0 = blockIdx.x * C;
+    const T *x = static_cast<const T *>(a.x);
+
+    for (int e = threadIdx.x; e < C * kMaxHops; e += kBlock) {
+        const int c = e / kMaxHops, k = e - c * kMaxHops;
+        const int idn = idn0 + c;
+        const int cnt = idn < ndn ? op.dn.cnt[idn] : 0;
+        if (k == 0) {
+            s_cnt[c] = cnt;
+            s_ed[c] = idn < ndn ? op.dn.energy[idn] : 0.0;
+            s_md[c] = idn < ndn ? op.dn.mask[idn] : 0;
+        }
+        if (k < cnt) {
+            s_col[c][k] = op.dn.col[(long long)k * ndn + idn];
+            s_val[c][k] = hopval<ACC>(op.dn, (long long)k * ndn + idn);
+        }
+    }
+    if constexpr (KAN) {
+        const int nn = op.norb * op.norb;
+        for (int e = threadIdx.x; e < C * nn; e += kBlock) {
+            const int c = e / nn, q = e - c * nn;
+            const int idn = idn0 + c;
+            s_orb[c][q] = idn < ndn ? op.dn.orb[(long long)q * ndn + idn] : kInvalid;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int idn = idn0 + c;
+        if (idn < ndn) {
+            const T *xc = x + (long long)idn * ld;
+            for (int i = threadIdx.x; i < nup; i += kBlock) xs[i * C + c] = xc[i];
+        } else {
+            for (int i = threadIdx.x; i < nup; i += kBlock) xs[i * C + c] = T(0);
+        }
+    }
+    __syncthreads();
+
+    double sums[3] = {0.0, 0.0, 0.0};
+    double sx = 1.0, bsp = 0.0;
+    if constexpr (MODE == HV_LANCZOS) {
+        sx = *a.sx;
+        if (a.p) bsp = (*a.beta) * (*a.sp);
+    }
+    T *y = static_cast<T *>(a.y);
+    const T *pv = static_cast<const T *>(a.p);
+
+    for (int iup = threadIdx.x; iup < nup; iup += kBlock) {
+        ACC acc[C];
+        const Row<T, C> xr = *reinterpret_cast<const Row<T, C> *>(xs + iup * C);
+        const double eu = op.up.energy[iup];
+        const int mu = op.up.mask[iup];
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+            acc[c] = (ACC)(eu + s_ed[c] + cross_energy(mu, s_md[c], op.U, op.Up)) * (ACC)xr.v[c];
+        const int cu = op.up.cnt[iup];
+        for (int k = 0; k < cu; ++k) {
+            const long long e = (long long)k * nup + iup;
+            const int col = op.up.col[e];
+            const ACC v = hopval<ACC>(op.up, e);
+            const Row<T, C> src = *reinterpret_cast<const Row<T, C> *>(xs + col * C);
+#pragma unroll
+            for (int c = 0; c < C; ++c) acc[c] += v * (ACC)src.v[c];
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int cnt = s_cnt[c];
+            for (int k = 0; k < cnt; ++k)
+                acc[c] += s_val[c][k] * (ACC)x[(long long)s_col[c][k] * ld + iup];
+        }
+        if constexpr (KAN) {
+            const int norb = op.norb;
+            for (int i = 0; i < norb; ++i)
+                for (int j = 0; j < norb; ++j) {
+                    if (i == j) continue;
+                    const uint32_t tu = op.up.orb[(long long)(i * norb + j) * nup + iup];
+                    if (tu == kInvalid) continue;
+                    const double su = (tu >> 31) ? -op.Jp : op.Jp;
+                    const long long ru = tu & 0x7FFFFFFFu;
+#pragma unroll
+                    for (int c = 0; c < C; ++c) {
+                        const uint32_t tsf = s_orb[c][j * norb + i];  // dn i -> j: spin flip
+                        const uint32_t tph = s_orb[c][i * norb + j];  // dn j -> i: pair hop
+                        if (tsf != kInvalid)
+                            acc[c] += (ACC)((tsf >> 31) ? -su : su) *
+                                      (ACC)x[(long long)(tsf & 0x7FFFFFFFu) * ld + ru];
+                        if (tph != kInvalid)
+                            acc[c] += (ACC)((tph >> 31) ? -su : su) *
+                                      (ACC)x[(long long)(tph & 0x7FFFFFFFu) * ld + ru];
+                    }
+                }
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            if (idn0 + c >= ndn) break;
+            const long long e = (long long)(idn0 + c) * ld + iup;
+            if constexpr (MODE == HV_PLAIN) {
+                y[e] = a.accumulate ? (T)((ACC)y[e] + acc[c]) : (T)acc[c];
+            } else if constexpr (MODE == HV_LANCZOS) {
+                double r = sx * (double)acc[c];
+                if (pv) r -= bsp * (double)pv[e];
+                const T rt = (T)r;
+                y[e] = rt;
+                sums[0] += (double)xr.v[c] * (double)rt;
+            } else {
+                const double xv = (double)xr.v[c], hv = (double)acc[c];
+                sums[0] += xv * hv;
+                sums[1] += hv * hv;
+                sums[2] += xv * xv;
+            }
+        }
+    }
+    if constexpr (MODE == HV_LANCZOS) {
+        double mine[1] = {sums[0]}, tot[1];
+        if (reduce_last<1>(mine, a.partial, a.ticket, tot) && threadIdx.x == 0)
+            a.result[0] = sx * tot[0];  // alpha_j = <q_j, r>
+    } else if constexpr (MODE == HV_RQ) {
+        double tot[3];
+        if (reduce_last<3>(sums, a.partial, a.ticket, tot) && threadIdx.x == 0) {
+            a.result[0] = tot[0];
+            a.result[1] = tot[1];
+            a.result[2] = tot[2];
+        }
+    }
+}
+
+template <typename T, typename ACC, int C, bool KAN>
+static int launch_hv_c(hipStream_t s, const HvOp &op, int mode, const HvArgs &a)
+{
+    const size_t lds = (size_t)op.nup * C * sizeof(T);
+    const int grid = (op.ndn + C - 1) / C;
+    void (*f)(HvOp, HvArgs);
+    if (mode == HV_PLAIN) f = k_hv<T, ACC, C, HV_PLAIN, KAN>;
+    else if (mode == HV_LANCZOS) f = k_hv<T, ACC, C, HV_LANCZOS, KAN>;
+    else f = k_hv<T, double, C, HV_RQ, KAN>;
+    if (lds > 64 * 1024) {
+        if (hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds) != hipSuccess)
+            return -1;
+    }
+    hipLaunchKernelGGL(f, dim3(grid), dim3(kBlock), lds, s, op, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename T, typename ACC, bool KAN>
+static int launch_hv_k(hipStream_t s, const HvOp &op, int mode, int cols, const HvArgs &a)
+{
+    switch (cols) {
+    case 1: return launch_hv_c<T, ACC, 1, KAN>(s, op, mode, a);
+    case 2: return launch_hv_c<T, ACC, 2, KAN>(s, op, mode, a);
+    case 4: return launch_hv_c<T, ACC, 4, KAN>(s, op, mode, a);
+    case 8: return launch_hv_c<T, ACC, 8, KAN>(s, op, mode, a);
+    default: return -2;
+    }
+}
+
+// Dynamic LDS budget of one workgroup: gfx950 has 160 KiB per CU; keep 8 KiB
+// for the static per-column hop lists and reduction scratch.
+constexpr size_t kLdsBudget = 152 * 1024;
+
+template <typename T>
+int launch_hv(hipStream_t s, const HvOp &op, int mode, int cols, bool acc64, const HvArgs &a)
+{
+    const bool kan = op.norb >= 2;
+    while (cols > 1 && (size_t)op.nup * cols * sizeof(T) > kLdsBudget) cols >>= 1;
+    if ((size_t)op.nup * cols * sizeof(T) > kLdsBudget) return -3;
+    if (sizeof(T) == 8 || acc64 || mode == HV_RQ)
+        return kan ? launch_hv_k<T, double, true>(s, op, mode, cols, a)
+                   : launch_hv_k<T, double, false>(s, op, mode, cols, a);
+    return kan ? launch_hv_k<T, float, true>(s, op, mode, cols, a)
+               : launch_hv_k<T, float, false>(s, op, mode, cols, a);
+}
+template int launch_hv<float>(hipStream_t, const HvOp &, int, int, bool, const HvArgs &);
+template int launch_hv<double>(hipStream_t, const HvOp &, int, int, bool, const HvArgs &);
+
+// ----------------------------------------------------------- vector ops
+
+long long max_blocks_needed(long long n) { return 2048; }
+
+static inline int grid_for(long long n)
+{
+    long long g = (n + (long long)kBlock * 8 - 1) / ((long long)kBlock * 8);
+    if (g < 1) g = 1;
+    if (g > 2048) g = 2048;
+    return (int)g;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_norm(const T *x, long long n, double *partial,
+                                                 unsigned *ticket, double *norm_out,
+                                                 double *scale_out, const int *stop)
+{
+    if (stop && *stop) return;
+    double v[1] = {0.0}, tot[1];
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (long long)gridDim.x * kBlock) {
+        const double t = (double)x[i];
+        v[0] += t * t;
+    }
+    if (reduce_last<1>(v, partial, ticket, tot) && threadIdx.x == 0) {
+        const double nr = sqrt(tot[0]);
+        if (norm_out) *norm_out = nr;
+        if (scale_out) *scale_out = 1.0 / nr;
+    }
+}
+
+template <typename T>
+int launch_norm(hipStream_t s, const T *x, long long n, double *partial, unsigned *ticket,
+                double *norm_out, double *scale_out, const int *stop)
+{
+    hipLaunchKernelGGL(k_norm<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, x, n, partial, ticket,
+                       norm_out, scale_out, stop);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+template int launch_norm<float>(hipStream_t, const float *, long long, double *, unsigned *,
+                                double *, double *, const int *);
+template int launch_norm<double>(hipStream_t, const double *, long long, double *, unsigned *,
+                                 double *, double *, const int *);
+
+// r <- r - alpha_j * s_j * u_j ; beta_{j+1} = |r| ; scale_{j+1} = 1/beta.
+// A beta below 1e-16 ends the chain with nstep = step_index + 1.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_axpy_norm(T *r, const T *u, long long n,
+                                                      const double *alpha, const double *su,
+                                                      double *partial, unsigned *ticket,
+                                                      double *beta_out, double *scale_out,
+                                                      int *stop, int *nstep, int step_index)
+{
+    if (*stop) return;
+    const double coef = (*alpha) * (*su);
+    double v[1] = {0.0}, tot[1];
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (long long)gridDim.x * kBlock) {
+        const T t = (T)((double)r[i] - coef * (double)u[i]);
+        r[i] = t;
+        v[0] += (double)t * (double)t;
+    }
+    if (reduce_last<1>(v, partial, ticket, tot) && threadIdx.x == 0) {
+        const double b = sqrt(tot[0]);
+        *beta_out = b;
+        *scale_out = b > 0.0 ? 1.0 / b : 0.0;
+        if (b < 1.0e-16) {
+            *stop = 1;
+            *nstep = step_index + 1;
+        }
+    }
+}
+
+template <typename T>
+int launch_axpy_norm(hipStream_t s, T *r, const T *u, long long n, const double *alpha,
+                     const double *su, double *partial, unsigned *ticket, double *beta_out,
+                     double *scale_out, int *stop, int *nstep, int step_index)
+{
+    hipLaunchKernelGGL(k_axpy_norm<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, r, u, n, alpha, su,
+                       partial, ticket, beta_out, scale_out, stop, nstep, step_index);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+template int launch_axpy_norm<float>(hipStream_t, float *, const float *, long long,
+                                     const double *, const double *, double *, unsigned *,
+                                     double *, double *, int *, int *, int);
+template int launch_axpy_norm<double>(hipStream_t, double *, const double *, long long,
+                                      const double *, const double *, double *, unsigned *,
+                                      double *, double *, int *, int *, int);
+
+// Counter-based U[0,1): splitmix64(seed * K + i) >> 11 scaled by 2^-53.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_start(T *x, long long n, unsigned long long seed)
+{
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (long long)gridDim.x * kBlock)
+        x[i] = (T)((double)(splitmix64(seed * 0xD1B54A32D192ED03ull + (uint64_t)i) >> 11) *
+                   0x1.0p-53);
+}
+
+template <typename T>
+int launch_start(hipStream_t s, T *x, long long n, unsigned long long seed)
+{
+    hipLaunchKernelGGL(k_start<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, x, n, seed);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+template int launch_start<float>(hipStream_t, float *, long long, unsigned long long);
+template int launch_start<double>(hipStream_t, double *, long long, unsigned long long);
+
+// out = sum_j coef_j * basis_j (FP64 accumulation), plus |out|^2.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_combine(T *out, const T *basis, long long stride,
+                                                    int nvec, const double *coef, long long n,
+                                                    double *partial, unsigned *ticket,
+                                                    double *norm_out, double *scale_out)
+{
+    __shared__ double c[1024];
+    for (int j = threadIdx.x; j < nvec && j < 1024; j += kBlock) c[j] = coef[j];
+    __syncthreads();
+    double v[1] = {0.0}, tot[1];
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (long long)gridDim.x * kBlock) {
+        double acc = 0.0;
+        for (int j = 0; j < nvec; ++j) acc += c[j] * (double)basis[(long long)j * stride + i];
+        const T t = (T)acc;
+        out[i] = t;
+        v[0] += (double)t * (double)t;
+    }
+    if (reduce_last<1>(v, partial, ticket, tot) && threadIdx.x == 0) {
+        const double nr = sqrt(tot[0]);
+        *norm_out = nr;
+        *scale_out = 1.0 / nr;
+    }
+}
+
+template <typename T>
+int launch_combine(hipStream_t s, T *out, const T *basis, long long stride, int nvec,
+                   const double *coef, long long n, double *partial, unsigned *ticket,
+                   double *norm_out, double *scale_out)
+{
+    if (nvec > 1024) return -2;
+    hipLaunchKernelGGL(k_combine<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, out, basis, stride,
+                       nvec, coef, n, partial, ticket, norm_out, scale_out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+template int launch_combine<float>(hipStream_t, float *, const float *, long long, int,
+                                   const double *, long long, double *, unsigned *, double *,
+                                   double *);
+template int launch_combine<double>(hipStream_t, double *, const double *, long long, int,
+                                    const double *, long long, double *, unsigned *, double *,
+                                    double *);
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scale(T *x, long long n, const double *scale)
+{
+    const double sc = *scale;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (long long)gridDim.x * kBlock)
+        x[i] = (T)((double)x[i] * sc);
+}
+template <typename T>
+int launch_scale(hipStream_t s, T *x, long long n, const double *scale)
+{
+    hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, x, n, scale);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+template int launch_scale<float>(hipStream_t, float *, long long, const double *);
+template int launch_scale<double>(hipStream_t, double *, long long, const double *);
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_from_f64(T *dst, const double *src, long long n)
+{
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (long long)gridDim.x * kBlock)
+        dst[i] = (T)src[i];
+}
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_to_f64(double *dst, const T *src, long long n)
+{
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (long long)gridDim.x * kBlock)
+        dst[i] = (double)src[i];
+}
+template <typename T>
+int launch_from_f64(hipStream_t s, T *dst, const double *src, long long n)
+{
+    hipLaunchKernelGGL(k_from_f64<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+template <typename T>
+int launch_to_f64(hipStream_t s, double *dst, const T *src, long long n)
+{
+    hipLaunchKernelGGL(k_to_f64<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+template int launch_from_f64<float>(hipStream_t, float *, const double *, long long);
+template int launch_from_f64<double>(hipStream_t, double *, const double *, long long);
+template int launch_to_f64<float>(hipStream_t, double *, const float *, long long);
+template int launch_to_f64<double>(hipStream_t, double *, const double *, long long);
+
+// ------------------------------------------------------------ apply_c
+
+// map[r'] for output configs r' of the changed spin (ed_operator.f90:31-54):
+// c+ (pm=1) needs bit set in the output pattern, source = bit cleared;
+// c (pm=2) needs bit clear, source = bit set.  Sign = parity of the output
+// pattern's bits below `bit` (same-spin part of the reference's sgn).
+__global__ __launch_bounds__(kBlock) void k_cmap(const uint32_t *pattern_out, int n_out,
+                                                 int nsite, int ne_in, int bit, int pm,
+                                                 uint32_t *map)
+{
+    __shared__ uint32_t B[16][16];
+    for (int e = threadIdx.x; e < 256; e += kBlock) {
+        int n = e >> 4, k = e & 15;
+        uint32_t c = 0;
+        if (k <= n) {
+            c = 1;
+            for (int i = 1; i <= k; ++i) c = c * (uint32_t)(n - k + i) / (uint32_t)i;
+        }
+        B[n][k] = c;
+    }
+    __syncthreads();
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n_out) return;
+    const uint32_t p = pattern_out[r];
+    const uint32_t set = (p >> bit) & 1u;
+    if ((pm == 1 && !set) || (pm == 2 && set)) { map[r] = kInvalid; return; }
+    const uint32_t q = p ^ (1u << bit);
+    uint32_t s = 0;
+    int k = 0;
+    for (int b = 0; b < nsite; ++b)
+        if ((q >> b) & 1u) { ++k; s += binom_tab(B, b, k); }
+    (void)ne_in;
+    const uint32_t odd = __popc(p & ((1u << bit) - 1u)) & 1u;
+    map[r] = s | (odd << 31);
+}
+
+int launch_cmap(hipStream_t s, const uint32_t *pattern_out, int n_out, int nsite, int ne_in,
+                int bit, int pm, uint32_t *map)
+{
+    hipLaunchKernelGGL(k_cmap, dim3((n_out + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                       pattern_out, n_out, nsite, ne_in, bit, pm, map);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// spin 1: out(iup', idn) = sgn * in(map(iup'), idn)
+// spin 2: out(iup, idn') = sgn * (-1)^{N_up} * in(iup, map(idn'))
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_apply_c(const T *in, T *out, int nup_out,
+                                                    int ndn_out, int nup_in, int spin,
+                                                    const uint32_t *map, int global_odd)
+{
+    const long long n = (long long)nup_out * ndn_out;
+    for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < n;
+         e += (long long)gridDim.x * kBlock) {
+        const int idn = (int)(e / nup_out);
+        const int iup = (int)(e - (long long)idn * nup_out);
+        T v = T(0);
+        if (spin == 1) {
+            const uint32_t m = map[iup];
+            if (m != kInvalid) {
+                const T t = in[(long long)idn * nup_in + (m & 0x7FFFFFFFu)];
+                v = (m >> 31) ? -t : t;
+            }
+        } else {
+            const uint32_t m = map[idn];
+            if (m != kInvalid) {
+                const T t = in[(long long)(m & 0x7FFFFFFFu) * nup_in + iup];
+                v = (((m >> 31) ^ (uint32_t)global_odd) & 1u) ? -t : t;
+            }
+        }
+        out[e] = v;
+    }
+}
+template <typename T>
+int launch_apply_c(hipStream_t s, const T *in, T *out, int nup_out, int ndn_out, int nup_in,
+                   int spin, const uint32_t *map, int global_odd)
+{
+    const long long n = (long long)nup_out * ndn_out;
+    hipLaunchKernelGGL(k_apply_c<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, in, out, nup_out,
+                       ndn_out, nup_in, spin, map, global_odd);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+template int launch_apply_c<float>(hipStream_t, const float *, float *, int, int, int, int,
+                                   const uint32_t *, int);
+template int launch_apply_c<double>(hipStream_t, const double *, double *, int, int, int, int,
+                                    const uint32_t *, int);
+
+// -------------------------------------------------------- observables
+
+// post_process.f90:63-112: out[m] = n(m,up), out[norb+m] = n(m,dn),
+// out[2 norb + m] = docc(m), with gs scaled by *scale.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_observables(const T *gs, HvOp op,
+                                                        const double *scale, double *partial,
+                                                        unsigned *ticket, double *out)
+{
+    constexpr int NS = 3 * kMaxNorb;
+    double v[NS], tot[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) v[s] = 0.0;
+    const double sc = scale ? *scale : 1.0;
+    const long long n = (long long)op.nup * op.ndn;
+    const int norb = op.norb;
+    for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < n;
+         e += (long long)gridDim.x * kBlock) {
+        const int idn = (int)(e / op.nup);
+        const int iup = (int)(e - (long long)idn * op.nup);
+        const double g = (double)gs[e] * sc;
+        const double w = g * g;
+        const int mu = op.up.mask[iup], md = op.dn.mask[idn];
+#pragma unroll
+        for (int m = 0; m < kMaxNorb; ++m) {
+            if (m < norb) {
+                if ((mu >> m) & 1) v[m] += w;
+                if ((md >> m) & 1) v[kMaxNorb + m] += w;
+                if ((mu >> m) & (md >> m) & 1) v[2 * kMaxNorb + m] += w;
+            }
+        }
+    }
+    if (reduce_last<NS>(v, partial, ticket, tot) && threadIdx.x == 0) {
+        for (int m = 0; m < norb; ++m) {
+            out[m] = tot[m];
+            out[norb + m] = tot[kMaxNorb + m];
+            out[2 * norb + m] = tot[2 * kMaxNorb + m];
+        }
+    }
+}
+template <typename T>
+int launch_observables(hipStream_t s, const T *gs, const HvOp &op, const double *scale,
+                       double *partial, unsigned *ticket, double *out)
+{
+    const long long n = (long long)op.nup * op.ndn;
+    hipLaunchKernelGGL(k_observables<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, gs, op, scale,
+                       partial, ticket, out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+template int launch_observables<float>(hipStream_t, const float *, const HvOp &, const double *,
+                                       double *, unsigned *, double *);
+template int launch_observables<double>(hipStream_t, const double *, const HvOp &,
+                                        const double *, double *, unsigned *, double *);
+
+// --------------------------------------------------- continued fraction
+
+struct cplx {
+    double re, im;
+};
+__device__ __forceinline__ cplx cdiv(double num, cplx d)
+{
+    // num / d for real num, Smith's scaling
+    if (fabs(d.re) >= fabs(d.im)) {
+        const double r = d.im / d.re, den = d.re + d.im * r;
+        return {num / den, -num * r / den};
+    }
+    const double r = d.re / d.im, den = d.re * r + d.im;
+    return {num * r / den, -num / den};
+}
+
+// numeric_utils.f90:11-44: f = b1^2/(z -+ a1 - b2^2/(z -+ a2 - ...)),
+// evaluated bottom-up; the last level is applied after the loop even for n=1.
+__device__ cplx cfrac(cplx z, int n, const double *a, const double *b, double sgn)
+{
+    cplx f = cdiv(b[n - 1] * b[n - 1], {z.re + sgn * a[n - 1], z.im});
+    for (int i = n - 2; i >= 1; --i) f = cdiv(b[i] * b[i], {z.re + sgn * a[i] - f.re, z.im - f.im});
+    return cdiv(b[0] * b[0], {z.re + sgn * a[0] - f.re, z.im - f.im});
+}
+
+// g[iw] (+)= CF_p(E0 + i w; ap, bp) + CF_m(-E0 + i w; am, bm)
+__global__ void k_cf(int nw, const double *omega, double e0, int np, const double *ap,
+                     const double *bp, int nm, const double *am, const double *bm, double *g,
+                     int accumulate)
+{
+    const int iw = blockIdx.x * blockDim.x + threadIdx.x;
+    if (iw >= nw) return;
+    double re = 0.0, im = 0.0;
+    if (np > 0) {
+        const cplx f = cfrac({e0, omega[iw]}, np, ap, bp, -1.0);
+        re += f.re;
+        im += f.im;
+    }
+    if (nm > 0) {
+        const cplx f = cfrac({-e0, omega[iw]}, nm, am, bm, 1.0);
+        re += f.re;
+        im += f.im;
+    }
+    if (accumulate) {
+        g[2 * iw] += re;
+        g[2 * iw + 1] += im;
+    } else {
+        g[2 * iw] = re;
+        g[2 * iw + 1] = im;
+    }
+}
+
+int launch_cf(hipStream_t s, int nw, const double *omega, double e0, int np, const double *ap,
+              const double *bp, int nm, const double *am, const double *bm, double *g,
+              int accumulate)
+{
+    hipLaunchKernelGGL(k_cf, dim3((nw + 127) / 128), dim3(128), 0, s, nw, omega, e0, np, ap, bp,
+                       nm, am, bm, g, accumulate);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace edg

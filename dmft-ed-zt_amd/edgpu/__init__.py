@@ -1,0 +1,300 @@
+"""edgpu -- host-side mirror of the reference's ED solver interface over the
+libedgpu.so C ABI (include/edgpu.h).
+
+The names, argument meaning and array shapes follow the reference routines:
+  hx                (impurity_hamiltonian.f90:77)
+  lanczos_iteration (lanczos_solver.f90:92)
+  ground_state      (ground_state.f90:1)
+  cluster_green_ftn (dmft_green.f90:47)
+  apply_c           (ed_operator.f90:13)
+  observables       (post_process.f90:63-112)
+with em(norb,2), ek(nbath,2), vmk(norb,nbath,2) FP64 arrays.  Everything is
+computed by the HIP kernels of libedgpu.so on a GPU; there is no CPU
+fallback: a missing library or device raises EdgpuError.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libedgpu.so")
+
+EDG_FP64, EDG_FP32 = 0, 1
+STATUS = {0: "ok", 2: "admission refused", 3: "bad argument", 4: "eigensolver failed",
+          5: "bad state", -1: "HIP error", -2: "out of device memory"}
+
+
+class EdgpuError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"edgpu status {code} ({STATUS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class _Params(C.Structure):
+    _fields_ = [("norb", C.c_int), ("nbath", C.c_int), ("nspin", C.c_int),
+                ("U", C.c_double), ("J", C.c_double), ("Up", C.c_double), ("Jp", C.c_double),
+                ("mu", C.c_double), ("maxnstep", C.c_int), ("maxnstep_green", C.c_int),
+                ("diag_solver", C.c_int),
+                ("precision", C.c_int), ("both_spins", C.c_int), ("seed", C.c_ulonglong)]
+
+
+EXPORTS = ["edg_default_params", "edg_create", "edg_destroy", "edg_last_error",
+           "edg_set_memory_budget", "edg_device_count", "edg_set_tuning", "edg_spin_table",
+           "edg_hx", "edg_lanczos_iteration", "edg_ground_state", "edg_gs_info",
+           "edg_get_gs_vector", "edg_set_gs", "edg_cluster_green", "edg_observables",
+           "edg_apply_c", "edg_time_hv", "edg_last_solve_stats"]
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """dlopen libedgpu.so and declare the C signatures (no device work)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise EdgpuError(-1, f"{path} not built: run `python -m edgpu.build` in dmft-ed-zt_amd/")
+    L = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    ctx = C.c_void_p
+    L.edg_default_params.argtypes = [C.POINTER(_Params)]
+    L.edg_default_params.restype = None
+    L.edg_create.argtypes = [C.POINTER(_Params), C.c_int, C.POINTER(C.c_void_p)]
+    L.edg_destroy.argtypes = [ctx]
+    L.edg_last_error.argtypes = [ctx]
+    L.edg_last_error.restype = C.c_char_p
+    L.edg_set_memory_budget.argtypes = [ctx, C.c_ulonglong]
+    L.edg_device_count.argtypes = [ip]
+    L.edg_set_tuning.argtypes = [ctx, C.c_int, C.c_int]
+    L.edg_spin_table.argtypes = [ctx, dp, dp, dp, C.c_int, C.c_int, ip, ip,
+                                 C.POINTER(C.c_uint), ip, dp, ip, dp]
+    L.edg_hx.argtypes = [ctx, dp, dp, dp, C.c_int, C.c_int, dp, dp, C.c_int]
+    L.edg_lanczos_iteration.argtypes = [ctx, dp, dp, dp, C.c_int, C.c_int, dp, ip, dp, dp]
+    L.edg_ground_state.argtypes = [ctx, dp, dp, dp, C.c_int, ip, ip, dp]
+    L.edg_gs_info.argtypes = [ctx, ip, ip, C.POINTER(C.c_longlong), dp, dp, dp, ip]
+    L.edg_get_gs_vector.argtypes = [ctx, dp, C.c_longlong]
+    L.edg_set_gs.argtypes = [ctx, C.c_int, C.c_int, C.c_double, dp, C.c_longlong]
+    L.edg_cluster_green.argtypes = [ctx, dp, dp, dp, C.c_int, dp, dp, dp]
+    L.edg_observables.argtypes = [ctx, dp, dp]
+    L.edg_apply_c.argtypes = [ctx, C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_int, dp]
+    L.edg_time_hv.argtypes = [ctx, dp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
+    L.edg_last_solve_stats.argtypes = [ctx, dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    for name in EXPORTS:
+        if name not in ("edg_default_params", "edg_last_error"):
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _fortran(a, shape):
+    a = np.asarray(a, dtype=np.float64)
+    if a.shape != shape:
+        raise EdgpuError(3, f"expected shape {shape}, got {a.shape}")
+    return np.ascontiguousarray(a.reshape(-1, order="F"))
+
+
+def device_count():
+    L = load_library()
+    n = C.c_int(0)
+    L.edg_device_count(C.byref(n))
+    return n.value
+
+
+def binom(n, k):
+    from math import comb
+    return comb(n, k) if 0 <= k <= n else 0
+
+
+class EigPair:
+    """eigpair_t (eigpair.f90:5-9): isector (1-based), val (E0), device-held vec."""
+
+    def __init__(self, solver, isector, val):
+        self._s, self.isector, self.val = solver, isector, val
+
+    @property
+    def vec(self):
+        return self._s.gs_vector()
+
+
+class Solver:
+    """One solver context on one GPU (edg_create).  Parameters mirror the
+    reference's module globals (dmft_params.f90:78-188)."""
+
+    def __init__(self, norb=1, nbath=5, U=1.0, J=0.3, Up=-1.0, Jp=-1.0, mu=0.5, maxnstep=80,
+                 nspin=1, precision="fp64", both_spins=False, seed=1, device=0, diag_solver=1,
+                 maxnstep_green=0):
+        L = load_library()
+        p = _Params()
+        L.edg_default_params(C.byref(p))
+        p.norb, p.nbath, p.nspin = norb, nbath, nspin
+        p.U, p.J, p.Up, p.Jp, p.mu = U, J, Up, Jp, mu
+        p.maxnstep, p.diag_solver = maxnstep, diag_solver
+        p.maxnstep_green = int(maxnstep_green)
+        p.precision = {"fp64": EDG_FP64, "fp32": EDG_FP32}[precision]
+        p.both_spins, p.seed = int(bool(both_spins)), int(seed)
+        h = C.c_void_p()
+        rc = L.edg_create(C.byref(p), int(device), C.byref(h))
+        if rc != 0:
+            raise EdgpuError(rc, "edg_create failed (no GPU, or invalid parameters)")
+        self._L, self._h = L, h
+        self.norb, self.nbath, self.nsite, self.nspin = norb, nbath, norb + nbath, nspin
+        self.maxnstep, self.precision = maxnstep, precision
+        self.maxnstep_green = maxnstep_green if maxnstep_green > 0 else maxnstep
+        self.U, self.J, self.mu = U, J, mu
+        self.Up = Up if Up >= 0 else U - 2 * J
+        self.Jp = Jp if Jp >= 0 else J
+
+    # ---- plumbing
+    def _chk(self, rc):
+        if rc != 0:
+            raise EdgpuError(rc, self._L.edg_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.edg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _params(self, em, ek, vmk):
+        return (_fortran(em, (self.norb, 2)), _fortran(ek, (self.nbath, 2)),
+                _fortran(vmk, (self.norb, self.nbath, 2)))
+
+    def dim(self, ne_up, ne_dn):
+        return binom(self.nsite, ne_up) * binom(self.nsite, ne_dn)
+
+    def set_memory_budget(self, nbytes):
+        self._chk(self._L.edg_set_memory_budget(self._h, int(nbytes)))
+
+    def set_tuning(self, cols_per_block, fp64_accumulate=False):
+        self._chk(self._L.edg_set_tuning(self._h, int(cols_per_block), int(bool(fp64_accumulate))))
+
+    # ---- reference interface
+    def spin_table(self, em, ek, vmk, ispin, ne):
+        """Per-spin basis + hop tables built on the device (ed_basis.f90:42-114)."""
+        em, ek, vmk = self._params(em, ek, vmk)
+        n = binom(self.nsite, ne)
+        K = self.norb * self.nbath
+        pat = np.zeros(n, dtype=np.uint32)
+        col = np.zeros(max(K * n, 1), dtype=np.int32)
+        val = np.zeros(max(K * n, 1))
+        cnt = np.zeros(n, dtype=np.int32)
+        en = np.zeros(n)
+        nn, kk = C.c_int(0), C.c_int(0)
+        self._chk(self._L.edg_spin_table(self._h, _d(em), _d(ek), _d(vmk), ispin, ne, C.byref(nn),
+                                         C.byref(kk), pat.ctypes.data_as(C.POINTER(C.c_uint)),
+                                         col.ctypes.data_as(C.POINTER(C.c_int)), _d(val),
+                                         cnt.ctypes.data_as(C.POINTER(C.c_int)), _d(en)))
+        return dict(pattern=pat, col=col[:K * n].reshape(K, n), val=val[:K * n].reshape(K, n),
+                    cnt=cnt, energy=en)
+
+    def hx(self, em, ek, vmk, ne_up, ne_dn, x, y=None, py=False):
+        """y = H x or y += H x (impurity_hamiltonian.f90:77-247)."""
+        em, ek, vmk = self._params(em, ek, vmk)
+        D = self.dim(ne_up, ne_dn)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.size != D:
+            raise EdgpuError(3, "x has wrong length")
+        y = np.zeros(D) if y is None else np.array(y, dtype=np.float64, copy=True)
+        self._chk(self._L.edg_hx(self._h, _d(em), _d(ek), _d(vmk), ne_up, ne_dn, _d(x), _d(y),
+                                 int(bool(py))))
+        return y
+
+    def lanczos_iteration(self, em, ek, vmk, ne_up, ne_dn, v_init):
+        """(nstep, a, b) of lanczos_solver.f90:92-173."""
+        em, ek, vmk = self._params(em, ek, vmk)
+        v = np.ascontiguousarray(v_init, dtype=np.float64)
+        if v.size != self.dim(ne_up, ne_dn):
+            raise EdgpuError(3, "v_init has wrong length")
+        a = np.zeros(self.maxnstep)
+        b = np.zeros(self.maxnstep)
+        n = C.c_int(0)
+        self._chk(self._L.edg_lanczos_iteration(self._h, _d(em), _d(ek), _d(vmk), ne_up, ne_dn,
+                                                _d(v), C.byref(n), _d(a), _d(b)))
+        return n.value, a, b
+
+    def ground_state(self, em, ek, vmk, sectors):
+        """ground_state.f90:1-81 over sectors [(ne_up, ne_dn), ...] -> EigPair."""
+        em, ek, vmk = self._params(em, ek, vmk)
+        sec = np.ascontiguousarray(np.asarray(sectors, dtype=np.int32).reshape(-1, 2))
+        isec, e0 = C.c_int(0), C.c_double(0)
+        self._chk(self._L.edg_ground_state(self._h, _d(em), _d(ek), _d(vmk), len(sec),
+                                           sec.ctypes.data_as(C.POINTER(C.c_int)),
+                                           C.byref(isec), C.byref(e0)))
+        return EigPair(self, isec.value, e0.value)
+
+    def gs_info(self):
+        nu, nd, ns = C.c_int(0), C.c_int(0), C.c_int(0)
+        dim = C.c_longlong(0)
+        er, e0, res = C.c_double(0), C.c_double(0), C.c_double(0)
+        self._chk(self._L.edg_gs_info(self._h, C.byref(nu), C.byref(nd), C.byref(dim),
+                                      C.byref(er), C.byref(e0), C.byref(res), C.byref(ns)))
+        return dict(ne_up=nu.value, ne_dn=nd.value, dim=dim.value, e0_ritz=er.value, e0=e0.value,
+                    residual=res.value, nstep=ns.value)
+
+    def gs_vector(self):
+        info = self.gs_info()
+        v = np.zeros(info["dim"])
+        self._chk(self._L.edg_get_gs_vector(self._h, _d(v), info["dim"]))
+        return v
+
+    def set_gs(self, ne_up, ne_dn, e0, vec):
+        v = np.ascontiguousarray(vec, dtype=np.float64)
+        self._chk(self._L.edg_set_gs(self._h, ne_up, ne_dn, float(e0), _d(v), v.size))
+
+    def cluster_green_ftn(self, em, ek, vmk, omega):
+        """G_cl(nw,norb,nspin) of dmft_green.f90:47-126 plus the g_coeffs.dat
+        coefficients coeffs[(iorb, ispin)] = (ap, bp, am, bm)."""
+        em, ek, vmk = self._params(em, ek, vmk)
+        om = np.ascontiguousarray(omega, dtype=np.float64)
+        nw = om.size
+        G = np.zeros(2 * nw * self.norb * self.nspin)
+        ng = self.maxnstep_green
+        co = np.zeros(ng * 4 * 2 * self.norb)
+        self._chk(self._L.edg_cluster_green(self._h, _d(em), _d(ek), _d(vmk), nw, _d(om), _d(G),
+                                            _d(co)))
+        Gc = (G[0::2] + 1j * G[1::2]).reshape(nw, self.norb, self.nspin, order="F")
+        co = co.reshape(ng, 4, 2, self.norb, order="F")
+        coeffs = {(i + 1, s + 1): tuple(co[:, q, s, i].copy() for q in range(4))
+                  for i in range(self.norb) for s in range(2)}
+        return Gc, coeffs
+
+    def observables(self):
+        """occ(norb,2), docc(norb) of the held ground state (post_process.f90:63-112)."""
+        occ = np.zeros(2 * self.norb)
+        docc = np.zeros(self.norb)
+        self._chk(self._L.edg_observables(self._h, _d(occ), _d(docc)))
+        return occ.reshape(self.norb, 2, order="F"), docc
+
+    def apply_c(self, ne_up, ne_dn, vec, pm, iorb, ispin):
+        """ed_operator.f90:13-57 apply_c (pm=1 create, pm=2 destroy)."""
+        nu2 = ne_up + (0 if ispin == 2 else (1 if pm == 1 else -1))
+        nd2 = ne_dn + (0 if ispin == 1 else (1 if pm == 1 else -1))
+        v = np.ascontiguousarray(vec, dtype=np.float64)
+        out = np.zeros(self.dim(nu2, nd2))
+        self._chk(self._L.edg_apply_c(self._h, ne_up, ne_dn, _d(v), pm, iorb, ispin, _d(out)))
+        return out
+
+    def time_hv(self, em, ek, vmk, ne_up, ne_dn, warmup=5, reps=50):
+        """(ms per H.v launch, algorithmic bytes per launch) from HIP events."""
+        em, ek, vmk = self._params(em, ek, vmk)
+        ms, by = C.c_double(0), C.c_double(0)
+        self._chk(self._L.edg_time_hv(self._h, _d(em), _d(ek), _d(vmk), ne_up, ne_dn, warmup,
+                                      reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+    def last_solve_stats(self):
+        ph = (C.c_double * 2)()
+        nh, ns = C.c_longlong(0), C.c_longlong(0)
+        self._chk(self._L.edg_last_solve_stats(self._h, ph, C.byref(nh), C.byref(ns)))
+        return dict(gs_ms=ph[0], green_ms=ph[1], n_hv=nh.value, n_steps=ns.value)

@@ -1,0 +1,165 @@
+/*
+ * edgpu.h -- C ABI of the MI355X exact-diagonalization impurity solver.
+ *
+ * Plain extern "C" entry points with pointers and sizes only.  A Fortran
+ * caller binds them through iso_c_binding; the Fortran module in
+ * dmft-ed-zt_amd/fortran/edgpu_mod.f90 re-exposes them with the reference's
+ * argument lists.  Each entry point names the reference routine it replaces
+ * (paths relative to the yw-choi/DMFT-ED-ZT tree).
+ *
+ * Array conventions (identical to the reference):
+ *   em(norb,2), ek(nbath,2), vmk(norb,nbath,2): FP64, column-major;
+ *   sector vectors: FP64, length C(nsite,N_up)*C(nsite,N_dn), row index
+ *   (rank_dn-1)*nup + rank_up (up rank fastest), ranks in increasing
+ *   bit-pattern order (ed_basis.f90:42-151);
+ *   complex arrays: interleaved (re, im) FP64, Fortran double complex layout.
+ *
+ * Threading: one context per host thread; calls on a context are serialized
+ * on its HIP stream.  Every call returns an edg_status (0 = ok).
+ */
+#ifndef EDGPU_H
+#define EDGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    EDG_OK = 0,
+    EDG_ERR_ADMISSION = 2, /* request exceeds the device-memory budget        */
+    EDG_ERR_ARG = 3,       /* bad sector, size or argument                     */
+    EDG_ERR_SOLVER = 4,    /* tridiagonal eigensolve failed (dstevx info /= 0) */
+    EDG_ERR_STATE = 5,     /* no ground state held / wrong call order          */
+    EDG_ERR_HIP = -1,      /* HIP runtime or launch error                      */
+    EDG_ERR_NOMEM = -2     /* device allocation failed                         */
+} edg_status;
+
+typedef enum { EDG_FP64 = 0, EDG_FP32 = 1 } edg_precision;
+
+/* Module globals of the reference made explicit (dmft_params.f90:78-188,
+ * impurity_hamiltonian.f90:4-6, lanczos_solver.f90:4, ground_state.f90:7-8,
+ * dmft_green.f90:5-6). */
+typedef struct edg_params {
+    int norb;          /* Norb                                                */
+    int nbath;         /* Nbath  (nsite = norb + nbath <= 15)                 */
+    int nspin;         /* Nspin (1 or 2)                                      */
+    double U;          /* U                                                   */
+    double J;          /* J (Jex)                                             */
+    double Up;         /* Up; < 0 -> U - 2J  (dmft_params.f90:90-92)          */
+    double Jp;         /* Jp; < 0 -> J       (dmft_params.f90:94-96)          */
+    double mu;         /* Mu, on the impurity only                            */
+    int maxnstep;      /* LanczosSteps                                        */
+    int maxnstep_green;/* steps of each Green's-function chain; 0 = maxnstep
+                          (the reference uses LanczosSteps for both)         */
+    int diag_solver;   /* DiagSolver: 1 = Lanczos; 2 (ARPACK) -> EDG_ERR_ARG  */
+    int precision;     /* edg_precision of the stored Lanczos vectors         */
+    int both_spins;    /* 1: also run spin-down chains when spin-symmetric    */
+    unsigned long long seed; /* start-vector seed (counter-based U[0,1))      */
+} edg_params;
+
+typedef struct edg_ctx edg_ctx;
+
+/* Fill p with the reference's defaults (dmft_params.f90:78-188). */
+void edg_default_params(edg_params *p);
+
+/* Create a solver context on HIP device `device`. */
+int edg_create(const edg_params *p, int device, edg_ctx **out);
+int edg_destroy(edg_ctx *ctx);
+/* Human-readable message of the last failing call on ctx (never NULL). */
+const char *edg_last_error(const edg_ctx *ctx);
+/* Device-memory budget for admission (bytes); 0 = no limit. */
+int edg_set_memory_budget(edg_ctx *ctx, unsigned long long bytes);
+/* Number of HIP devices visible to the process (no compute). */
+int edg_device_count(int *n);
+/* Kernel schedule: sector columns per workgroup of the H.v kernel (1, 2, 4
+ * or 8) and FP64 accumulation of the row sums (1) or accumulation in the
+ * storage precision (0).  No counterpart in the reference (its hx is one
+ * scalar loop); set by the autotuner or a caller that has measured. */
+int edg_set_tuning(edg_ctx *ctx, int cols_per_block, int fp64_accumulate);
+
+/* Per-spin basis and hop tables of one (spin, N) sector half, built on the
+ * device: ed_basis.f90:42-114 (generate_basis), the hybridization part of
+ * impurity_hamiltonian.f90:146-174 with signs of :251-260, and the one-spin
+ * diagonal of :119-141,232-239.  Out arrays (n = C(nsite,ne) entries,
+ * kmax = norb*nbath hop slots, slot-major [k*n + r]):
+ *   pattern[n]        bit pattern of rank r (0-based, increasing order)
+ *   hop_col[kmax*n]   0-based target rank of hop k of rank r (-1 if unused)
+ *   hop_val[kmax*n]   matrix element <r|H|target> (sign * V)
+ *   hop_cnt[n]        number of used hop slots
+ *   energy[n]         one-spin diagonal energy
+ * Pass NULL for arrays not wanted; *n and *kmax are always filled. */
+int edg_spin_table(edg_ctx *ctx, const double *em, const double *ek, const double *vmk,
+                   int ispin, int ne, int *n, int *kmax, unsigned *pattern, int *hop_col,
+                   double *hop_val, int *hop_cnt, double *energy);
+
+/* hx (impurity_hamiltonian.f90:77-247): y = H x (py = 0) or y += H x
+ * (py = 1) on sector (ne_up, ne_dn).  Host FP64 vectors; computed on the
+ * device in the context's precision. */
+int edg_hx(edg_ctx *ctx, const double *em, const double *ek, const double *vmk, int ne_up,
+           int ne_dn, const double *x, double *y, int py);
+
+/* lanczos_iteration (lanczos_solver.f90:92-173): plain 3-term Lanczos from
+ * an unnormalized v_init for maxnstep steps.  a, b have maxnstep entries,
+ * zero beyond *nstep; b[0] = 0.  Stops early when b(k) < 1e-16. */
+int edg_lanczos_iteration(edg_ctx *ctx, const double *em, const double *ek, const double *vmk,
+                          int ne_up, int ne_dn, const double *v_init, int *nstep, double *a,
+                          double *b);
+
+/* ground_state (ground_state.f90:1-81) with diag_lanczos
+ * (lanczos_solver.f90:20-90): sweep `nsector` sectors given as
+ * sectors[2*i] = ne_up, sectors[2*i+1] = ne_dn, keep the strict minimum.
+ * The ground-state vector (gs%vec) stays on the device in ctx.
+ * Outputs: *isector (1-based), *e0.  Sector i starts from the seeded
+ * vector of seed + i - 1. */
+int edg_ground_state(edg_ctx *ctx, const double *em, const double *ek, const double *vmk,
+                     int nsector, const int *sectors, int *isector, double *e0);
+
+/* Details of the held ground state: sector, dimension, Ritz value of the
+ * tridiagonal (the reference's E0), Rayleigh quotient <gs|H|gs> accumulated
+ * in FP64 (reported E0), residual |H gs - E0 gs|^2 (lanczos_solver.f90:293-297)
+ * and Lanczos step count. */
+int edg_gs_info(edg_ctx *ctx, int *ne_up, int *ne_dn, long long *dim, double *e0_ritz,
+                double *e0, double *residual, int *nstep);
+/* Copy gs%vec to the host (FP64, dim entries). */
+int edg_get_gs_vector(edg_ctx *ctx, double *vec, long long dim);
+/* Install a ground state (sector, E0, vector) given by the caller, e.g. one
+ * computed elsewhere, so that cluster_green / observables can run on it. */
+int edg_set_gs(edg_ctx *ctx, int ne_up, int ne_dn, double e0, const double *vec, long long dim);
+
+/* cluster_green_ftn (dmft_green.f90:47-126) with green_diag (:177-260):
+ * G_cl(nw,norb,nspin) complex (interleaved), omega[nw] Matsubara
+ * frequencies.  coeffs (optional, may be NULL): ng x 4 x 2 x norb with
+ * ng = maxnstep_green (or maxnstep)
+ * FP64, column-major (i, {ap,bp,am,bm}, ispin, iorb) as written to
+ * g_coeffs.dat (:89-109). */
+int edg_cluster_green(edg_ctx *ctx, const double *em, const double *ek, const double *vmk,
+                      int nw, const double *omega, double *G_cl, double *coeffs);
+
+/* Occupancy n(iorb,ispin) (norb x 2, column-major) and double occupancy
+ * docc(norb) of the held ground state (post_process.f90:63-112). */
+int edg_observables(edg_ctx *ctx, double *occ, double *docc);
+
+/* apply_c (ed_operator.f90:13-76): vec_out = c+_{iorb,ispin} vec (pm = 1)
+ * or c_{iorb,ispin} vec (pm = 2), from sector (ne_up, ne_dn) to the sector
+ * with one more / one fewer electron of spin ispin. */
+int edg_apply_c(edg_ctx *ctx, int ne_up, int ne_dn, const double *vec, int pm, int iorb,
+                int ispin, double *vec_out);
+
+/* Timing of the H.v kernel on device-resident vectors of sector
+ * (ne_up, ne_dn) in the context's precision: `reps` launches bracketed by
+ * HIP events on the context's stream after `warmup` launches.  Returns the
+ * average milliseconds per launch and the algorithmic bytes of one launch
+ * (x read once + y written once = 2 * s * dim). */
+int edg_time_hv(edg_ctx *ctx, const double *em, const double *ek, const double *vmk, int ne_up,
+                int ne_dn, int warmup, int reps, double *ms_per_launch, double *bytes_per_launch);
+
+/* Full solve timing helper: ground state in one sector + all Green's
+ * function chains, returning wall milliseconds of each phase measured with
+ * HIP events on the context stream (phase_ms[0] ground state,
+ * phase_ms[1] chains) and the number of H.v applications performed. */
+int edg_last_solve_stats(edg_ctx *ctx, double *phase_ms, long long *n_hv, long long *n_steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EDGPU_H */

@@ -1,0 +1,346 @@
+"""GPU parity of the HIP path (through the C ABI) against the FP64 oracle.
+
+Tolerances (written per test):
+  per-spin tables, apply_c           bit-exact
+  hx FP64                            1e-13 of max|y|
+  hx FP32 vectors                    2e-6 of max|y|
+  E0 FP64 / FP32                     1e-10 / 1e-6 (absolute)
+  G(i w) FP64 / FP32                 1e-9 / 1e-5 (max abs over the grid)
+  occupancy, docc FP64 / FP32        1e-10 / 1e-6
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def rand_params(rng, norb, nbath, spin_sym=False):
+    em = rng.uniform(-1, 1, (norb, 2))
+    ek = rng.uniform(-1, 1, (nbath, 2))
+    vmk = rng.uniform(0.2, 0.6, (norb, nbath, 2))
+    if spin_sym:
+        em[:, 1] = em[:, 0]
+        ek[:, 1] = ek[:, 0]
+        vmk[:, :, 1] = vmk[:, :, 0]
+    return em, ek, vmk
+
+
+def make(edgpu, oracle, norb, nbath, U=2.0, J=0.3, mu=1.0, maxnstep=80, precision="fp64", **kw):
+    s = edgpu.Solver(norb=norb, nbath=nbath, U=U, J=J, mu=mu, maxnstep=maxnstep,
+                     precision=precision, **kw)
+    m = oracle.Model(norb, nbath, U=U, J=J, mu=mu)
+    return s, m
+
+
+# ------------------------------------------------------------ tables
+
+@pytest.mark.parametrize("norb,nbath,ne", [(1, 5, 3), (1, 7, 2), (2, 4, 3), (1, 13, 7), (1, 13, 8),
+                                           (2, 12, 7)])
+def test_spin_tables_bit_exact(edgpu, oracle, norb, nbath, ne):
+    """Patterns/ranks from the reference enumeration (oracle basis) and hop
+    elements from the oracle's literal hx on the one-spin sector (ne, 0)."""
+    print(f"tables norb={norb} nbath={nbath} ne={ne}", flush=True)
+    rng = np.random.default_rng(ne + 10 * nbath)
+    em, ek, vmk = rand_params(rng, norb, nbath)
+    if norb == 2:
+        vmk[0, nbath // 2:, :] = 0.0  # block-diagonal baths: zero V is skipped
+        vmk[1, :nbath // 2, :] = 0.0
+    s, m = make(edgpu, oracle, norb, nbath)
+    ns = norb + nbath
+    for ispin in (1, 2):
+        t = s.spin_table(em, ek, vmk, ispin, ne)
+        b = oracle.Basis(ns, ne, ne)
+        assert np.array_equal(t["pattern"].astype(np.int64), b.up)
+        # one-spin sector: (ne, 0) for spin up, (0, ne) for spin down
+        sec = (ne, 0) if ispin == 1 else (0, ne)
+        bs = oracle.Basis(ns, *sec)
+        n = bs.ntot
+        rows = rng.choice(n, size=min(n, 40), replace=False)
+        for r in rows:
+            e = np.zeros(n)
+            e[r] = 1.0
+            col = oracle.hx(m, em, ek, vmk, bs, e)  # column r of H = row r (H symmetric)
+            nz = {int(i): col[i] for i in np.nonzero(col)[0] if i != r}
+            got = {int(t["col"][k, r]): t["val"][k, r] for k in range(t["cnt"][r])}
+            assert got == nz, (ispin, r)
+            assert np.all(t["col"][t["cnt"][r]:, r] == -1)
+
+
+# ---------------------------------------------------------------- hx
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 2e-6)])
+@pytest.mark.parametrize("norb,nbath,nu,nd", [(1, 5, 3, 3), (1, 5, 2, 4), (1, 9, 5, 5), (2, 4, 3, 3),
+                                              (2, 6, 4, 3), (3, 3, 3, 3), (1, 0, 1, 0)])
+def test_hx_parity(edgpu, oracle, precision, tol, norb, nbath, nu, nd):
+    print(f"hx {precision} norb={norb} nbath={nbath} sector=({nu},{nd})", flush=True)
+    rng = np.random.default_rng(100 + nbath + nu)
+    em, ek, vmk = rand_params(rng, norb, nbath)
+    s, m = make(edgpu, oracle, norb, nbath, U=2.7, J=0.5, mu=0.9, precision=precision)
+    b = oracle.Basis(norb + nbath, nu, nd)
+    x = rng.normal(size=b.ntot)
+    if precision == "fp32":
+        x = x.astype(np.float32).astype(np.float64)
+    for cols in (1, 2, 4, 8):
+        s.set_tuning(cols, False)
+        y = s.hx(em, ek, vmk, nu, nd, x)
+        ref = oracle.hx(m, em, ek, vmk, b, x)
+        assert np.abs(y - ref).max() <= tol * max(1.0, np.abs(ref).max()), cols
+    y0 = rng.normal(size=b.ntot)
+    if precision == "fp32":
+        y0 = y0.astype(np.float32).astype(np.float64)
+    y = s.hx(em, ek, vmk, nu, nd, x, y0, py=True)
+    assert np.abs(y - (y0 + ref)).max() <= 2 * tol * max(1.0, np.abs(ref).max())
+
+
+def test_hx_fp32_fp64_accumulate_both_close(edgpu, oracle):
+    rng = np.random.default_rng(5)
+    em, ek, vmk = rand_params(rng, 1, 9)
+    s, m = make(edgpu, oracle, 1, 9, precision="fp32")
+    b = oracle.Basis(10, 5, 5)
+    x = rng.normal(size=b.ntot).astype(np.float32).astype(np.float64)
+    ref = oracle.hx(m, em, ek, vmk, b, x)
+    for acc64 in (False, True):
+        s.set_tuning(4, acc64)
+        y = s.hx(em, ek, vmk, 5, 5, x)
+        # output is rounded to FP32: error bounded by FP32 rounding of the result
+        assert np.abs(y - ref).max() <= 2e-6 * np.abs(ref).max()
+
+
+# ----------------------------------------------------------- apply_c
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("norb,pm,ispin", [(1, 1, 1), (1, 2, 1), (1, 1, 2), (1, 2, 2), (2, 1, 2), (2, 2, 1)])
+def test_apply_c_parity(edgpu, oracle, precision, norb, pm, ispin):
+    rng = np.random.default_rng(pm + 3 * ispin)
+    ns = norb + 5
+    nu, nd = 3, 2
+    s, m = make(edgpu, oracle, norb, 5, precision=precision)
+    b = oracle.Basis(ns, nu, nd)
+    vec = rng.normal(size=b.ntot)
+    if precision == "fp32":
+        vec = vec.astype(np.float32).astype(np.float64)
+    for iorb in range(1, norb + 1):
+        out = s.apply_c(nu, nd, vec, pm, iorb, ispin)
+        nu2 = nu + (0 if ispin == 2 else (1 if pm == 1 else -1))
+        nd2 = nd + (0 if ispin == 1 else (1 if pm == 1 else -1))
+        ref = oracle.apply_c(b, vec, pm, iorb, ispin, oracle.Basis(ns, nu2, nd2))
+        assert np.array_equal(out, ref)
+
+
+# ------------------------------------------------------------ Lanczos
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_lanczos_iteration_parity(edgpu, oracle, precision):
+    rng = np.random.default_rng(21)
+    em, ek, vmk = rand_params(rng, 1, 7, True)
+    s, m = make(edgpu, oracle, 1, 7, U=2.0, mu=1.0, maxnstep=60, precision=precision)
+    b = oracle.Basis(8, 4, 4)
+    v0 = oracle.start_vector(4, b.ntot)
+    n, a, bb = s.lanczos_iteration(em, ek, vmk, 4, 4, v0)
+    n2, a2, b2 = oracle.lanczos_iteration(m, em, ek, vmk, b, v0, 60)
+    assert n == n2 == 60
+    assert bb[0] == 0.0
+    tol = 1e-11 if precision == "fp64" else 2e-5
+    k = 25
+    assert np.abs(a[:k] - a2[:k]).max() < tol * 10
+    assert np.abs(bb[:k] - b2[:k]).max() < tol * 10
+    # the lowest Ritz values agree regardless of late-step drift
+    w1 = oracle.tridiag_eig(a[:n], bb[1:n])[0][0]
+    w2 = oracle.tridiag_eig(a2[:n2], b2[1:n2])[0][0]
+    assert abs(w1 - w2) < (1e-10 if precision == "fp64" else 1e-6)
+
+
+def test_lanczos_breakdown_exact_eigenvector(edgpu, oracle, dense):
+    """An exact eigenvector start: b(2) < 1e-16 ends the chain after one step
+    with finite coefficients (the reference divides by b(2) before testing)."""
+    rng = np.random.default_rng(22)
+    em, ek, vmk = rand_params(rng, 1, 3)
+    s, m = make(edgpu, oracle, 1, 3, U=1.0, mu=0.5, maxnstep=10)
+    # sector (0,0)-like trivial space: D=1 in sector (0,0) is an exact eigenvector
+    n, a, bb = s.lanczos_iteration(em, ek, vmk, 0, 0, np.array([0.7]))
+    assert n == 1 and np.isfinite(a).all() and np.isfinite(bb).all()
+    assert abs(a[0] - 0.0) < 1e-15
+    # Krylov exhaustion in a tiny sector: D=4 runs out before maxnstep
+    b = oracle.Basis(4, 1, 0)
+    n, a, bb = s.lanczos_iteration(em, ek, vmk, 1, 0, oracle.start_vector(2, b.ntot))
+    assert np.isfinite(a).all() and np.isfinite(bb).all()
+    H = np.zeros((4, 4))
+    for j in range(4):
+        e = np.zeros(4)
+        e[j] = 1
+        H[:, j] = oracle.hx(m, em, ek, vmk, b, e)
+    w = np.linalg.eigvalsh(H)
+    rz = oracle.tridiag_eig(a[:n], bb[1:n])[0]
+    assert abs(rz[0] - w[0]) < 1e-10
+
+
+# -------------------------------------------------------- ground state
+
+@pytest.mark.parametrize("precision,etol,otol", [("fp64", 1e-10, 1e-10), ("fp32", 1e-6, 1e-6)])
+def test_ground_state_config1_golden(edgpu, oracle, precision, etol, otol):
+    g = np.load(os.path.join(HERE, "golden", "config1_ns6.npz"))
+    s = edgpu.Solver(norb=1, nbath=5, U=float(g["U"]), J=float(g["J"]), mu=float(g["mu"]),
+                     maxnstep=200, precision=precision, seed=9)
+    gs = s.ground_state(g["em"], g["ek"], g["vmk"], [(3, 3)])
+    assert gs.isector == 1
+    assert abs(gs.val - float(g["E0"])) < etol
+    info = s.gs_info()
+    assert info["dim"] == 400 and info["nstep"] == 200
+    assert abs(info["e0_ritz"] - float(g["E0"])) < etol * 10
+    occ, docc = s.observables()
+    assert abs(occ[0, 0] - float(g["n_up"])) < otol and abs(occ[0, 1] - float(g["n_dn"])) < otol
+    assert abs(docc[0] - float(g["docc"])) < otol
+    G, co = s.cluster_green_ftn(g["em"], g["ek"], g["vmk"], g["omega"])
+    assert G.shape == (2048, 1, 1)
+    assert np.abs(G[:, 0, 0] - g["G"]).max() < (1e-9 if precision == "fp64" else 1e-5)
+    ap, bp, am, bm = co[(1, 1)]
+    assert abs(bp[0] ** 2 - float(g["w_plus"])) < otol and abs(bm[0] ** 2 - float(g["w_minus"])) < otol
+    # the reference writes spin-1 coefficients again for spin 2 when `even`
+    assert all(np.array_equal(x, y) for x, y in zip(co[(1, 1)], co[(1, 2)]))
+
+
+def test_ground_state_vector_and_green_vs_oracle(edgpu, oracle):
+    rng = np.random.default_rng(31)
+    em, ek, vmk = rand_params(rng, 1, 7, True)
+    s, m = make(edgpu, oracle, 1, 7, U=3.0, mu=1.5, maxnstep=120, seed=3)
+    gs = s.ground_state(em, ek, vmk, [(4, 4)])
+    ogs = oracle.ground_state(m, em, ek, vmk, [(4, 4)], maxnstep=120, seed=3)
+    assert abs(gs.val - ogs.val) < 1e-10
+    v = s.gs_vector()
+    assert abs(abs(v @ ogs.vec) - 1.0) < 1e-10
+    om = oracle.matsubara(512, 100.0)
+    G, co = s.cluster_green_ftn(em, ek, vmk, om)
+    G2, co2 = oracle.cluster_green_ftn(m, em, ek, vmk, ogs, 120, om)
+    assert np.abs(G - G2).max() < 1e-9
+    for q in range(4):
+        assert np.abs(co[(1, 1)][q][:20] - co2[(1, 1)][q][:20]).max() < 1e-8
+
+
+def test_sector_sweep_and_spin_average(edgpu, oracle):
+    rng = np.random.default_rng(32)
+    em, ek, vmk = rand_params(rng, 1, 5, True)
+    s, m = make(edgpu, oracle, 1, 5, U=2.0, mu=0.3, maxnstep=100)
+    secs = [(3, 3), (2, 3), (3, 2), (2, 2)]
+    gs = s.ground_state(em, ek, vmk, secs)
+    ogs = oracle.ground_state(m, em, ek, vmk, secs, maxnstep=100)
+    assert gs.isector == ogs.isector
+    assert abs(gs.val - ogs.val) < 1e-10
+    om = oracle.matsubara(128, 40.0)
+    G, _ = s.cluster_green_ftn(em, ek, vmk, om)
+    G2, _ = oracle.cluster_green_ftn(m, em, ek, vmk, ogs, 100, om)
+    assert np.abs(G - G2).max() < 1e-9
+
+
+def test_set_gs_then_green_matches_oracle_fp32(edgpu, oracle):
+    """Chains from an identical installed ground state isolate the chain path."""
+    rng = np.random.default_rng(33)
+    em, ek, vmk = rand_params(rng, 1, 9, True)
+    s, m = make(edgpu, oracle, 1, 9, U=4.0, mu=2.0, maxnstep=150, precision="fp32")
+    ogs = oracle.ground_state(m, em, ek, vmk, [(5, 5)], maxnstep=150, seed=5)
+    s.set_gs(5, 5, ogs.val, ogs.vec)
+    om = oracle.matsubara(2048, 100.0)
+    G, _ = s.cluster_green_ftn(em, ek, vmk, om)
+    G2, _ = oracle.cluster_green_ftn(m, em, ek, vmk, ogs, 150, om)
+    assert np.abs(G - G2).max() < 1e-5
+
+
+def test_kanamori_two_orbital_vs_oracle(edgpu, oracle):
+    from edgpu import workloads as W
+    em, ek, vmk, mu, J, Up = W.two_orbital_kanamori(2, 3.0, seed=4)
+    for precision, etol, gtol in (("fp64", 1e-10, 1e-9), ("fp32", 1e-6, 1e-5)):
+        s = edgpu.Solver(norb=2, nbath=4, U=3.0, J=J, mu=mu, maxnstep=150, precision=precision)
+        m = oracle.Model(2, 4, U=3.0, J=J, mu=mu)
+        gs = s.ground_state(em, ek, vmk, [(3, 3)])
+        ogs = oracle.ground_state(m, em, ek, vmk, [(3, 3)], maxnstep=150)
+        assert abs(gs.val - ogs.val) < etol
+        om = oracle.matsubara(256, 100.0)
+        G, _ = s.cluster_green_ftn(em, ek, vmk, om)
+        G2, _ = oracle.cluster_green_ftn(m, em, ek, vmk, ogs, 150, om)
+        assert np.abs(G - G2).max() < gtol
+        occ, docc = s.observables()
+        o2, d2 = oracle.observables(m, ogs.basis, ogs.vec)
+        assert np.abs(occ - o2).max() < etol * 100 and np.abs(docc - d2).max() < etol * 100
+
+
+def test_nspin2_outputs_both_spins(edgpu, oracle):
+    rng = np.random.default_rng(34)
+    em, ek, vmk = rand_params(rng, 1, 5, False)
+    s = edgpu.Solver(norb=1, nbath=5, U=2.0, mu=1.0, maxnstep=100, nspin=2)
+    m = oracle.Model(1, 5, U=2.0, mu=1.0)
+    gs = s.ground_state(em, ek, vmk, [(3, 3)])
+    ogs = oracle.ground_state(m, em, ek, vmk, [(3, 3)], maxnstep=100)
+    om = oracle.matsubara(64, 50.0)
+    G, _ = s.cluster_green_ftn(em, ek, vmk, om)
+    G2, _ = oracle.cluster_green_ftn(m, em, ek, vmk, ogs, 100, om, nspin=2)
+    assert G.shape == (64, 1, 2)
+    assert np.abs(G - G2).max() < 1e-9
+
+
+# ---------------------------------------------------------------- errors
+
+def test_error_paths(edgpu):
+    rng = np.random.default_rng(35)
+    em, ek, vmk = rand_params(rng, 1, 5)
+    s = edgpu.Solver(norb=1, nbath=5, maxnstep=50)
+    with pytest.raises(edgpu.EdgpuError) as e:
+        s.cluster_green_ftn(em, ek, vmk, np.array([1.0]))
+    assert e.value.code == 5
+    with pytest.raises(edgpu.EdgpuError) as e:
+        s.ground_state(em, ek, vmk, [(7, 3)])
+    assert e.value.code == 3
+    s.set_memory_budget(1000)
+    with pytest.raises(edgpu.EdgpuError) as e:
+        s.ground_state(em, ek, vmk, [(3, 3)])
+    assert e.value.code == 2
+    s.set_memory_budget(0)
+    gs = s.ground_state(em, ek, vmk, [(3, 3)])
+    assert np.isfinite(gs.val)
+
+
+# ----------------------------------------------- Ns=14 (config 2 shape)
+
+def test_ns14_properties_fp32_vs_fp64(edgpu, oracle):
+    """Config-2 shape (Ns=14, sector (7,7), D=11,778,624): hx against the
+    oracle on one vector; symmetry <y,Hx> = <x,Hy>; FP32-vector solve vs
+    FP64 solve: |dE0| <= 1e-6 and max|dG| <= 1e-5; sum rule; n_up = n_dn."""
+    from edgpu import workloads as W
+    print("ns14 hx parity", flush=True)
+    oracle.set_threads(16)
+    em, ek, vmk, mu = W.single_orbital(13, 2.0, seed=1)
+    s64 = edgpu.Solver(norb=1, nbath=13, U=2.0, mu=mu, maxnstep=200, precision="fp64", both_spins=True)
+    m = oracle.Model(1, 13, U=2.0, mu=mu)
+    b = oracle.Basis(14, 7, 7)
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=b.ntot)
+    y = s64.hx(em, ek, vmk, 7, 7, x)
+    ref = oracle.hx(m, em, ek, vmk, b, x)
+    assert np.abs(y - ref).max() < 1e-13 * np.abs(ref).max()
+    x2 = rng.normal(size=b.ntot)
+    y2 = s64.hx(em, ek, vmk, 7, 7, x2)
+    assert abs(x2 @ y - x @ y2) < 1e-10 * abs(x @ y2)
+    print("ns14 solves", flush=True)
+    om = W.matsubara(2048, 100.0)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        s = edgpu.Solver(norb=1, nbath=13, U=2.0, mu=mu, maxnstep=200, maxnstep_green=150,
+                         precision=prec, both_spins=True)
+        gs = s.ground_state(em, ek, vmk, [(7, 7)])
+        occ, docc = s.observables()
+        G, co = s.cluster_green_ftn(em, ek, vmk, om)
+        res[prec] = (gs.val, G, occ, docc, co, s.gs_info())
+        print(prec, gs.val, s.gs_info(), flush=True)
+    e64, G64, o64, d64, c64, i64 = res["fp64"]
+    e32, G32, o32, d32, c32, i32 = res["fp32"]
+    assert abs(e32 - e64) <= 1e-6
+    assert np.abs(G32 - G64).max() <= 1e-5
+    assert abs(o64[0, 0] - o64[0, 1]) < 1e-9  # spin-symmetric bath, N_up = N_dn
+    assert abs(d32[0] - d64[0]) < 1e-6
+    for co in (c64, c32):
+        ap, bp, am, bm = co[(1, 1)]
+        assert abs(bp[0] ** 2 + bm[0] ** 2 - 1.0) < 1e-6
+        for q in range(4):  # spin symmetry of the chains
+            assert np.abs(co[(1, 1)][q][:10] - co[(1, 2)][q][:10]).max() < 1e-5
+    assert i64["residual"] < 1e-8
