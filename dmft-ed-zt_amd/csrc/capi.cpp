@@ -175,10 +175,10 @@ int get_tab(edg_ctx *c, int spin, int ne, const SpinTab **out)
         const size_t n = t.n, K = t.K;
         int rc = 0;
         rc |= al((void **)&t.pattern, n * 4);
-        rc |= al((void **)&t.col, K * n * 2);
-        rc |= al((void **)&t.valf, K * n * 4);
-        rc |= al((void **)&t.vald, K * n * 8);
+        rc |= al((void **)&t.hf, K * n * sizeof(HopF));
+        rc |= al((void **)&t.hd, K * n * sizeof(HopD));
         rc |= al((void **)&t.cnt, n);
+        rc |= al((void **)&t.maxcnt, sizeof(int));
         rc |= al((void **)&t.energy, n * 8);
         rc |= al((void **)&t.mask, n);
         if (norb >= 2) rc |= al((void **)&t.orb, (size_t)norb * norb * n * 4);
@@ -190,8 +190,13 @@ int get_tab(edg_ctx *c, int spin, int ne, const SpinTab **out)
         const double *ek_s = dp + 2 * norb + (spin - 1) * nbath;
         const double *v_s = dp + 2 * norb + 2 * nbath + (size_t)(spin - 1) * nbath * norb;
         const double UpmJ = c->P.Up - c->P.J;
+        HIPCHK(c, hipMemsetAsync(te.t.maxcnt, 0, sizeof(int), c->stream));
         if (launch_spin_table(c->stream, ns, norb, nbath, ne, em_s, ek_s, v_s, c->P.mu, UpmJ, te.t))
             return fail(c, EDG_ERR_HIP, "spin table kernel launch failed");
+        int km = 0;
+        HIPCHK(c, hipMemcpyAsync(&km, te.t.maxcnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        te.t.kmax = km;
         te.version = c->version;
     }
     *out = &te.t;
@@ -730,15 +735,18 @@ int edg_spin_table(edg_ctx *c, const double *em, const double *ek, const double 
     *kmax = t->K;
     const size_t N = t->n, K = t->K;
     if (pattern) HIPCHK(c, hipMemcpyAsync(pattern, t->pattern, N * 4, hipMemcpyDeviceToHost, c->stream));
-    std::vector<uint16_t> col(K * N);
+    std::vector<HopD> hd(K * N);
     std::vector<uint8_t> cnt(N);
-    if (hop_col && K) HIPCHK(c, hipMemcpyAsync(col.data(), t->col, K * N * 2, hipMemcpyDeviceToHost, c->stream));
-    if (hop_val && K) HIPCHK(c, hipMemcpyAsync(hop_val, t->vald, K * N * 8, hipMemcpyDeviceToHost, c->stream));
-    if (hop_cnt) HIPCHK(c, hipMemcpyAsync(cnt.data(), t->cnt, N, hipMemcpyDeviceToHost, c->stream));
+    if (K) HIPCHK(c, hipMemcpyAsync(hd.data(), t->hd, K * N * sizeof(HopD), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(cnt.data(), t->cnt, N, hipMemcpyDeviceToHost, c->stream));
     if (energy) HIPCHK(c, hipMemcpyAsync(energy, t->energy, N * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (hop_col)
-        for (size_t i = 0; i < K * N; ++i) hop_col[i] = col[i] == kNoCol ? -1 : (int)col[i];
+    for (size_t k = 0; k < K; ++k)
+        for (size_t r = 0; r < N; ++r) {
+            const bool used = k < cnt[r];
+            if (hop_col) hop_col[k * N + r] = used ? (int)hd[k * N + r].col : -1;
+            if (hop_val) hop_val[k * N + r] = used ? hd[k * N + r].val : 0.0;
+        }
     if (hop_cnt)
         for (size_t i = 0; i < N; ++i) hop_cnt[i] = cnt[i];
     return EDG_OK;

@@ -5,25 +5,39 @@
 
 namespace edg {
 
-constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
+constexpr int kBlock = 256;       // threads per workgroup of the vector kernels
+constexpr int kHvBlock = 512;     // threads per workgroup of the H.v kernel (8 wave64)
 constexpr int kMaxHops = 64;      // per-spin hop slots per config (norb*nbath)
 constexpr int kMaxNorb = 5;       // observables accumulate 3*norb sums
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
-constexpr uint16_t kNoCol = 0xFFFFu;
+
+// One hop of a per-spin table: target rank and matrix element.  Unused
+// slots point at the row itself with a zero element, so that unrolled
+// loops can read every slot without bounds tests.
+struct HopF {
+    uint32_t col;
+    float val;
+};
+struct HopD {
+    uint32_t col;
+    uint32_t pad;
+    double val;
+};
 
 // Device arrays describing one spin half (spin sigma, N_sigma electrons) of
 // a sector under one set of Hamiltonian parameters.
 struct SpinTab {
-    int n = 0;               // C(nsite, N)
-    int K = 0;               // ELL width = norb*nbath
+    int n = 0;                     // C(nsite, N)
+    int K = 0;                     // slots per config = norb*nbath
+    int kmax = 0;                  // largest hop count over all configs
     uint32_t *pattern = nullptr;   // [n]
-    uint16_t *col = nullptr;       // [K][n] target rank
-    float *valf = nullptr;         // [K][n] matrix element (FP32 copy)
-    double *vald = nullptr;        // [K][n] matrix element (FP64)
+    HopF *hf = nullptr;            // [K][n] (FP32 element)
+    HopD *hd = nullptr;            // [K][n] (FP64 element)
     uint8_t *cnt = nullptr;        // [n]
     double *energy = nullptr;      // [n]
     uint8_t *mask = nullptr;       // [n] occupied impurity orbitals
     uint32_t *orb = nullptr;       // [norb*norb][n] orbital moves dst<-src
+    int *maxcnt = nullptr;         // device scalar: max over cnt
 };
 
 // Everything the H.v kernel needs besides the vectors.

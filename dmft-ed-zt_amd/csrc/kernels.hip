@@ -26,10 +26,10 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 
 // Sum NS per-thread values over the block; result valid in thread 0.
-template <int NS>
+template <int NS, int BS = kBlock>
 __device__ __forceinline__ void block_sums(double (&v)[NS], double (&out)[NS])
 {
-    __shared__ double red[NS][kBlock / 64];
+    __shared__ double red[NS][BS / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -42,7 +42,7 @@ __device__ __forceinline__ void block_sums(double (&v)[NS], double (&out)[NS])
         for (int s = 0; s < NS; ++s) {
             double t = 0.0;
 #pragma unroll
-            for (int k = 0; k < kBlock / 64; ++k) t += red[s][k];
+            for (int k = 0; k < BS / 64; ++k) t += red[s][k];
             out[s] = t;
         }
     }
@@ -58,13 +58,13 @@ __device__ __forceinline__ double load_agent(const double *p)
 
 // Publish this block's partial sums; the last block to arrive reduces all
 // partials in a fixed order into tot (thread 0) and returns true.
-template <int NS>
+template <int NS, int BS = kBlock>
 __device__ bool reduce_last(double (&mine)[NS], double *partial, unsigned *ticket,
                             double (&tot)[NS])
 {
     __shared__ int s_last;
     double blk[NS];
-    block_sums<NS>(mine, blk);
+    block_sums<NS, BS>(mine, blk);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int s = 0; s < NS; ++s) partial[(long long)blockIdx.x * NS + s] = blk[s];
@@ -78,10 +78,10 @@ __device__ bool reduce_last(double (&mine)[NS], double *partial, unsigned *ticke
     double v[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) v[s] = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += BS)
 #pragma unroll
         for (int s = 0; s < NS; ++s) v[s] += load_agent(&partial[(long long)b * NS + s]);
-    block_sums<NS>(v, tot);
+    block_sums<NS, BS>(v, tot);
     if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return true;
 }
@@ -142,18 +142,18 @@ __global__ __launch_bounds__(kBlock) void k_spin_table(int nsite, int norb, int 
             const uint32_t q = p ^ ((1u << i) | (1u << j));
             const uint32_t between = p & ((1u << j) - 1u) & ~((2u << i) - 1u);
             const double val = (__popc(between) & 1) ? -v : v;
-            t.col[(long long)slot * t.n + r] = (uint16_t)rank_of(q);
-            t.vald[(long long)slot * t.n + r] = val;
-            t.valf[(long long)slot * t.n + r] = (float)val;
+            const uint32_t c = rank_of(q);
+            t.hd[(long long)slot * t.n + r] = HopD{c, 0u, val};
+            t.hf[(long long)slot * t.n + r] = HopF{c, (float)val};
             ++slot;
         }
     }
     t.cnt[r] = (uint8_t)slot;
     for (int k = slot; k < t.K; ++k) {
-        t.col[(long long)k * t.n + r] = kNoCol;
-        t.vald[(long long)k * t.n + r] = 0.0;
-        t.valf[(long long)k * t.n + r] = 0.0f;
+        t.hd[(long long)k * t.n + r] = HopD{(uint32_t)r, 0u, 0.0};
+        t.hf[(long long)k * t.n + r] = HopF{(uint32_t)r, 0.0f};
     }
+    atomicMax(t.maxcnt, slot);
     // one-spin diagonal energy
     double e = 0.0;
     for (int m = 0; m < norb; ++m)
@@ -199,11 +199,17 @@ struct alignas(sizeof(T) * C) Row {
 };
 
 template <typename ACC>
-__device__ __forceinline__ ACC hopval(const SpinTab &t, long long e);
+struct HopOf;
 template <>
-__device__ __forceinline__ float hopval<float>(const SpinTab &t, long long e) { return t.valf[e]; }
+struct HopOf<float> {
+    using type = HopF;
+    static __device__ __forceinline__ const HopF *tab(const SpinTab &t) { return t.hf; }
+};
 template <>
-__device__ __forceinline__ double hopval<double>(const SpinTab &t, long long e) { return t.vald[e]; }
+struct HopOf<double> {
+    using type = HopD;
+    static __device__ __forceinline__ const HopD *tab(const SpinTab &t) { return t.hd; }
+};
 
 __device__ __forceinline__ double cross_energy(int mu, int md, double U, double Up)
 {
@@ -211,10 +217,16 @@ __device__ __forceinline__ double cross_energy(int mu, int md, double U, double 
     return U * both + Up * (double)(__popc(mu) * __popc(md) - both);
 }
 
-template <typename T, typename ACC, int C, int MODE, bool KAN>
-__global__ __launch_bounds__(kBlock) void k_hv(HvOp op, HvArgs a)
+// y = H x over the C columns [idn0, idn0+C) of the sector matrix.
+//   T    storage type of the vectors, ACC arithmetic type of the row sums
+//   KH   compile-time bound on the hop count of both spins (0 = any, loops)
+//   MODE HV_PLAIN: y = Hx or y += Hx; HV_LANCZOS: y = s_j Hx - beta_j s_{j-1} p
+//        with alpha_j = s_j <x, y>; HV_RQ: no store, sums <x,Hx>, <Hx,Hx>, <x,x>
+template <typename T, typename ACC, int C, int MODE, bool KAN, int KH>
+__global__ __launch_bounds__(kHvBlock) void k_hv(HvOp op, HvArgs a)
 {
     if (a.stop && *a.stop) return;
+    using Hop = typename HopOf<ACC>::type;
     extern __shared__ __align__(64) unsigned char smem[];
     T *xs = reinterpret_cast<T *>(smem);
     __shared__ int s_cnt[C];
@@ -228,8 +240,10 @@ __global__ __launch_bounds__(kBlock) void k_hv(HvOp op, HvArgs a)
     const long long ld = nup;
     const int idn0 = blockIdx.x * C;
     const T *x = static_cast<const T *>(a.x);
+    const Hop *dtab = HopOf<ACC>::tab(op.dn);
+    const Hop *utab = HopOf<ACC>::tab(op.up);
 
-    for (int e = threadIdx.x; e < C * kMaxHops; e += kBlock) {
+    for (int e = threadIdx.x; e < C * kMaxHops; e += kHvBlock) {
         const int c = e / kMaxHops, k = e - c * kMaxHops;
         const int idn = idn0 + c;
         const int cnt = idn < ndn ? op.dn.cnt[idn] : 0;
@@ -239,26 +253,28 @@ __global__ __launch_bounds__(kBlock) void k_hv(HvOp op, HvArgs a)
             s_md[c] = idn < ndn ? op.dn.mask[idn] : 0;
         }
         if (k < cnt) {
-            s_col[c][k] = op.dn.col[(long long)k * ndn + idn];
-            s_val[c][k] = hopval<ACC>(op.dn, (long long)k * ndn + idn);
+            const Hop h = dtab[(long long)k * ndn + idn];
+            s_col[c][k] = (int)h.col;
+            s_val[c][k] = (ACC)h.val;
         }
     }
     if constexpr (KAN) {
         const int nn = op.norb * op.norb;
-        for (int e = threadIdx.x; e < C * nn; e += kBlock) {
+        for (int e = threadIdx.x; e < C * nn; e += kHvBlock) {
             const int c = e / nn, q = e - c * nn;
             const int idn = idn0 + c;
             s_orb[c][q] = idn < ndn ? op.dn.orb[(long long)q * ndn + idn] : kInvalid;
         }
     }
+    // stage the C columns, row-interleaved: xs[i*C + c] = x(i, idn0 + c)
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         const int idn = idn0 + c;
         if (idn < ndn) {
             const T *xc = x + (long long)idn * ld;
-            for (int i = threadIdx.x; i < nup; i += kBlock) xs[i * C + c] = xc[i];
+            for (int i = threadIdx.x; i < nup; i += kHvBlock) xs[i * C + c] = xc[i];
         } else {
-            for (int i = threadIdx.x; i < nup; i += kBlock) xs[i * C + c] = T(0);
+            for (int i = threadIdx.x; i < nup; i += kHvBlock) xs[i * C + c] = T(0);
         }
     }
     __syncthreads();
@@ -271,8 +287,9 @@ __global__ __launch_bounds__(kBlock) void k_hv(HvOp op, HvArgs a)
     }
     T *y = static_cast<T *>(a.y);
     const T *pv = static_cast<const T *>(a.p);
+    const int Ku = op.up.K;
 
-    for (int iup = threadIdx.x; iup < nup; iup += kBlock) {
+    for (int iup = threadIdx.x; iup < nup; iup += kHvBlock) {
         ACC acc[C];
         const Row<T, C> xr = *reinterpret_cast<const Row<T, C> *>(xs + iup * C);
         const double eu = op.up.energy[iup];
@@ -280,20 +297,47 @@ __global__ __launch_bounds__(kBlock) void k_hv(HvOp op, HvArgs a)
 #pragma unroll
         for (int c = 0; c < C; ++c)
             acc[c] = (ACC)(eu + s_ed[c] + cross_energy(mu, s_md[c], op.U, op.Up)) * (ACC)xr.v[c];
-        const int cu = op.up.cnt[iup];
-        for (int k = 0; k < cu; ++k) {
-            const long long e = (long long)k * nup + iup;
-            const int col = op.up.col[e];
-            const ACC v = hopval<ACC>(op.up, e);
-            const Row<T, C> src = *reinterpret_cast<const Row<T, C> *>(xs + col * C);
+        if constexpr (KH > 0) {
+            // up hops: all table records, then all LDS rows, then the FMAs
+            Hop h[KH];
 #pragma unroll
-            for (int c = 0; c < C; ++c) acc[c] += v * (ACC)src.v[c];
-        }
+            for (int k = 0; k < KH; ++k)
+                if (k < Ku) h[k] = utab[(long long)k * nup + iup];
+                else h[k] = Hop{};
+            Row<T, C> src[KH];
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
-            const int cnt = s_cnt[c];
-            for (int k = 0; k < cnt; ++k)
-                acc[c] += s_val[c][k] * (ACC)x[(long long)s_col[c][k] * ld + iup];
+            for (int k = 0; k < KH; ++k)
+                src[k] = *reinterpret_cast<const Row<T, C> *>(xs + (k < Ku ? h[k].col : iup) * C);
+#pragma unroll
+            for (int k = 0; k < KH; ++k)
+#pragma unroll
+                for (int c = 0; c < C; ++c) acc[c] += (ACC)h[k].val * (ACC)src[k].v[c];
+            // down hops: neighbour columns from global memory
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int cnt = s_cnt[c];
+                T g[KH];
+#pragma unroll
+                for (int k = 0; k < KH; ++k)
+                    g[k] = k < cnt ? x[(long long)s_col[c][k] * ld + iup] : T(0);
+#pragma unroll
+                for (int k = 0; k < KH; ++k)
+                    if (k < cnt) acc[c] += s_val[c][k] * (ACC)g[k];
+            }
+        } else {
+            const int cu = op.up.cnt[iup];
+            for (int k = 0; k < cu; ++k) {
+                const Hop hk = utab[(long long)k * nup + iup];
+                const Row<T, C> src = *reinterpret_cast<const Row<T, C> *>(xs + hk.col * C);
+#pragma unroll
+                for (int c = 0; c < C; ++c) acc[c] += (ACC)hk.val * (ACC)src.v[c];
+            }
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int cnt = s_cnt[c];
+                for (int k = 0; k < cnt; ++k)
+                    acc[c] += s_val[c][k] * (ACC)x[(long long)s_col[c][k] * ld + iup];
+            }
         }
         if constexpr (KAN) {
             const int norb = op.norb;
@@ -339,11 +383,11 @@ __global__ __launch_bounds__(kBlock) void k_hv(HvOp op, HvArgs a)
     }
     if constexpr (MODE == HV_LANCZOS) {
         double mine[1] = {sums[0]}, tot[1];
-        if (reduce_last<1>(mine, a.partial, a.ticket, tot) && threadIdx.x == 0)
+        if (reduce_last<1, kHvBlock>(mine, a.partial, a.ticket, tot) && threadIdx.x == 0)
             a.result[0] = sx * tot[0];  // alpha_j = <q_j, r>
     } else if constexpr (MODE == HV_RQ) {
         double tot[3];
-        if (reduce_last<3>(sums, a.partial, a.ticket, tot) && threadIdx.x == 0) {
+        if (reduce_last<3, kHvBlock>(sums, a.partial, a.ticket, tot) && threadIdx.x == 0) {
             a.result[0] = tot[0];
             a.result[1] = tot[1];
             a.result[2] = tot[2];
@@ -351,51 +395,58 @@ __global__ __launch_bounds__(kBlock) void k_hv(HvOp op, HvArgs a)
     }
 }
 
-template <typename T, typename ACC, int C, bool KAN>
+// Dynamic LDS budget of one workgroup: gfx950 has 160 KiB per CU; keep 8 KiB
+// for the static per-column hop lists and reduction scratch.
+constexpr size_t kLdsBudget = 152 * 1024;
+
+template <typename T, typename ACC, int C, bool KAN, int KH>
 static int launch_hv_c(hipStream_t s, const HvOp &op, int mode, const HvArgs &a)
 {
     const size_t lds = (size_t)op.nup * C * sizeof(T);
     const int grid = (op.ndn + C - 1) / C;
     void (*f)(HvOp, HvArgs);
-    if (mode == HV_PLAIN) f = k_hv<T, ACC, C, HV_PLAIN, KAN>;
-    else if (mode == HV_LANCZOS) f = k_hv<T, ACC, C, HV_LANCZOS, KAN>;
-    else f = k_hv<T, double, C, HV_RQ, KAN>;
+    if (mode == HV_PLAIN) f = k_hv<T, ACC, C, HV_PLAIN, KAN, KH>;
+    else if (mode == HV_LANCZOS) f = k_hv<T, ACC, C, HV_LANCZOS, KAN, KH>;
+    else f = k_hv<T, double, C, HV_RQ, KAN, KH>;
     if (lds > 64 * 1024) {
         if (hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds) != hipSuccess)
             return -1;
     }
-    hipLaunchKernelGGL(f, dim3(grid), dim3(kBlock), lds, s, op, a);
+    hipLaunchKernelGGL(f, dim3(grid), dim3(kHvBlock), lds, s, op, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <typename T, typename ACC, bool KAN>
+template <typename T, typename ACC, bool KAN, int KH>
 static int launch_hv_k(hipStream_t s, const HvOp &op, int mode, int cols, const HvArgs &a)
 {
     switch (cols) {
-    case 1: return launch_hv_c<T, ACC, 1, KAN>(s, op, mode, a);
-    case 2: return launch_hv_c<T, ACC, 2, KAN>(s, op, mode, a);
-    case 4: return launch_hv_c<T, ACC, 4, KAN>(s, op, mode, a);
-    case 8: return launch_hv_c<T, ACC, 8, KAN>(s, op, mode, a);
+    case 1: return launch_hv_c<T, ACC, 1, KAN, KH>(s, op, mode, a);
+    case 2: return launch_hv_c<T, ACC, 2, KAN, KH>(s, op, mode, a);
+    case 4: return launch_hv_c<T, ACC, 4, KAN, KH>(s, op, mode, a);
+    case 8: return launch_hv_c<T, ACC, 8, KAN, KH>(s, op, mode, a);
     default: return -2;
     }
 }
 
-// Dynamic LDS budget of one workgroup: gfx950 has 160 KiB per CU; keep 8 KiB
-// for the static per-column hop lists and reduction scratch.
-constexpr size_t kLdsBudget = 152 * 1024;
+template <typename T, typename ACC>
+static int launch_hv_a(hipStream_t s, const HvOp &op, int mode, int cols, const HvArgs &a)
+{
+    const bool kan = op.norb >= 2;
+    const int km = op.up.kmax > op.dn.kmax ? op.up.kmax : op.dn.kmax;
+    if (kan) return launch_hv_k<T, ACC, true, 0>(s, op, mode, cols, a);
+    if (km <= 8) return launch_hv_k<T, ACC, false, 8>(s, op, mode, cols, a);
+    if (km <= 16) return launch_hv_k<T, ACC, false, 16>(s, op, mode, cols, a);
+    return launch_hv_k<T, ACC, false, 0>(s, op, mode, cols, a);
+}
 
 template <typename T>
 int launch_hv(hipStream_t s, const HvOp &op, int mode, int cols, bool acc64, const HvArgs &a)
 {
-    const bool kan = op.norb >= 2;
     while (cols > 1 && (size_t)op.nup * cols * sizeof(T) > kLdsBudget) cols >>= 1;
     if ((size_t)op.nup * cols * sizeof(T) > kLdsBudget) return -3;
-    if (sizeof(T) == 8 || acc64 || mode == HV_RQ)
-        return kan ? launch_hv_k<T, double, true>(s, op, mode, cols, a)
-                   : launch_hv_k<T, double, false>(s, op, mode, cols, a);
-    return kan ? launch_hv_k<T, float, true>(s, op, mode, cols, a)
-               : launch_hv_k<T, float, false>(s, op, mode, cols, a);
+    if (sizeof(T) == 8 || acc64 || mode == HV_RQ) return launch_hv_a<T, double>(s, op, mode, cols, a);
+    return launch_hv_a<T, float>(s, op, mode, cols, a);
 }
 template int launch_hv<float>(hipStream_t, const HvOp &, int, int, bool, const HvArgs &);
 template int launch_hv<double>(hipStream_t, const HvOp &, int, int, bool, const HvArgs &);

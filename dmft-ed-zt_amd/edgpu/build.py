@@ -29,6 +29,27 @@ FLAGS = [
 ]
 
 
+FORTRAN = os.path.join(PKG, "fortran")
+FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
+EXAMPLE = os.path.join(FORTRAN, "example_solve")
+
+
+def build_fortran(verbose=False):
+    """Compile the iso_c_binding module and the example caller (amdflang)."""
+    if not os.path.exists(FLANG):
+        return None
+    mod = os.path.join(FORTRAN, "edgpu_c.f90")
+    cmds = [[FLANG, "-c", mod, "-module-dir", FORTRAN, "-o", os.path.join(FORTRAN, "edgpu_c.o")],
+            [FLANG, "-I", FORTRAN, os.path.join(FORTRAN, "example_solve.f90"),
+             os.path.join(FORTRAN, "edgpu_c.o"), "-L", HERE, "-ledgpu",
+             "-Wl,-rpath,$ORIGIN/../edgpu", "-o", EXAMPLE]]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd, cwd=FORTRAN)
+    return EXAMPLE
+
+
 def build(verbose=False):
     """Compile every HIP/C++ source into one shared library (always rebuilds)."""
     objs = []
@@ -49,6 +70,7 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    build_fortran(verbose)
     return LIB
 
 
