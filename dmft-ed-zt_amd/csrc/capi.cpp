@@ -163,7 +163,7 @@ int get_tab(edg_ctx *c, int spin, int ne, const SpinTab **out)
     if (te.allocs.empty()) {
         SpinTab &t = te.t;
         t.n = (int)binom(ns, ne);
-        t.K = norb * nbath;
+        t.K = norb * nbath > kMinSlots ? norb * nbath : kMinSlots;
         auto al = [&](void **p, size_t bytes) -> int {
             if (hipMalloc(p, bytes < 16 ? 16 : bytes) != hipSuccess) {
                 (void)hipGetLastError();
@@ -180,6 +180,7 @@ int get_tab(edg_ctx *c, int spin, int ne, const SpinTab **out)
         rc |= al((void **)&t.cnt, n);
         rc |= al((void **)&t.maxcnt, sizeof(int));
         rc |= al((void **)&t.energy, n * 8);
+        rc |= al((void **)&t.energyf, n * 4);
         rc |= al((void **)&t.mask, n);
         if (norb >= 2) rc |= al((void **)&t.orb, (size_t)norb * norb * n * 4);
         if (rc) return rc;
@@ -197,6 +198,25 @@ int get_tab(edg_ctx *c, int spin, int ne, const SpinTab **out)
         HIPCHK(c, hipMemcpyAsync(&km, te.t.maxcnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         te.t.kmax = km;
+        te.t.vsrc = v_s;
+        if (norb == 1 && km <= 16) {  // compressed table for the single-orbital kernel
+            te.t.KC = km <= 8 ? 8 : 16;
+            if (!te.t.ccol) {
+                auto al2 = [&](void **p, size_t bytes) -> int {
+                    if (hipMalloc(p, bytes) != hipSuccess) {
+                        (void)hipGetLastError();
+                        return fail(c, EDG_ERR_NOMEM, "table allocation failed");
+                    }
+                    te.allocs.push_back(*p);
+                    return 0;
+                };
+                if (int rc = al2((void **)&te.t.ccol, (size_t)te.t.n * 16 * 2)) return rc;
+                if (int rc = al2((void **)&te.t.cidx, (size_t)te.t.n * 16)) return rc;
+            }
+            if (launch_compress(c->stream, te.t)) return fail(c, EDG_ERR_HIP, "compress launch failed");
+        } else {
+            te.t.KC = 0;
+        }
         te.version = c->version;
     }
     *out = &te.t;
@@ -212,6 +232,7 @@ int make_op(edg_ctx *c, int nu, int nd, HvOp *op)
     op->nup = tu->n;
     op->ndn = td->n;
     op->norb = c->P.norb;
+    op->nvup = c->P.norb * c->P.nbath;
     op->up = *tu;
     op->dn = *td;
     op->U = c->P.U;
@@ -657,6 +678,9 @@ int edg_create(const edg_params *p, int device, edg_ctx **out)
     if (c->P.maxnstep_green <= 0) c->P.maxnstep_green = c->P.maxnstep;
     c->device = device;
     c->cols = p->precision == EDG_FP32 ? 4 : 2;
+    // Kanamori models (norb >= 2) accumulate FP32-vector row sums in FP64: the
+    // spin-flip / pair-hop terms make FP32 sums the leading error in G(i w)
+    c->acc64 = p->precision == EDG_FP32 && p->norb >= 2;
     auto bad = [&]() {
         (void)hipGetLastError();
         delete c;
@@ -732,8 +756,8 @@ int edg_spin_table(edg_ctx *c, const double *em, const double *ek, const double 
     const SpinTab *t;
     if (int rc = get_tab(c, ispin, ne, &t)) return rc;
     *n = t->n;
-    *kmax = t->K;
-    const size_t N = t->n, K = t->K;
+    *kmax = c->P.norb * c->P.nbath;  // slots beyond norb*nbath are padding only
+    const size_t N = t->n, K = t->K, Kr = (size_t)c->P.norb * c->P.nbath;
     if (pattern) HIPCHK(c, hipMemcpyAsync(pattern, t->pattern, N * 4, hipMemcpyDeviceToHost, c->stream));
     std::vector<HopD> hd(K * N);
     std::vector<uint8_t> cnt(N);
@@ -741,11 +765,11 @@ int edg_spin_table(edg_ctx *c, const double *em, const double *ek, const double 
     HIPCHK(c, hipMemcpyAsync(cnt.data(), t->cnt, N, hipMemcpyDeviceToHost, c->stream));
     if (energy) HIPCHK(c, hipMemcpyAsync(energy, t->energy, N * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (size_t k = 0; k < K; ++k)
+    for (size_t k = 0; k < Kr; ++k)
         for (size_t r = 0; r < N; ++r) {
             const bool used = k < cnt[r];
-            if (hop_col) hop_col[k * N + r] = used ? (int)hd[k * N + r].col : -1;
-            if (hop_val) hop_val[k * N + r] = used ? hd[k * N + r].val : 0.0;
+            if (hop_col) hop_col[k * N + r] = used ? (int)hd[r * K + k].col : -1;
+            if (hop_val) hop_val[k * N + r] = used ? hd[r * K + k].val : 0.0;
         }
     if (hop_cnt)
         for (size_t i = 0; i < N; ++i) hop_cnt[i] = cnt[i];

@@ -8,6 +8,7 @@ namespace edg {
 constexpr int kBlock = 256;       // threads per workgroup of the vector kernels
 constexpr int kHvBlock = 512;     // threads per workgroup of the H.v kernel (8 wave64)
 constexpr int kMaxHops = 64;      // per-spin hop slots per config (norb*nbath)
+constexpr int kMinSlots = 16;     // tables hold at least this many slots (padding)
 constexpr int kMaxNorb = 5;       // observables accumulate 3*norb sums
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 
@@ -28,21 +29,29 @@ struct HopD {
 // a sector under one set of Hamiltonian parameters.
 struct SpinTab {
     int n = 0;                     // C(nsite, N)
-    int K = 0;                     // slots per config = norb*nbath
+    int K = 0;                     // slots per config = max(norb*nbath, kMinSlots)
     int kmax = 0;                  // largest hop count over all configs
     uint32_t *pattern = nullptr;   // [n]
     HopF *hf = nullptr;            // [K][n] (FP32 element)
     HopD *hd = nullptr;            // [K][n] (FP64 element)
     uint8_t *cnt = nullptr;        // [n]
     double *energy = nullptr;      // [n]
+    float *energyf = nullptr;      // [n] FP32 copy for FP32 row sums
     uint8_t *mask = nullptr;       // [n] occupied impurity orbitals
     uint32_t *orb = nullptr;       // [norb*norb][n] orbital moves dst<-src
     int *maxcnt = nullptr;         // device scalar: max over cnt
+    // compressed hop table of the single-orbital fast path, [n][KC]:
+    // ccol = target rank, cidx = index into vsrc (bit 7: negative), 64 = zero
+    int KC = 0;                    // 8 or 16 (0: not built)
+    uint16_t *ccol = nullptr;
+    uint8_t *cidx = nullptr;
+    const double *vsrc = nullptr;  // hybridizations of this spin, [l*norb + m]
 };
 
 // Everything the H.v kernel needs besides the vectors.
 struct HvOp {
     int nup, ndn, norb;
+    int nvup;  // hybridization elements of the up spin (norb*nbath)
     SpinTab up, dn;
     double U, Up, Jp;
 };
@@ -67,6 +76,7 @@ struct HvArgs {
 int launch_spin_table(hipStream_t s, int nsite, int norb, int nbath, int ne, const double *em_s,
                       const double *ek_s, const double *v_s, double mu, double UpmJ,
                       const SpinTab &t);
+int launch_compress(hipStream_t s, const SpinTab &t);
 template <typename T>
 int launch_hv(hipStream_t s, const HvOp &op, int mode, int cols, bool acc64, const HvArgs &a);
 template <typename T>

@@ -143,15 +143,15 @@ __global__ __launch_bounds__(kBlock) void k_spin_table(int nsite, int norb, int 
             const uint32_t between = p & ((1u << j) - 1u) & ~((2u << i) - 1u);
             const double val = (__popc(between) & 1) ? -v : v;
             const uint32_t c = rank_of(q);
-            t.hd[(long long)slot * t.n + r] = HopD{c, 0u, val};
-            t.hf[(long long)slot * t.n + r] = HopF{c, (float)val};
+            t.hd[(long long)r * t.K + slot] = HopD{c, (uint32_t)(l * norb + m) | ((val < 0) << 7), val};
+            t.hf[(long long)r * t.K + slot] = HopF{c, (float)val};
             ++slot;
         }
     }
     t.cnt[r] = (uint8_t)slot;
     for (int k = slot; k < t.K; ++k) {
-        t.hd[(long long)k * t.n + r] = HopD{(uint32_t)r, 0u, 0.0};
-        t.hf[(long long)k * t.n + r] = HopF{(uint32_t)r, 0.0f};
+        t.hd[(long long)r * t.K + k] = HopD{(uint32_t)r, 64u, 0.0};
+        t.hf[(long long)r * t.K + k] = HopF{(uint32_t)r, 0.0f};
     }
     atomicMax(t.maxcnt, slot);
     // one-spin diagonal energy
@@ -164,6 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_spin_table(int nsite, int norb, int 
     for (int l = 0; l < nbath; ++l)
         if ((p >> (norb + l)) & 1u) e += ek_s[l];
     t.energy[r] = e;
+    t.energyf[r] = (float)e;
     t.mask[r] = (uint8_t)(p & ((1u << norb) - 1u));
     // orbital moves dst <- src (Kanamori terms), sign of modes strictly between
     if (norb >= 2) {
@@ -179,6 +180,25 @@ __global__ __launch_bounds__(kBlock) void k_spin_table(int nsite, int norb, int 
                 t.orb[(long long)(dst * norb + src) * t.n + r] = code;
             }
     }
+}
+
+// Compressed [n][KC] copy of the first KC slots: 16-bit target rank and
+// 8-bit element index (bit 7 = negative sign, 64 = zero element).
+__global__ __launch_bounds__(kBlock) void k_compress(SpinTab t)
+{
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= t.n) return;
+    for (int k = 0; k < t.KC; ++k) {
+        const HopD h = t.hd[(long long)r * t.K + k];
+        t.ccol[(long long)r * t.KC + k] = (uint16_t)h.col;
+        t.cidx[(long long)r * t.KC + k] = (uint8_t)h.pad;
+    }
+}
+
+int launch_compress(hipStream_t s, const SpinTab &t)
+{
+    hipLaunchKernelGGL(k_compress, dim3((t.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, t);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_spin_table(hipStream_t s, int nsite, int norb, int nbath, int ne, const double *em_s,
@@ -253,7 +273,7 @@ __global__ __launch_bounds__(kHvBlock) void k_hv(HvOp op, HvArgs a)
             s_md[c] = idn < ndn ? op.dn.mask[idn] : 0;
         }
         if (k < cnt) {
-            const Hop h = dtab[(long long)k * ndn + idn];
+            const Hop h = dtab[(long long)idn * op.dn.K + k];
             s_col[c][k] = (int)h.col;
             s_val[c][k] = (ACC)h.val;
         }
@@ -287,7 +307,6 @@ __global__ __launch_bounds__(kHvBlock) void k_hv(HvOp op, HvArgs a)
     }
     T *y = static_cast<T *>(a.y);
     const T *pv = static_cast<const T *>(a.p);
-    const int Ku = op.up.K;
 
     for (int iup = threadIdx.x; iup < nup; iup += kHvBlock) {
         ACC acc[C];
@@ -298,28 +317,34 @@ __global__ __launch_bounds__(kHvBlock) void k_hv(HvOp op, HvArgs a)
         for (int c = 0; c < C; ++c)
             acc[c] = (ACC)(eu + s_ed[c] + cross_energy(mu, s_md[c], op.U, op.Up)) * (ACC)xr.v[c];
         if constexpr (KH > 0) {
-            // up hops: all table records, then all LDS rows, then the FMAs
+            // up hops: every table slot < KH exists (tables hold >= kMinSlots
+            // slots, unused ones point at the row itself with a zero element),
+            // so all records, then all LDS rows, are loaded before any FMA
             Hop h[KH];
 #pragma unroll
-            for (int k = 0; k < KH; ++k)
-                if (k < Ku) h[k] = utab[(long long)k * nup + iup];
-                else h[k] = Hop{};
+            for (int k = 0; k < KH; ++k) h[k] = utab[(long long)iup * op.up.K + k];
             Row<T, C> src[KH];
 #pragma unroll
-            for (int k = 0; k < KH; ++k)
-                src[k] = *reinterpret_cast<const Row<T, C> *>(xs + (k < Ku ? h[k].col : iup) * C);
+            for (int k = 0; k < KH; ++k) src[k] = *reinterpret_cast<const Row<T, C> *>(xs + h[k].col * C);
 #pragma unroll
             for (int k = 0; k < KH; ++k)
 #pragma unroll
                 for (int c = 0; c < C; ++c) acc[c] += (ACC)h[k].val * (ACC)src[k].v[c];
-            // down hops: neighbour columns from global memory
+            // down hops: neighbour columns from global memory; the column base
+            // is wave-uniform, so each load is scalar base + lane offset
 #pragma unroll
             for (int c = 0; c < C; ++c) {
-                const int cnt = s_cnt[c];
+                const int cnt = __builtin_amdgcn_readfirstlane(s_cnt[c]);
                 T g[KH];
 #pragma unroll
-                for (int k = 0; k < KH; ++k)
-                    g[k] = k < cnt ? x[(long long)s_col[c][k] * ld + iup] : T(0);
+                for (int k = 0; k < KH; ++k) {
+                    if (k < cnt) {
+                        const int col = __builtin_amdgcn_readfirstlane(s_col[c][k]);
+                        g[k] = x[(long long)col * ld + iup];
+                    } else {
+                        g[k] = T(0);
+                    }
+                }
 #pragma unroll
                 for (int k = 0; k < KH; ++k)
                     if (k < cnt) acc[c] += s_val[c][k] * (ACC)g[k];
@@ -327,7 +352,7 @@ __global__ __launch_bounds__(kHvBlock) void k_hv(HvOp op, HvArgs a)
         } else {
             const int cu = op.up.cnt[iup];
             for (int k = 0; k < cu; ++k) {
-                const Hop hk = utab[(long long)k * nup + iup];
+                const Hop hk = utab[(long long)iup * op.up.K + k];
                 const Row<T, C> src = *reinterpret_cast<const Row<T, C> *>(xs + hk.col * C);
 #pragma unroll
                 for (int c = 0; c < C; ++c) acc[c] += (ACC)hk.val * (ACC)src.v[c];
@@ -395,6 +420,168 @@ __global__ __launch_bounds__(kHvBlock) void k_hv(HvOp op, HvArgs a)
     }
 }
 
+template <typename ACC>
+__device__ __forceinline__ ACC energy_of(const SpinTab &t, int r);
+template <>
+__device__ __forceinline__ float energy_of<float>(const SpinTab &t, int r) { return t.energyf[r]; }
+template <>
+__device__ __forceinline__ double energy_of<double>(const SpinTab &t, int r) { return t.energy[r]; }
+
+// Single-orbital H.v (norb == 1): the diagonal is E_up(iup) + E_dn(idn) +
+// U n_up n_dn with n the impurity bit, and both spins hop through KH-slot
+// tables whose unused slots are self references with a zero element.  The
+// down-hop column offsets and elements of the block's C columns are
+// wave-uniform and loaded once into scalar registers; every slot is read
+// unconditionally, so the row loop has no branches.
+template <typename T, typename ACC, int C, int MODE, int KH>
+__global__ __launch_bounds__(kHvBlock) void k_hv1(HvOp op, HvArgs a)
+{
+    if (a.stop && *a.stop) return;
+    using Hop = typename HopOf<ACC>::type;
+    extern __shared__ __align__(64) unsigned char smem[];
+    T *xs = reinterpret_cast<T *>(smem);
+    __shared__ Row<ACC, C> s_dval[KH];  // down-hop elements, slot-major
+    __shared__ ACC s_v[2 * (kMaxHops + 1)];  // up-spin elements: [idx] = +V, [65 + idx] = -V, idx 64 = 0
+
+    const int nup = op.nup, ndn = op.ndn;
+    const int idn0 = blockIdx.x * C;
+    const T *x = static_cast<const T *>(a.x);
+    const unsigned char *xb = static_cast<const unsigned char *>(a.x);
+    const Hop *dtab = HopOf<ACC>::tab(op.dn);
+    for (int i = threadIdx.x; i <= kMaxHops; i += kHvBlock) {
+        const ACC v = i < op.nvup ? (ACC)op.up.vsrc[i] : ACC(0);
+        s_v[i] = v;
+        s_v[kMaxHops + 1 + i] = -v;
+    }
+
+    // byte offsets of the neighbour columns (wave-uniform, scalar registers)
+    uint32_t coff[C][KH];
+    ACC ed0[C], edU[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        // columns past ndn (last block) repeat column ndn-1 and are not stored
+        const int idn = idn0 + c < ndn ? idn0 + c : ndn - 1;
+        const Hop *hr = dtab + (long long)idn * op.dn.K;
+#pragma unroll
+        for (int k = 0; k < KH; ++k) {
+            const Hop h = hr[k];
+            coff[c][k] = h.col * (uint32_t)nup * (uint32_t)sizeof(T);
+            if (threadIdx.x == 0) s_dval[k].v[c] = (ACC)h.val;
+        }
+        const ACC e = energy_of<ACC>(op.dn, idn);
+        ed0[c] = e;
+        edU[c] = op.dn.mask[idn] ? e + (ACC)op.U : e;
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int idn = idn0 + c < ndn ? idn0 + c : ndn - 1;
+        const T *xc = x + (long long)idn * nup;
+        for (int i = threadIdx.x; i < nup; i += kHvBlock) xs[i * C + c] = xc[i];
+    }
+    __syncthreads();
+
+    double sums[3] = {0.0, 0.0, 0.0};
+    double sx = 1.0, bsp = 0.0;
+    if constexpr (MODE == HV_LANCZOS) {
+        sx = *a.sx;
+        if (a.p) bsp = (*a.beta) * (*a.sp);
+    }
+    T *y = static_cast<T *>(a.y);
+    const T *pv = static_cast<const T *>(a.p);
+    for (int iup = threadIdx.x; iup < nup; iup += kHvBlock) {
+        const uint32_t ib = (uint32_t)iup * (uint32_t)sizeof(T);
+        // the row's KH compressed hop records are contiguous: wide loads
+        const Row<uint16_t, KH> hc = *reinterpret_cast<const Row<uint16_t, KH> *>(op.up.ccol + (long long)iup * KH);
+        const Row<uint8_t, KH> hi = *reinterpret_cast<const Row<uint8_t, KH> *>(op.up.cidx + (long long)iup * KH);
+        T g[C][KH];
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int k = 0; k < KH; ++k) g[c][k] = *reinterpret_cast<const T *>(xb + (coff[c][k] + ib));
+        const ACC eu = energy_of<ACC>(op.up, iup);
+        const bool mu = op.up.mask[iup] != 0;
+        const Row<T, C> xr = *reinterpret_cast<const Row<T, C> *>(xs + iup * C);
+        ACC acc[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) acc[c] = (eu + (mu ? edU[c] : ed0[c])) * (ACC)xr.v[c];
+#pragma unroll
+        for (int k = 0; k < KH; ++k) {
+            const uint32_t ix = hi.v[k];
+            const ACC val = s_v[(ix & 0x7Fu) + ((ix >> 7) ? kMaxHops + 1 : 0)];
+            const Row<T, C> src = *reinterpret_cast<const Row<T, C> *>(xs + (uint32_t)hc.v[k] * C);
+#pragma unroll
+            for (int c = 0; c < C; ++c) acc[c] += val * (ACC)src.v[c];
+        }
+#pragma unroll
+        for (int k = 0; k < KH; ++k) {
+            const Row<ACC, C> dv = s_dval[k];
+#pragma unroll
+            for (int c = 0; c < C; ++c) acc[c] += dv.v[c] * (ACC)g[c][k];
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            if (idn0 + c >= ndn) break;
+            const long long e = (long long)(idn0 + c) * nup + iup;
+            if constexpr (MODE == HV_PLAIN) {
+                y[e] = a.accumulate ? (T)((ACC)y[e] + acc[c]) : (T)acc[c];
+            } else if constexpr (MODE == HV_LANCZOS) {
+                double r = sx * (double)acc[c];
+                if (pv) r -= bsp * (double)pv[e];
+                const T rt = (T)r;
+                y[e] = rt;
+                sums[0] += (double)xr.v[c] * (double)rt;
+            } else {
+                const double xv = (double)xr.v[c], hv = (double)acc[c];
+                sums[0] += xv * hv;
+                sums[1] += hv * hv;
+                sums[2] += xv * xv;
+            }
+        }
+    }
+    if constexpr (MODE == HV_LANCZOS) {
+        double mine[1] = {sums[0]}, tot[1];
+        if (reduce_last<1, kHvBlock>(mine, a.partial, a.ticket, tot) && threadIdx.x == 0)
+            a.result[0] = sx * tot[0];
+    } else if constexpr (MODE == HV_RQ) {
+        double tot[3];
+        if (reduce_last<3, kHvBlock>(sums, a.partial, a.ticket, tot) && threadIdx.x == 0) {
+            a.result[0] = tot[0];
+            a.result[1] = tot[1];
+            a.result[2] = tot[2];
+        }
+    }
+}
+
+template <typename T, typename ACC, int C, int KH>
+static int launch_hv1_c(hipStream_t s, const HvOp &op, int mode, const HvArgs &a)
+{
+    const size_t lds = (size_t)op.nup * C * sizeof(T);
+    const int grid = (op.ndn + C - 1) / C;
+    void (*f)(HvOp, HvArgs);
+    if (mode == HV_PLAIN) f = k_hv1<T, ACC, C, HV_PLAIN, KH>;
+    else if (mode == HV_LANCZOS) f = k_hv1<T, ACC, C, HV_LANCZOS, KH>;
+    else f = k_hv1<T, double, C, HV_RQ, KH>;
+    if (lds > 64 * 1024) {
+        if (hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds) != hipSuccess)
+            return -1;
+    }
+    hipLaunchKernelGGL(f, dim3(grid), dim3(kHvBlock), lds, s, op, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename T, typename ACC, int KH>
+static int launch_hv1_k(hipStream_t s, const HvOp &op, int mode, int cols, const HvArgs &a)
+{
+    switch (cols) {
+    case 1: return launch_hv1_c<T, ACC, 1, KH>(s, op, mode, a);
+    case 2: return launch_hv1_c<T, ACC, 2, KH>(s, op, mode, a);
+    case 4: return launch_hv1_c<T, ACC, 4, KH>(s, op, mode, a);
+    case 8: return launch_hv1_c<T, ACC, 8, KH>(s, op, mode, a);
+    default: return -2;
+    }
+}
+
 // Dynamic LDS budget of one workgroup: gfx950 has 160 KiB per CU; keep 8 KiB
 // for the static per-column hop lists and reduction scratch.
 constexpr size_t kLdsBudget = 152 * 1024;
@@ -435,6 +622,8 @@ static int launch_hv_a(hipStream_t s, const HvOp &op, int mode, int cols, const 
     const bool kan = op.norb >= 2;
     const int km = op.up.kmax > op.dn.kmax ? op.up.kmax : op.dn.kmax;
     if (kan) return launch_hv_k<T, ACC, true, 0>(s, op, mode, cols, a);
+    if (op.norb == 1 && op.up.KC == 8 && km <= 8) return launch_hv1_k<T, ACC, 8>(s, op, mode, cols, a);
+    if (op.norb == 1 && op.up.KC == 16 && km <= 16) return launch_hv1_k<T, ACC, 16>(s, op, mode, cols, a);
     if (km <= 8) return launch_hv_k<T, ACC, false, 8>(s, op, mode, cols, a);
     if (km <= 16) return launch_hv_k<T, ACC, false, 16>(s, op, mode, cols, a);
     return launch_hv_k<T, ACC, false, 0>(s, op, mode, cols, a);
@@ -445,8 +634,12 @@ int launch_hv(hipStream_t s, const HvOp &op, int mode, int cols, bool acc64, con
 {
     while (cols > 1 && (size_t)op.nup * cols * sizeof(T) > kLdsBudget) cols >>= 1;
     if ((size_t)op.nup * cols * sizeof(T) > kLdsBudget) return -3;
-    if (sizeof(T) == 8 || acc64 || mode == HV_RQ) return launch_hv_a<T, double>(s, op, mode, cols, a);
-    return launch_hv_a<T, float>(s, op, mode, cols, a);
+    if constexpr (sizeof(T) == 8) {
+        return launch_hv_a<T, double>(s, op, mode, cols, a);
+    } else {
+        if (acc64 || mode == HV_RQ) return launch_hv_a<T, double>(s, op, mode, cols, a);
+        return launch_hv_a<T, float>(s, op, mode, cols, a);
+    }
 }
 template int launch_hv<float>(hipStream_t, const HvOp &, int, int, bool, const HvArgs &);
 template int launch_hv<double>(hipStream_t, const HvOp &, int, int, bool, const HvArgs &);
