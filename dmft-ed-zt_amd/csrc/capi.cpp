@@ -23,6 +23,7 @@ using namespace edg;
 namespace {
 
 constexpr long long kPad = 64;  // element alignment of stored vectors
+constexpr long long kPartialHalf = 65536;  // doubles per half of the partial-sum buffer
 
 long long round_up(long long n, long long m) { return (n + m - 1) / m * m; }
 
@@ -61,6 +62,7 @@ struct edg_ctx {
     unsigned long long budget = 0;
     int cols = 4;
     bool acc64 = false;
+    int ncu = 0;
 
     std::vector<double> em, ek, vmk;
     unsigned long long version = 0;
@@ -233,6 +235,8 @@ int make_op(edg_ctx *c, int nu, int nd, HvOp *op)
     op->ndn = td->n;
     op->norb = c->P.norb;
     op->nvup = c->P.norb * c->P.nbath;
+    op->ncu = c->ncu;
+    op->sched = 0;
     op->up = *tu;
     op->dn = *td;
     op->U = c->P.U;
@@ -245,7 +249,6 @@ HvArgs base_args(edg_ctx *c)
 {
     HvArgs a{};
     a.partial = c->partial;
-    a.ticket = c->ticket;
     return a;
 }
 
@@ -268,25 +271,35 @@ int run_lanczos(edg_ctx *c, const HvOp &op, T *base, long long stride, bool stor
     HIPCHK(c, hipMemcpyAsync(S.flags, fl, sizeof(fl), hipMemcpyHostToDevice, c->stream));
     if (launch_norm<T>(c->stream, slot(0), D, c->partial, c->ticket, &S.misc[0], &S.scale[0], nullptr))
         return fail(c, EDG_ERR_HIP, "norm launch failed");
+    // H.v partials and axpy partials live in separate halves of the buffer:
+    // step j's H.v reads step j-1's axpy partials while writing its own
+    double *hv_part = c->partial, *ax_part = c->partial + kPartialHalf;
+    int ax_n = 0;
     for (int j = 0; j < maxn; ++j) {
         HvArgs ha = base_args(c);
         ha.x = slot(j);
         ha.y = slot(j + 1);
         ha.p = j > 0 ? slot(j - 1) : nullptr;
-        ha.sx = &S.scale[j];
-        ha.sp = j > 0 ? &S.scale[j - 1] : nullptr;
-        ha.beta = &S.beta[j];
-        ha.result = &S.alpha[j];
+        ha.partial = hv_part;
+        ha.step = j;
+        ha.prev_part = ax_part;
+        ha.prev_n = ax_n;
+        ha.scale = S.scale;
+        ha.beta = S.beta;
         ha.stop = S.flags;
-        if (launch_hv<T>(c->stream, op, HV_LANCZOS, c->cols, c->acc64, ha))
-            return fail(c, EDG_ERR_HIP, "H.v launch failed");
+        ha.nstep = S.flags + 1;
+        const int hv_n = launch_hv<T>(c->stream, op, HV_LANCZOS, c->cols, c->acc64, ha);
+        if (hv_n < 0) return fail(c, EDG_ERR_HIP, "H.v launch failed");
         ++c->n_hv;
         ++c->n_steps;
-        if (j == maxn - 1) break;
-        if (launch_axpy_norm<T>(c->stream, slot(j + 1), slot(j), D, &S.alpha[j], &S.scale[j],
-                                c->partial, c->ticket, &S.beta[j + 1], &S.scale[j + 1], S.flags,
-                                S.flags + 1, j))
-            return fail(c, EDG_ERR_HIP, "axpy launch failed");
+        if (j == maxn - 1) {
+            if (launch_alpha(c->stream, hv_part, hv_n, &S.scale[j], &S.alpha[j], S.flags))
+                return fail(c, EDG_ERR_HIP, "alpha launch failed");
+            break;
+        }
+        ax_n = launch_axpy_norm<T>(c->stream, slot(j + 1), slot(j), D, hv_part, hv_n, S.scale, S.alpha,
+                                   j, ax_part, S.flags);
+        if (ax_n < 0) return fail(c, EDG_ERR_HIP, "axpy launch failed");
     }
     std::vector<double> ha(maxn), hb(maxn + 1), hs(maxn + 1);
     int hf[2];
@@ -377,7 +390,7 @@ int diag_sector(edg_ctx *c, int nu, int nd, unsigned long long seed, T *gs_out, 
     HvArgs ha = base_args(c);
     ha.x = gs_out;
     ha.result = &misc[1];
-    if (launch_hv<T>(c->stream, op, HV_RQ, c->cols, true, ha)) return fail(c, EDG_ERR_HIP, "RQ launch failed");
+    if (launch_hv<T>(c->stream, op, HV_RQ, c->cols, true, ha) < 0) return fail(c, EDG_ERR_HIP, "RQ launch failed");
     ++c->n_hv;
     double sums[3];
     HIPCHK(c, hipMemcpyAsync(sums, &misc[1], sizeof(sums), hipMemcpyDeviceToHost, c->stream));
@@ -551,7 +564,7 @@ int hx_t(edg_ctx *c, int nu, int nd, const double *x, double *y, int py)
     a.x = dx;
     a.y = dy;
     a.accumulate = py;
-    if (launch_hv<T>(c->stream, op, HV_PLAIN, c->cols, c->acc64, a)) return fail(c, EDG_ERR_HIP, "H.v launch failed");
+    if (launch_hv<T>(c->stream, op, HV_PLAIN, c->cols, c->acc64, a) < 0) return fail(c, EDG_ERR_HIP, "H.v launch failed");
     ++c->n_hv;
     return download<T>(c, y, dy, D);
 }
@@ -608,10 +621,10 @@ int time_hv_t(edg_ctx *c, int nu, int nd, int warmup, int reps, double *ms, doub
     a.x = dx;
     a.y = dy;
     for (int i = 0; i < warmup; ++i)
-        if (launch_hv<T>(c->stream, op, HV_PLAIN, c->cols, c->acc64, a)) return fail(c, EDG_ERR_HIP, "H.v launch failed");
+        if (launch_hv<T>(c->stream, op, HV_PLAIN, c->cols, c->acc64, a) < 0) return fail(c, EDG_ERR_HIP, "H.v launch failed");
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     for (int i = 0; i < reps; ++i)
-        if (launch_hv<T>(c->stream, op, HV_PLAIN, c->cols, c->acc64, a)) return fail(c, EDG_ERR_HIP, "H.v launch failed");
+        if (launch_hv<T>(c->stream, op, HV_PLAIN, c->cols, c->acc64, a) < 0) return fail(c, EDG_ERR_HIP, "H.v launch failed");
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev[1]));
     float t = 0;
@@ -677,7 +690,9 @@ int edg_create(const edg_params *p, int device, edg_ctx **out)
     if (c->P.Jp < 0) c->P.Jp = c->P.J;                  // dmft_params.f90:94-96
     if (c->P.maxnstep_green <= 0) c->P.maxnstep_green = c->P.maxnstep;
     c->device = device;
-    c->cols = p->precision == EDG_FP32 ? 4 : 2;
+    // measured on MI355X at Ns=14 (sector (7,7)): one column per workgroup
+    // is the fastest schedule for both storage precisions
+    c->cols = 1;
     // Kanamori models (norb >= 2) accumulate FP32-vector row sums in FP64: the
     // spin-flip / pair-hop terms make FP32 sums the leading error in G(i w)
     c->acc64 = p->precision == EDG_FP32 && p->norb >= 2;
@@ -688,6 +703,8 @@ int edg_create(const edg_params *p, int device, edg_ctx **out)
     };
     if (hipSetDevice(device) != hipSuccess) return bad();
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bad();
+    if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return bad();
     if (hipMalloc((void **)&c->partial, sizeof(double) * 16 * 8192 + 64) != hipSuccess) return bad();
     if (hipMalloc((void **)&c->ticket, 256) != hipSuccess) return bad();
     if (hipMemset(c->ticket, 0, 256) != hipSuccess) return bad();
@@ -737,7 +754,8 @@ int edg_set_memory_budget(edg_ctx *c, unsigned long long bytes)
 
 int edg_set_tuning(edg_ctx *c, int cols_per_block, int fp64_accumulate)
 {
-    if (!c || (cols_per_block != 1 && cols_per_block != 2 && cols_per_block != 4 && cols_per_block != 8))
+    if (!c || (cols_per_block != 1 && cols_per_block != 2 && cols_per_block != 4 && cols_per_block != 8 &&
+               cols_per_block != 16 && cols_per_block != 32 && cols_per_block != 64))
         return EDG_ERR_ARG;
     c->cols = cols_per_block;
     c->acc64 = fp64_accumulate != 0;

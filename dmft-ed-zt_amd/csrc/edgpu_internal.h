@@ -52,6 +52,8 @@ struct SpinTab {
 struct HvOp {
     int nup, ndn, norb;
     int nvup;  // hybridization elements of the up spin (norb*nbath)
+    int ncu;   // compute units of the device (persistent grid size)
+    int sched; // 0: one workgroup per column group; 16/32/64: persistent, C = sched/16
     SpinTab up, dn;
     double U, Up, Jp;
 };
@@ -59,17 +61,22 @@ struct HvOp {
 enum HvMode { HV_PLAIN = 0, HV_LANCZOS = 1, HV_RQ = 2 };
 
 struct HvArgs {
-    const void *x;        // input vector (T)
-    void *y;              // output vector (T)
-    const void *p;        // previous Lanczos vector (T) or null
-    const double *sx;     // scale of x (Lanczos) or null
-    const double *sp;     // scale of p
-    const double *beta;   // beta_j
-    int accumulate;       // HV_PLAIN: y += Hx
-    double *partial;      // per-block partial sums
-    unsigned *ticket;     // last-block counter (returns to 0)
-    double *result;       // finalized sums
-    const int *stop;      // skip when *stop != 0
+    const void *x;          // input vector (T)
+    void *y;                // output vector (T)
+    const void *p;          // previous Lanczos vector (T) or null
+    int accumulate;         // HV_PLAIN: y += Hx
+    double *partial;        // per-block partial sums written by this kernel
+    double *result;         // HV_RQ: the three sums
+    // HV_LANCZOS step j: s_j = 1/beta_j with beta_j^2 = sum of prev_part
+    // (the axpy partials of step j-1; step 0 uses scale[0]); workgroup 0
+    // records beta[j] and scale[j]; beta_j < 1e-16 stops the chain
+    int step;
+    const double *prev_part;
+    int prev_n;
+    double *scale;
+    double *beta;
+    int *stop;              // skip when *stop != 0
+    int *nstep;
 };
 
 // ---- launchers (kernels.hip) ----
@@ -77,15 +84,21 @@ int launch_spin_table(hipStream_t s, int nsite, int norb, int nbath, int ne, con
                       const double *ek_s, const double *v_s, double mu, double UpmJ,
                       const SpinTab &t);
 int launch_compress(hipStream_t s, const SpinTab &t);
+// Returns the number of workgroups (= partial sums written) or < 0 on error.
 template <typename T>
 int launch_hv(hipStream_t s, const HvOp &op, int mode, int cols, bool acc64, const HvArgs &a);
+// alpha[step] = scale[step] * sum(partial[0..nblk)) unless stopped.
+int launch_alpha(hipStream_t s, const double *partial, int nblk, const double *scale_j,
+                 double *alpha_j, int *stop);
 template <typename T>
 int launch_norm(hipStream_t s, const T *x, long long n, double *partial, unsigned *ticket,
                 double *norm_out, double *scale_out, const int *stop);
+// r <- r - alpha_j s_j u_j with alpha_j = s_j * sum(hv_part) (recorded in
+// alpha[step]); writes partial sums of |r|^2.  Returns the workgroup count.
 template <typename T>
-int launch_axpy_norm(hipStream_t s, T *r, const T *u, long long n, const double *alpha,
-                     const double *su, double *partial, unsigned *ticket, double *beta_out,
-                     double *scale_out, int *stop, int *nstep, int step_index);
+int launch_axpy_norm(hipStream_t s, T *r, const T *u, long long n, const double *hv_part,
+                     int hv_n, const double *scale, double *alpha, int step, double *partial,
+                     int *stop);
 template <typename T>
 int launch_start(hipStream_t s, T *x, long long n, unsigned long long seed);
 template <typename T>

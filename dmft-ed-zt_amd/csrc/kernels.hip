@@ -49,41 +49,147 @@ __device__ __forceinline__ void block_sums(double (&v)[NS], double (&out)[NS])
     __syncthreads();
 }
 
-__device__ __forceinline__ double load_agent(const double *p)
-{
-    unsigned long long b = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-    return __longlong_as_double((long long)b);
-}
-
-// Publish this block's partial sums; the last block to arrive reduces all
-// partials in a fixed order into tot (thread 0) and returns true.
+// Write this block's NS partial sums to partial[blockIdx.x*NS + s].  A
+// separate one-workgroup finisher kernel combines them in block order: the
+// kernel boundary makes the partials visible across XCDs without the
+// device-scope release fences a last-block-done scheme would need in every
+// workgroup (each of which writes back the XCD's L2).
 template <int NS, int BS = kBlock>
-__device__ bool reduce_last(double (&mine)[NS], double *partial, unsigned *ticket,
-                            double (&tot)[NS])
+__device__ __forceinline__ void block_partial(double (&mine)[NS], double *partial)
 {
-    __shared__ int s_last;
     double blk[NS];
     block_sums<NS, BS>(mine, blk);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
         for (int s = 0; s < NS; ++s) partial[(long long)blockIdx.x * NS + s] = blk[s];
-        __threadfence();
-        unsigned t = atomicAdd(ticket, 1u);
-        s_last = (t == gridDim.x - 1);
-    }
+}
+
+// Sum of n partials in a fixed order: thread t adds partials t, t+BS, ...,
+// then a fixed block tree; every workgroup of size BS gets the same value.
+template <int BS>
+__device__ __forceinline__ double sum_partials(const double *part, int n)
+{
+    __shared__ double s_sum;
+    double v[1] = {0.0}, out[1];
+    for (int i = threadIdx.x; i < n; i += BS) v[0] += part[i];
+    block_sums<1, BS>(v, out);
+    if (threadIdx.x == 0) s_sum = out[0];
     __syncthreads();
-    if (!s_last) return false;
-    __threadfence();
-    double v[NS];
+    return s_sum;
+}
+
+// Lanczos scales of step j for the H.v kernel: returns false when the chain
+// has broken down (beta_j < 1e-16); workgroup 0 records beta_j and s_j.
+template <int BS>
+__device__ __forceinline__ bool lanczos_scales(const HvArgs &a, double &sx, double &bsp)
+{
+    if (a.step == 0) {
+        sx = a.scale[0];
+        bsp = 0.0;
+        return true;
+    }
+    const double bj = sqrt(sum_partials<BS>(a.prev_part, a.prev_n));
+    const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+    if (bj < 1.0e-16) {
+        if (first) {
+            a.beta[a.step] = bj;
+            *a.stop = 1;
+            *a.nstep = a.step;
+        }
+        return false;
+    }
+    sx = 1.0 / bj;
+    bsp = bj * a.scale[a.step - 1];
+    if (first) {
+        a.beta[a.step] = bj;
+        a.scale[a.step] = sx;
+    }
+    return true;
+}
+
+enum FinMode { FIN_ALPHA = 0, FIN_BETA = 1, FIN_NORM = 2, FIN_COPY = 3, FIN_OBS = 4 };
+
+struct FinArgs {
+    int mode;
+    int nblk;
+    const double *sx;   // FIN_ALPHA: alpha = sx * sum
+    double *out0;       // alpha / beta / norm / sums
+    double *out1;       // scale (1/beta, 1/norm)
+    int *stop;          // FIN_BETA: set when beta < 1e-16; FIN_ALPHA/BETA skip when set
+    int *nstep;
+    int step;
+    int norb;           // FIN_OBS
+};
+
+constexpr int kFinBlock = 1024;
+
+template <int NS>
+__global__ __launch_bounds__(kFinBlock) void k_finish(const double *partial, FinArgs f)
+{
+    if ((f.mode == FIN_ALPHA || f.mode == FIN_BETA) && f.stop && *f.stop) return;
+    double v[NS], tot[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) v[s] = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += BS)
+    for (int b = threadIdx.x; b < f.nblk; b += kFinBlock)
 #pragma unroll
-        for (int s = 0; s < NS; ++s) v[s] += load_agent(&partial[(long long)b * NS + s]);
-    block_sums<NS, BS>(v, tot);
-    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
+        for (int s = 0; s < NS; ++s) v[s] += partial[(long long)b * NS + s];
+    block_sums<NS, kFinBlock>(v, tot);
+    if (threadIdx.x != 0) return;
+    switch (f.mode) {
+    case FIN_ALPHA:
+        f.out0[0] = (*f.sx) * tot[0];
+        break;
+    case FIN_BETA: {
+        const double b = sqrt(tot[0]);
+        f.out0[0] = b;
+        f.out1[0] = b > 0.0 ? 1.0 / b : 0.0;
+        if (b < 1.0e-16) {
+            *f.stop = 1;
+            *f.nstep = f.step + 1;
+        }
+        break;
+    }
+    case FIN_NORM: {
+        const double nr = sqrt(tot[0]);
+        if (f.out0) f.out0[0] = nr;
+        if (f.out1) f.out1[0] = 1.0 / nr;
+        break;
+    }
+    case FIN_COPY:
+#pragma unroll
+        for (int s = 0; s < NS; ++s) f.out0[s] = tot[s];
+        break;
+    default:  // FIN_OBS: sums laid out [m], [kMaxNorb+m], [2 kMaxNorb+m]
+        for (int m = 0; m < f.norb; ++m) {
+            f.out0[m] = tot[m];
+            f.out0[f.norb + m] = tot[kMaxNorb + m];
+            f.out0[2 * f.norb + m] = tot[2 * kMaxNorb + m];
+        }
+    }
+}
+
+template <int NS>
+static int launch_finish(hipStream_t s, const double *partial, const FinArgs &f)
+{
+    hipLaunchKernelGGL(k_finish<NS>, dim3(1), dim3(kFinBlock), 0, s, partial, f);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Finisher of the H.v reductions: alpha for HV_LANCZOS, the three sums for HV_RQ.
+static int finish_hv(hipStream_t s, int mode, int nblk, const HvArgs &a)
+{
+    if (mode == HV_RQ)
+        if (launch_finish<3>(s, a.partial, FinArgs{FIN_COPY, nblk, nullptr, a.result, nullptr,
+                                                   nullptr, nullptr, 0, 0}))
+            return -1;
+    return nblk;
+}
+
+int launch_alpha(hipStream_t s, const double *partial, int nblk, const double *scale_j,
+                 double *alpha_j, int *stop)
+{
+    return launch_finish<1>(s, partial, FinArgs{FIN_ALPHA, nblk, scale_j, alpha_j, nullptr, stop,
+                                                nullptr, 0, 0});
 }
 
 __device__ __forceinline__ uint32_t binom_tab(const uint32_t (*B)[16], int n, int k)
@@ -218,6 +324,8 @@ struct alignas(sizeof(T) * C) Row {
     T v[C];
 };
 
+__host__ __device__ constexpr size_t align16(size_t n) { return (n + 15) & ~size_t(15); }
+
 template <typename ACC>
 struct HopOf;
 template <>
@@ -302,8 +410,7 @@ __global__ __launch_bounds__(kHvBlock) void k_hv(HvOp op, HvArgs a)
     double sums[3] = {0.0, 0.0, 0.0};
     double sx = 1.0, bsp = 0.0;
     if constexpr (MODE == HV_LANCZOS) {
-        sx = *a.sx;
-        if (a.p) bsp = (*a.beta) * (*a.sp);
+        if (!lanczos_scales<kHvBlock>(a, sx, bsp)) return;
     }
     T *y = static_cast<T *>(a.y);
     const T *pv = static_cast<const T *>(a.p);
@@ -407,16 +514,10 @@ __global__ __launch_bounds__(kHvBlock) void k_hv(HvOp op, HvArgs a)
         }
     }
     if constexpr (MODE == HV_LANCZOS) {
-        double mine[1] = {sums[0]}, tot[1];
-        if (reduce_last<1, kHvBlock>(mine, a.partial, a.ticket, tot) && threadIdx.x == 0)
-            a.result[0] = sx * tot[0];  // alpha_j = <q_j, r>
+        double mine[1] = {sums[0]};
+        block_partial<1, kHvBlock>(mine, a.partial);  // alpha_j = s_j * sum (finisher)
     } else if constexpr (MODE == HV_RQ) {
-        double tot[3];
-        if (reduce_last<3, kHvBlock>(sums, a.partial, a.ticket, tot) && threadIdx.x == 0) {
-            a.result[0] = tot[0];
-            a.result[1] = tot[1];
-            a.result[2] = tot[2];
-        }
+        block_partial<3, kHvBlock>(sums, a.partial);
     }
 }
 
@@ -483,8 +584,7 @@ __global__ __launch_bounds__(kHvBlock) void k_hv1(HvOp op, HvArgs a)
     double sums[3] = {0.0, 0.0, 0.0};
     double sx = 1.0, bsp = 0.0;
     if constexpr (MODE == HV_LANCZOS) {
-        sx = *a.sx;
-        if (a.p) bsp = (*a.beta) * (*a.sp);
+        if (!lanczos_scales<kHvBlock>(a, sx, bsp)) return;
     }
     T *y = static_cast<T *>(a.y);
     const T *pv = static_cast<const T *>(a.p);
@@ -539,16 +639,191 @@ __global__ __launch_bounds__(kHvBlock) void k_hv1(HvOp op, HvArgs a)
         }
     }
     if constexpr (MODE == HV_LANCZOS) {
-        double mine[1] = {sums[0]}, tot[1];
-        if (reduce_last<1, kHvBlock>(mine, a.partial, a.ticket, tot) && threadIdx.x == 0)
-            a.result[0] = sx * tot[0];
+        double mine[1] = {sums[0]};
+        block_partial<1, kHvBlock>(mine, a.partial);  // alpha_j = s_j * sum (next kernel)
     } else if constexpr (MODE == HV_RQ) {
-        double tot[3];
-        if (reduce_last<3, kHvBlock>(sums, a.partial, a.ticket, tot) && threadIdx.x == 0) {
-            a.result[0] = tot[0];
-            a.result[1] = tot[1];
-            a.result[2] = tot[2];
+        block_partial<3, kHvBlock>(sums, a.partial);
+    }
+}
+
+// Persistent single-orbital H.v: one 1024-thread workgroup per CU copies
+// the compressed up-spin hop table into LDS once and then walks column
+// groups g = blockIdx.x, blockIdx.x + gridDim.x, ...  The table then costs
+// no L2 requests per column; what each group still reads from L2 is its own
+// C columns (staged in LDS for the up hops) and the down-hop neighbour
+// columns.  Partial sums are accumulated per workgroup over its groups in a
+// fixed order, so the reductions stay deterministic.
+constexpr int kHvpBlock = 1024;
+
+template <typename T, typename ACC, int C, int MODE, int KH>
+__global__ __launch_bounds__(kHvpBlock) void k_hvp(HvOp op, HvArgs a)
+{
+    if (a.stop && *a.stop) return;
+    extern __shared__ __align__(64) unsigned char smem[];
+    const int nup = op.nup, ndn = op.ndn;
+    // LDS layout (each region 16-byte aligned):
+    //   xs[nup*C] (T) | tcol[nup*KH] (u16) | tidx[nup*KH] (u8)
+    const size_t xsz = align16((size_t)nup * C * sizeof(T));
+    const size_t csz = align16((size_t)nup * KH * 2);
+    T *xs = reinterpret_cast<T *>(smem);
+    uint16_t *tcol = reinterpret_cast<uint16_t *>(smem + xsz);
+    uint8_t *tidx = reinterpret_cast<uint8_t *>(smem + xsz + csz);
+    __shared__ Row<ACC, C> s_dval[KH];
+    __shared__ ACC s_v[2 * (kMaxHops + 1)];
+
+    for (int i = threadIdx.x; i <= kMaxHops; i += kHvpBlock) {
+        const ACC v = i < op.nvup ? (ACC)op.up.vsrc[i] : ACC(0);
+        s_v[i] = v;
+        s_v[kMaxHops + 1 + i] = -v;
+    }
+    {   // table -> LDS: 16-byte chunks, then the element tail
+        const int nc = nup * KH;                 // entries of each array
+        const int c16 = nc * 2 / 16, i16 = nc / 16;
+        const uint4 *gc = reinterpret_cast<const uint4 *>(op.up.ccol);
+        const uint4 *gi = reinterpret_cast<const uint4 *>(op.up.cidx);
+        uint4 *lc = reinterpret_cast<uint4 *>(tcol);
+        uint4 *li = reinterpret_cast<uint4 *>(tidx);
+        for (int i = threadIdx.x; i < c16; i += kHvpBlock) lc[i] = gc[i];
+        for (int i = threadIdx.x; i < i16; i += kHvpBlock) li[i] = gi[i];
+        for (int i = c16 * 8 + threadIdx.x; i < nc; i += kHvpBlock) tcol[i] = op.up.ccol[i];
+        for (int i = i16 * 16 + threadIdx.x; i < nc; i += kHvpBlock) tidx[i] = op.up.cidx[i];
+    }
+    const T *x = static_cast<const T *>(a.x);
+    const unsigned char *xb = static_cast<const unsigned char *>(a.x);
+    const HopF *dtabf = op.dn.hf;
+    const HopD *dtabd = op.dn.hd;
+    T *y = static_cast<T *>(a.y);
+    const T *pv = static_cast<const T *>(a.p);
+    double sums[3] = {0.0, 0.0, 0.0};
+    double sx = 1.0, bsp = 0.0;
+    if constexpr (MODE == HV_LANCZOS) {
+        if (!lanczos_scales<kHvpBlock>(a, sx, bsp)) return;
+    }
+    const int ngroups = (ndn + C - 1) / C;
+
+    for (int g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        const int idn0 = g * C;
+        uint32_t coff[C][KH];
+        ACC ed0[C], edU[C];
+        __syncthreads();  // previous group is done with xs and s_dval
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int idn = idn0 + c < ndn ? idn0 + c : ndn - 1;
+#pragma unroll
+            for (int k = 0; k < KH; ++k) {
+                uint32_t col;
+                ACC val;
+                if constexpr (sizeof(ACC) == 8) {
+                    const HopD h = dtabd[(long long)idn * op.dn.K + k];
+                    col = h.col;
+                    val = h.val;
+                } else {
+                    const HopF h = dtabf[(long long)idn * op.dn.K + k];
+                    col = h.col;
+                    val = h.val;
+                }
+                coff[c][k] = col * (uint32_t)nup * (uint32_t)sizeof(T);
+                if (threadIdx.x == 0) s_dval[k].v[c] = val;
+            }
+            const ACC e = energy_of<ACC>(op.dn, idn);
+            ed0[c] = e;
+            edU[c] = op.dn.mask[idn] ? e + (ACC)op.U : e;
+            const T *xc = x + (long long)idn * nup;
+            for (int i = threadIdx.x; i < nup; i += kHvpBlock) xs[i * C + c] = xc[i];
         }
+        __syncthreads();
+
+        for (int iup = threadIdx.x; iup < nup; iup += kHvpBlock) {
+            const uint32_t ib = (uint32_t)iup * (uint32_t)sizeof(T);
+            T gv[C][KH];
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+#pragma unroll
+                for (int k = 0; k < KH; ++k) gv[c][k] = *reinterpret_cast<const T *>(xb + (coff[c][k] + ib));
+            const Row<uint16_t, KH> hc = *reinterpret_cast<const Row<uint16_t, KH> *>(tcol + iup * KH);
+            const Row<uint8_t, KH> hi = *reinterpret_cast<const Row<uint8_t, KH> *>(tidx + iup * KH);
+            const ACC eu = energy_of<ACC>(op.up, iup);
+            const bool mu = op.up.mask[iup] != 0;
+            const Row<T, C> xr = *reinterpret_cast<const Row<T, C> *>(xs + iup * C);
+            ACC acc[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) acc[c] = (eu + (mu ? edU[c] : ed0[c])) * (ACC)xr.v[c];
+#pragma unroll
+            for (int k = 0; k < KH; ++k) {
+                const uint32_t ix = hi.v[k];
+                const ACC val = s_v[(ix & 0x7Fu) + ((ix >> 7) ? kMaxHops + 1 : 0)];
+                const Row<T, C> src = *reinterpret_cast<const Row<T, C> *>(xs + (uint32_t)hc.v[k] * C);
+#pragma unroll
+                for (int c = 0; c < C; ++c) acc[c] += val * (ACC)src.v[c];
+            }
+#pragma unroll
+            for (int k = 0; k < KH; ++k) {
+                const Row<ACC, C> dv = s_dval[k];
+#pragma unroll
+                for (int c = 0; c < C; ++c) acc[c] += dv.v[c] * (ACC)gv[c][k];
+            }
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                if (idn0 + c >= ndn) break;
+                const long long e = (long long)(idn0 + c) * nup + iup;
+                if constexpr (MODE == HV_PLAIN) {
+                    y[e] = a.accumulate ? (T)((ACC)y[e] + acc[c]) : (T)acc[c];
+                } else if constexpr (MODE == HV_LANCZOS) {
+                    double r = sx * (double)acc[c];
+                    if (pv) r -= bsp * (double)pv[e];
+                    const T rt = (T)r;
+                    y[e] = rt;
+                    sums[0] += (double)xr.v[c] * (double)rt;
+                } else {
+                    const double xv = (double)xr.v[c], hv = (double)acc[c];
+                    sums[0] += xv * hv;
+                    sums[1] += hv * hv;
+                    sums[2] += xv * xv;
+                }
+            }
+        }
+    }
+    if constexpr (MODE == HV_LANCZOS) {
+        double mine[1] = {sums[0]};
+        block_partial<1, kHvpBlock>(mine, a.partial);  // alpha_j = s_j * sum (finisher)
+    } else if constexpr (MODE == HV_RQ) {
+        block_partial<3, kHvpBlock>(sums, a.partial);
+    }
+}
+
+// LDS bytes of the persistent kernel (dynamic part).
+static inline size_t hvp_lds(const HvOp &op, int C, int KH, size_t esize)
+{
+    return align16((size_t)op.nup * C * esize) + align16((size_t)op.nup * KH * 2) +
+           align16((size_t)op.nup * KH);
+}
+
+template <typename T, typename ACC, int C, int KH>
+static int launch_hvp_c(hipStream_t s, const HvOp &op, int mode, const HvArgs &a)
+{
+    const size_t lds = hvp_lds(op, C, KH, sizeof(T));
+    const int ngroups = (op.ndn + C - 1) / C;
+    const int grid = ngroups < op.ncu ? ngroups : op.ncu;
+    void (*f)(HvOp, HvArgs);
+    if (mode == HV_PLAIN) f = k_hvp<T, ACC, C, HV_PLAIN, KH>;
+    else if (mode == HV_LANCZOS) f = k_hvp<T, ACC, C, HV_LANCZOS, KH>;
+    else f = k_hvp<T, double, C, HV_RQ, KH>;
+    if (hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+        hipSuccess)
+        return -1;
+    hipLaunchKernelGGL(f, dim3(grid), dim3(kHvpBlock), lds, s, op, a);
+    if (hipGetLastError() != hipSuccess) return -1;
+    return finish_hv(s, mode, grid, a);
+}
+
+template <typename T, typename ACC, int KH>
+static int launch_hvp_k(hipStream_t s, const HvOp &op, int mode, int cols, const HvArgs &a)
+{
+    switch (cols) {
+    case 1: return launch_hvp_c<T, ACC, 1, KH>(s, op, mode, a);
+    case 2: return launch_hvp_c<T, ACC, 2, KH>(s, op, mode, a);
+    case 4: return launch_hvp_c<T, ACC, 4, KH>(s, op, mode, a);
+    default: return -2;
     }
 }
 
@@ -556,6 +831,9 @@ template <typename T, typename ACC, int C, int KH>
 static int launch_hv1_c(hipStream_t s, const HvOp &op, int mode, const HvArgs &a)
 {
     const size_t lds = (size_t)op.nup * C * sizeof(T);
+    // one workgroup per column group: the workgroup scheduler then overlaps
+    // one group's column staging with other groups' compute (measured faster
+    // on MI355X than a grid-stride loop with a few workgroups per CU)
     const int grid = (op.ndn + C - 1) / C;
     void (*f)(HvOp, HvArgs);
     if (mode == HV_PLAIN) f = k_hv1<T, ACC, C, HV_PLAIN, KH>;
@@ -567,7 +845,8 @@ static int launch_hv1_c(hipStream_t s, const HvOp &op, int mode, const HvArgs &a
             return -1;
     }
     hipLaunchKernelGGL(f, dim3(grid), dim3(kHvBlock), lds, s, op, a);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    if (hipGetLastError() != hipSuccess) return -1;
+    return finish_hv(s, mode, grid, a);
 }
 
 template <typename T, typename ACC, int KH>
@@ -601,7 +880,8 @@ static int launch_hv_c(hipStream_t s, const HvOp &op, int mode, const HvArgs &a)
             return -1;
     }
     hipLaunchKernelGGL(f, dim3(grid), dim3(kHvBlock), lds, s, op, a);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    if (hipGetLastError() != hipSuccess) return -1;
+    return finish_hv(s, mode, grid, a);
 }
 
 template <typename T, typename ACC, bool KAN, int KH>
@@ -622,6 +902,11 @@ static int launch_hv_a(hipStream_t s, const HvOp &op, int mode, int cols, const 
     const bool kan = op.norb >= 2;
     const int km = op.up.kmax > op.dn.kmax ? op.up.kmax : op.dn.kmax;
     if (kan) return launch_hv_k<T, ACC, true, 0>(s, op, mode, cols, a);
+    // persistent kernel: schedule 16/32/64 = C 1/2/4 with the table in LDS
+    if (op.norb == 1 && op.up.KC == 8 && km <= 8 && op.sched >= 16 && op.ncu > 0) {
+        const int pc = op.sched >> 4;
+        if (hvp_lds(op, pc, 8, sizeof(T)) <= kLdsBudget) return launch_hvp_k<T, ACC, 8>(s, op, mode, pc, a);
+    }
     if (op.norb == 1 && op.up.KC == 8 && km <= 8) return launch_hv1_k<T, ACC, 8>(s, op, mode, cols, a);
     if (op.norb == 1 && op.up.KC == 16 && km <= 16) return launch_hv1_k<T, ACC, 16>(s, op, mode, cols, a);
     if (km <= 8) return launch_hv_k<T, ACC, false, 8>(s, op, mode, cols, a);
@@ -630,8 +915,13 @@ static int launch_hv_a(hipStream_t s, const HvOp &op, int mode, int cols, const 
 }
 
 template <typename T>
-int launch_hv(hipStream_t s, const HvOp &op, int mode, int cols, bool acc64, const HvArgs &a)
+int launch_hv(hipStream_t s, const HvOp &op_in, int mode, int cols, bool acc64, const HvArgs &a)
 {
+    // cols 1/2/4/8: one workgroup per column group; 16/32/64: persistent
+    // workgroups with 1/2/4 columns (falls back to 1 column per workgroup)
+    HvOp op = op_in;
+    op.sched = cols >= 16 ? cols : 0;
+    if (cols >= 16) cols = 1;
     while (cols > 1 && (size_t)op.nup * cols * sizeof(T) > kLdsBudget) cols >>= 1;
     if ((size_t)op.nup * cols * sizeof(T) > kLdsBudget) return -3;
     if constexpr (sizeof(T) == 8) {
@@ -662,76 +952,72 @@ __global__ __launch_bounds__(kBlock) void k_norm(const T *x, long long n, double
                                                  double *scale_out, const int *stop)
 {
     if (stop && *stop) return;
-    double v[1] = {0.0}, tot[1];
+    double v[1] = {0.0};
     for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
          i += (long long)gridDim.x * kBlock) {
         const double t = (double)x[i];
         v[0] += t * t;
     }
-    if (reduce_last<1>(v, partial, ticket, tot) && threadIdx.x == 0) {
-        const double nr = sqrt(tot[0]);
-        if (norm_out) *norm_out = nr;
-        if (scale_out) *scale_out = 1.0 / nr;
-    }
+    block_partial<1>(v, partial);
 }
 
 template <typename T>
 int launch_norm(hipStream_t s, const T *x, long long n, double *partial, unsigned *ticket,
                 double *norm_out, double *scale_out, const int *stop)
 {
-    hipLaunchKernelGGL(k_norm<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, x, n, partial, ticket,
+    const int grid = grid_for(n);
+    hipLaunchKernelGGL(k_norm<T>, dim3(grid), dim3(kBlock), 0, s, x, n, partial, ticket,
                        norm_out, scale_out, stop);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    if (hipGetLastError() != hipSuccess) return -1;
+    return launch_finish<1>(s, partial, FinArgs{FIN_NORM, grid, nullptr, norm_out, scale_out,
+                                                nullptr, nullptr, 0, 0});
 }
 template int launch_norm<float>(hipStream_t, const float *, long long, double *, unsigned *,
                                 double *, double *, const int *);
 template int launch_norm<double>(hipStream_t, const double *, long long, double *, unsigned *,
                                  double *, double *, const int *);
 
-// r <- r - alpha_j * s_j * u_j ; beta_{j+1} = |r| ; scale_{j+1} = 1/beta.
-// A beta below 1e-16 ends the chain with nstep = step_index + 1.
+// Lanczos step j, second half: alpha_j = s_j * sum(hv_part) (every
+// workgroup reduces the H.v partials in the same order; workgroup 0 records
+// alpha_j), r <- r - alpha_j s_j u_j, partial sums of |r|^2 for beta_{j+1}.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_axpy_norm(T *r, const T *u, long long n,
-                                                      const double *alpha, const double *su,
-                                                      double *partial, unsigned *ticket,
-                                                      double *beta_out, double *scale_out,
-                                                      int *stop, int *nstep, int step_index)
+                                                      const double *hv_part, int hv_n,
+                                                      const double *scale, double *alpha,
+                                                      int step, double *partial, const int *stop)
 {
     if (*stop) return;
-    const double coef = (*alpha) * (*su);
-    double v[1] = {0.0}, tot[1];
+    const double sj = scale[step];
+    const double al = sj * sum_partials<kBlock>(hv_part, hv_n);
+    if (blockIdx.x == 0 && threadIdx.x == 0) alpha[step] = al;
+    const double coef = al * sj;
+    double v[1] = {0.0};
     for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
          i += (long long)gridDim.x * kBlock) {
         const T t = (T)((double)r[i] - coef * (double)u[i]);
         r[i] = t;
         v[0] += (double)t * (double)t;
     }
-    if (reduce_last<1>(v, partial, ticket, tot) && threadIdx.x == 0) {
-        const double b = sqrt(tot[0]);
-        *beta_out = b;
-        *scale_out = b > 0.0 ? 1.0 / b : 0.0;
-        if (b < 1.0e-16) {
-            *stop = 1;
-            *nstep = step_index + 1;
-        }
-    }
+    block_partial<1>(v, partial);
 }
 
 template <typename T>
-int launch_axpy_norm(hipStream_t s, T *r, const T *u, long long n, const double *alpha,
-                     const double *su, double *partial, unsigned *ticket, double *beta_out,
-                     double *scale_out, int *stop, int *nstep, int step_index)
+int launch_axpy_norm(hipStream_t s, T *r, const T *u, long long n, const double *hv_part,
+                     int hv_n, const double *scale, double *alpha, int step, double *partial,
+                     int *stop)
 {
-    hipLaunchKernelGGL(k_axpy_norm<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, r, u, n, alpha, su,
-                       partial, ticket, beta_out, scale_out, stop, nstep, step_index);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    int grid = grid_for(n);
+    if (grid > 1024) grid = 1024;
+    hipLaunchKernelGGL(k_axpy_norm<T>, dim3(grid), dim3(kBlock), 0, s, r, u, n, hv_part, hv_n,
+                       scale, alpha, step, partial, stop);
+    return hipGetLastError() == hipSuccess ? grid : -1;
 }
 template int launch_axpy_norm<float>(hipStream_t, float *, const float *, long long,
-                                     const double *, const double *, double *, unsigned *,
-                                     double *, double *, int *, int *, int);
+                                     const double *, int, const double *, double *, int,
+                                     double *, int *);
 template int launch_axpy_norm<double>(hipStream_t, double *, const double *, long long,
-                                      const double *, const double *, double *, unsigned *,
-                                      double *, double *, int *, int *, int);
+                                      const double *, int, const double *, double *, int,
+                                      double *, int *);
 
 // Counter-based U[0,1): splitmix64(seed * K + i) >> 11 scaled by 2^-53.
 __device__ __forceinline__ uint64_t splitmix64(uint64_t z)
@@ -770,7 +1056,7 @@ __global__ __launch_bounds__(kBlock) void k_combine(T *out, const T *basis, long
     __shared__ double c[1024];
     for (int j = threadIdx.x; j < nvec && j < 1024; j += kBlock) c[j] = coef[j];
     __syncthreads();
-    double v[1] = {0.0}, tot[1];
+    double v[1] = {0.0};
     for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
          i += (long long)gridDim.x * kBlock) {
         double acc = 0.0;
@@ -779,11 +1065,7 @@ __global__ __launch_bounds__(kBlock) void k_combine(T *out, const T *basis, long
         out[i] = t;
         v[0] += (double)t * (double)t;
     }
-    if (reduce_last<1>(v, partial, ticket, tot) && threadIdx.x == 0) {
-        const double nr = sqrt(tot[0]);
-        *norm_out = nr;
-        *scale_out = 1.0 / nr;
-    }
+    block_partial<1>(v, partial);
 }
 
 template <typename T>
@@ -792,9 +1074,12 @@ int launch_combine(hipStream_t s, T *out, const T *basis, long long stride, int 
                    double *norm_out, double *scale_out)
 {
     if (nvec > 1024) return -2;
-    hipLaunchKernelGGL(k_combine<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, out, basis, stride,
+    const int grid = grid_for(n);
+    hipLaunchKernelGGL(k_combine<T>, dim3(grid), dim3(kBlock), 0, s, out, basis, stride,
                        nvec, coef, n, partial, ticket, norm_out, scale_out);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    if (hipGetLastError() != hipSuccess) return -1;
+    return launch_finish<1>(s, partial, FinArgs{FIN_NORM, grid, nullptr, norm_out, scale_out,
+                                                nullptr, nullptr, 0, 0});
 }
 template int launch_combine<float>(hipStream_t, float *, const float *, long long, int,
                                    const double *, long long, double *, unsigned *, double *,
@@ -948,7 +1233,7 @@ __global__ __launch_bounds__(kBlock) void k_observables(const T *gs, HvOp op,
                                                         unsigned *ticket, double *out)
 {
     constexpr int NS = 3 * kMaxNorb;
-    double v[NS], tot[NS];
+    double v[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) v[s] = 0.0;
     const double sc = scale ? *scale : 1.0;
@@ -970,22 +1255,19 @@ __global__ __launch_bounds__(kBlock) void k_observables(const T *gs, HvOp op,
             }
         }
     }
-    if (reduce_last<NS>(v, partial, ticket, tot) && threadIdx.x == 0) {
-        for (int m = 0; m < norb; ++m) {
-            out[m] = tot[m];
-            out[norb + m] = tot[kMaxNorb + m];
-            out[2 * norb + m] = tot[2 * kMaxNorb + m];
-        }
-    }
+    block_partial<NS>(v, partial);
 }
 template <typename T>
 int launch_observables(hipStream_t s, const T *gs, const HvOp &op, const double *scale,
                        double *partial, unsigned *ticket, double *out)
 {
     const long long n = (long long)op.nup * op.ndn;
-    hipLaunchKernelGGL(k_observables<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, gs, op, scale,
+    const int grid = grid_for(n);
+    hipLaunchKernelGGL(k_observables<T>, dim3(grid), dim3(kBlock), 0, s, gs, op, scale,
                        partial, ticket, out);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    if (hipGetLastError() != hipSuccess) return -1;
+    return launch_finish<3 * kMaxNorb>(s, partial, FinArgs{FIN_OBS, grid, nullptr, out, nullptr,
+                                                           nullptr, nullptr, 0, op.norb});
 }
 template int launch_observables<float>(hipStream_t, const float *, const HvOp &, const double *,
                                        double *, unsigned *, double *);

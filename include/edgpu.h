@@ -71,9 +71,11 @@ const char *edg_last_error(const edg_ctx *ctx);
 int edg_set_memory_budget(edg_ctx *ctx, unsigned long long bytes);
 /* Number of HIP devices visible to the process (no compute). */
 int edg_device_count(int *n);
-/* Kernel schedule: sector columns per workgroup of the H.v kernel (1, 2, 4
- * or 8) and FP64 accumulation of the row sums (1) or accumulation in the
- * storage precision (0).  No counterpart in the reference (its hx is one
+/* Kernel schedule of the H.v kernel: sector columns per workgroup (1, 2, 4
+ * or 8, one workgroup per column group) or 16, 32, 64 for persistent
+ * workgroups (one per compute unit, hop table resident in LDS) with 1, 2 or
+ * 4 columns; and FP64 accumulation of the row sums (1) or accumulation in
+ * the storage precision (0).  No counterpart in the reference (its hx is one
  * scalar loop); set by the autotuner or a caller that has measured. */
 int edg_set_tuning(edg_ctx *ctx, int cols_per_block, int fp64_accumulate);
 

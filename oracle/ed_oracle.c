@@ -219,11 +219,27 @@ void orc_hx(const orc_model *p, const double *em, const double *ek, const double
 }
 
 /* numeric_utils.f90:159-179 mpi_dot_product / mpi_norm (single rank) */
+/* Deterministic for a given thread count: fixed contiguous chunks, partial
+ * sums combined in chunk order. */
 static double dot(const double *a, const double *b, long n)
 {
+    enum { MAXCH = 256 };
+    double part[MAXCH];
+    int nch = 1;
+#ifdef _OPENMP
+    nch = omp_get_max_threads();
+    if (nch > MAXCH) nch = MAXCH;
+#endif
+    long chunk = (n + nch - 1) / nch;
+#pragma omp parallel for schedule(static, 1)
+    for (int c = 0; c < nch; ++c) {
+        double s = 0.0;
+        long lo = (long)c * chunk, hi = lo + chunk < n ? lo + chunk : n;
+        for (long i = lo; i < hi; ++i) s += a[i] * b[i];
+        part[c] = s;
+    }
     double s = 0.0;
-#pragma omp parallel for reduction(+ : s) schedule(static)
-    for (long i = 0; i < n; ++i) s += a[i] * b[i];
+    for (int c = 0; c < nch; ++c) s += part[c];
     return s;
 }
 double orc_dot(const double *a, const double *b, long n) { return dot(a, b, n); }

@@ -83,7 +83,7 @@ def test_hx_parity(edgpu, oracle, precision, tol, norb, nbath, nu, nd):
     x = rng.normal(size=b.ntot)
     if precision == "fp32":
         x = x.astype(np.float32).astype(np.float64)
-    for cols in (1, 2, 4, 8):
+    for cols in (1, 2, 4, 8, 16, 32, 64):  # 16/32/64: persistent workgroups
         s.set_tuning(cols, False)
         y = s.hx(em, ek, vmk, nu, nd, x)
         ref = oracle.hx(m, em, ek, vmk, b, x)
@@ -358,3 +358,23 @@ def test_ns14_properties_fp32_vs_fp64(edgpu, oracle):
     G2, _ = s2.cluster_green_ftn(em, ek, vmk, om)
     assert np.abs(G2[:, 0, 0] - G2[:, 0, 1]).max() < 1e-5
     assert np.abs(G2[:, 0, 0] - G64[:, 0, 0]).max() < 1e-5
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("sched", [1, 2, 8, 16, 32, 64])
+def test_schedules_same_solve(edgpu, oracle, precision, sched):
+    """Every H.v schedule gives the same ground state and G(i w) (Ns=10,
+    chains in odd-dimension sectors exercise the persistent table tail)."""
+    from edgpu import workloads as W
+    em, ek, vmk, mu = W.single_orbital(9, 3.0, seed=11)
+    om = W.matsubara(512, 100.0)
+    ref = edgpu.Solver(norb=1, nbath=9, U=3.0, mu=mu, maxnstep=120, precision="fp64")
+    g0 = ref.ground_state(em, ek, vmk, [(5, 5)])
+    G0, _ = ref.cluster_green_ftn(em, ek, vmk, om)
+    s = edgpu.Solver(norb=1, nbath=9, U=3.0, mu=mu, maxnstep=120, precision=precision)
+    s.set_tuning(sched, False)
+    g = s.ground_state(em, ek, vmk, [(5, 5)])
+    G, _ = s.cluster_green_ftn(em, ek, vmk, om)
+    tol_e, tol_g = (1e-10, 1e-9) if precision == "fp64" else (1e-6, 1e-5)
+    assert abs(g.val - g0.val) < tol_e
+    assert np.abs(G - G0).max() < tol_g
