@@ -35,6 +35,17 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic():
+    """HBM bytes per H.v launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE profile (calibrated against a device copy), or None."""
+    path = os.path.join(ROOT, "profiles", "hv_ns14_fp32_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["traffic_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -71,6 +82,29 @@ def cpu_baseline(em, ek, vmk, U, mu, nsteps, threads):
                        f"steps per solve", ms_per_lanczos_step=dt * 1e3)
 
 
+def profile_hv(edgpu, em, ek, vmk, U, mu):
+    """H.v launches plus one plain device copy of 2*4*D bytes (read 4D, write
+    4D) whose counter readings calibrate FETCH_SIZE / WRITE_SIZE."""
+    import ctypes as C
+    s32 = make_solver(edgpu, "fp32", U, mu)
+    ms, by = s32.time_hv(em, ek, vmk, 7, 7, warmup=2, reps=10)
+    nbytes = 4 * 3432 * 3432
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(nbytes)) != 0:
+            raise SystemExit("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+        raise SystemExit("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"hv_us": ms * 1e3, "hv_bytes": by, "copy_bytes_read": nbytes,
+                      "copy_bytes_written": nbytes}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +114,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--profile-hv", action="store_true",
+                    help="only run H.v launches (FP32, Ns=14) and one device copy of the same "
+                         "byte count, for hardware-counter profiles")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} "
@@ -94,6 +131,9 @@ def main():
 
     U = args.U
     em, ek, vmk, mu = W.single_orbital(13, U, seed=1)
+    if args.profile_hv:
+        profile_hv(edgpu, em, ek, vmk, U, mu)
+        return
     om = W.matsubara(2048, 100.0)
     D = 3432 * 3432
 
@@ -169,7 +209,7 @@ def main():
         "e0": e32,
         "gs_residual": info32["residual"],
         "roofline": {"bound": "hbm", "achieved": hv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": hv_gbs / HBM_PEAK_GBS, "traffic": None},
+                     "frac": hv_gbs / HBM_PEAK_GBS, "traffic": measured_traffic()},
         "cpu_baseline": cpu,
     }
     res.update(out)

@@ -326,6 +326,20 @@ struct alignas(sizeof(T) * C) Row {
 
 __host__ __device__ constexpr size_t align16(size_t n) { return (n + 15) & ~size_t(15); }
 
+// XCD-aware group order.  Workgroups of one launch are dispatched round-robin
+// over the 8 XCDs (workgroup b runs on XCD b % 8), each with its own 4 MiB L2.
+// Giving XCD x a contiguous range of column groups keeps neighbour columns of
+// the down hops in that XCD's L2: in increasing-pattern (colex) order the high
+// bath bits change slowly, so a contiguous rank range is closed under most
+// hops.  Bijective for any ngroups.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_group(int b, int ngroups)
+{
+    const int q = ngroups / kXcds, r = ngroups - q * kXcds;
+    const int x = b % kXcds, i = b / kXcds;
+    return x * q + (x < r ? x : r) + i;
+}
+
 template <typename ACC>
 struct HopOf;
 template <>
@@ -366,7 +380,7 @@ __global__ __launch_bounds__(kHvBlock) void k_hv(HvOp op, HvArgs a)
 
     const int nup = op.nup, ndn = op.ndn;
     const long long ld = nup;
-    const int idn0 = blockIdx.x * C;
+    const int idn0 = xcd_group(blockIdx.x, (ndn + C - 1) / C) * C;
     const T *x = static_cast<const T *>(a.x);
     const Hop *dtab = HopOf<ACC>::tab(op.dn);
     const Hop *utab = HopOf<ACC>::tab(op.up);
@@ -545,7 +559,7 @@ __global__ __launch_bounds__(kHvBlock) void k_hv1(HvOp op, HvArgs a)
     __shared__ ACC s_v[2 * (kMaxHops + 1)];  // up-spin elements: [idx] = +V, [65 + idx] = -V, idx 64 = 0
 
     const int nup = op.nup, ndn = op.ndn;
-    const int idn0 = blockIdx.x * C;
+    const int idn0 = xcd_group(blockIdx.x, (ndn + C - 1) / C) * C;
     const T *x = static_cast<const T *>(a.x);
     const unsigned char *xb = static_cast<const unsigned char *>(a.x);
     const Hop *dtab = HopOf<ACC>::tab(op.dn);
