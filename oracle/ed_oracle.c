@@ -15,8 +15,9 @@
  *     (ed_basis.f90:132-135,149-150); C arrays here are 0-based;
  *   - em(norb,2), ek(nbath,2), vmk(norb,nbath,2) column-major FP64.
  *
- * Parity pinning of each function (what checks it) is listed in
- * oracle/README.md and exercised by tests/test_oracle_*.py.
+ * What pins each function (closed forms, brute-force enumeration, the dense
+ * Jordan-Wigner construction in oracle/dense.py, LAPACK, golden fixtures) is
+ * exercised by tests/test_oracle.py.
  */
 #include <complex.h>
 #include <math.h>
