@@ -40,11 +40,14 @@ struct SpinTab {
     uint8_t *mask = nullptr;       // [n] occupied impurity orbitals
     uint32_t *orb = nullptr;       // [norb*norb][n] orbital moves dst<-src
     int *maxcnt = nullptr;         // device scalar: max over cnt
-    // compressed hop table of the single-orbital fast path, [n][KC]:
-    // ccol = target rank, cidx = index into vsrc (bit 7: negative), 64 = zero
+    // compressed hop tables of the single-orbital kernels, [n][KC]:
+    // ccol = target rank, cidx = index into vsrc (bit 7: negative), 64 = zero;
+    // hrec = target rank | (value-table index << 16), the value table being
+    // [idx] = +v, [kMaxHops + 1 + idx] = -v, idx 64 = 0
     int KC = 0;                    // 8 or 16 (0: not built)
     uint16_t *ccol = nullptr;
     uint8_t *cidx = nullptr;
+    uint32_t *hrec = nullptr;
     const double *vsrc = nullptr;  // hybridizations of this spin, [l*norb + m]
 };
 
@@ -58,7 +61,9 @@ struct HvOp {
     double U, Up, Jp;
 };
 
-enum HvMode { HV_PLAIN = 0, HV_LANCZOS = 1, HV_RQ = 2 };
+// HV_PLAIN: y = Hx (y += Hx when HvArgs::accumulate; the launcher then runs
+// the HV_ACCUM instantiation, so that the plain form never reads y)
+enum HvMode { HV_PLAIN = 0, HV_LANCZOS = 1, HV_RQ = 2, HV_ACCUM = 3 };
 
 struct HvArgs {
     const void *x;          // input vector (T)

@@ -288,8 +288,9 @@ __global__ __launch_bounds__(kBlock) void k_spin_table(int nsite, int norb, int 
     }
 }
 
-// Compressed [n][KC] copy of the first KC slots: 16-bit target rank and
-// 8-bit element index (bit 7 = negative sign, 64 = zero element).
+// Compressed [n][KC] copies of the first KC slots: 16-bit target rank and
+// 8-bit element index (bit 7 = negative sign, 64 = zero element); and the
+// same pair as one 32-bit record, rank | (value-table index << 16).
 __global__ __launch_bounds__(kBlock) void k_compress(SpinTab t)
 {
     const int r = blockIdx.x * kBlock + threadIdx.x;
@@ -298,6 +299,8 @@ __global__ __launch_bounds__(kBlock) void k_compress(SpinTab t)
         const HopD h = t.hd[(long long)r * t.K + k];
         t.ccol[(long long)r * t.KC + k] = (uint16_t)h.col;
         t.cidx[(long long)r * t.KC + k] = (uint8_t)h.pad;
+        const uint32_t vi = (h.pad & 0x7Fu) + ((h.pad >> 7) & 1u ? (uint32_t)kMaxHops + 1u : 0u);
+        t.hrec[(long long)r * t.KC + k] = (h.col & 0xFFFFu) | (vi << 16);
     }
 }
 
@@ -362,7 +365,7 @@ __device__ __forceinline__ double cross_energy(int mu, int md, double U, double 
 // y = H x over the C columns [idn0, idn0+C) of the sector matrix.
 //   T    storage type of the vectors, ACC arithmetic type of the row sums
 //   KH   compile-time bound on the hop count of both spins (0 = any, loops)
-//   MODE HV_PLAIN: y = Hx or y += Hx; HV_LANCZOS: y = s_j Hx - beta_j s_{j-1} p
+//   MODE HV_PLAIN: y = Hx; HV_ACCUM: y += Hx; HV_LANCZOS: y = s_j Hx - beta_j s_{j-1} p
 //        with alpha_j = s_j <x, y>; HV_RQ: no store, sums <x,Hx>, <Hx,Hx>, <x,x>
 template <typename T, typename ACC, int C, int MODE, bool KAN, int KH>
 __global__ __launch_bounds__(kHvBlock) void k_hv(HvOp op, HvArgs a)
@@ -512,7 +515,9 @@ __global__ __launch_bounds__(kHvBlock) void k_hv(HvOp op, HvArgs a)
             if (idn0 + c >= ndn) break;
             const long long e = (long long)(idn0 + c) * ld + iup;
             if constexpr (MODE == HV_PLAIN) {
-                y[e] = a.accumulate ? (T)((ACC)y[e] + acc[c]) : (T)acc[c];
+                y[e] = (T)acc[c];
+            } else if constexpr (MODE == HV_ACCUM) {
+                y[e] = (T)((ACC)y[e] + acc[c]);
             } else if constexpr (MODE == HV_LANCZOS) {
                 double r = sx * (double)acc[c];
                 if (pv) r -= bsp * (double)pv[e];
@@ -543,19 +548,29 @@ template <>
 __device__ __forceinline__ double energy_of<double>(const SpinTab &t, int r) { return t.energy[r]; }
 
 // Single-orbital H.v (norb == 1): the diagonal is E_up(iup) + E_dn(idn) +
-// U n_up n_dn with n the impurity bit, and both spins hop through KH-slot
-// tables whose unused slots are self references with a zero element.  The
-// down-hop column offsets and elements of the block's C columns are
-// wave-uniform and loaded once into scalar registers; every slot is read
-// unconditionally, so the row loop has no branches.
+// U n_up n_dn with n the impurity bit, and both spins hop through tables
+// whose unused slots are self references with a zero element.
+//   up spin:   hrec[iup][KS] records (KS = 8 or 16 stored slots, the first KH
+//              processed), rank | (value index << 16), gathered from the C
+//              columns staged in LDS;
+//   down spin: the C columns' neighbour-column byte offsets and elements are
+//              wave-uniform, read once into scalar registers / LDS, and each
+//              hop is one coalesced load of a neighbour column.
+// Latency: the column staging issues all its loads before the LDS stores,
+// and the row loop keeps U rows per thread in flight (all of their loads
+// are issued before any arithmetic), so a workgroup waits on about
+// 1 + nup / (U * 512) memory round trips.
 template <typename T, typename ACC, int C, int MODE, int KH>
 __global__ __launch_bounds__(kHvBlock) void k_hv1(HvOp op, HvArgs a)
 {
+    constexpr int KS = KH <= 8 ? 8 : 16;
+    constexpr int U = C * KH * sizeof(T) <= 64 ? 2 : 1;  // keeps the wide forms out of spills
+    constexpr int kStage = C <= 8 ? 8 / C : 1;
     if (a.stop && *a.stop) return;
     using Hop = typename HopOf<ACC>::type;
     extern __shared__ __align__(64) unsigned char smem[];
     T *xs = reinterpret_cast<T *>(smem);
-    __shared__ Row<ACC, C> s_dval[KH];  // down-hop elements, slot-major
+    __shared__ Row<ACC, C> s_dval[KH];      // down-hop elements, slot-major
     __shared__ ACC s_v[2 * (kMaxHops + 1)];  // up-spin elements: [idx] = +V, [65 + idx] = -V, idx 64 = 0
 
     const int nup = op.nup, ndn = op.ndn;
@@ -587,11 +602,29 @@ __global__ __launch_bounds__(kHvBlock) void k_hv1(HvOp op, HvArgs a)
         ed0[c] = e;
         edU[c] = op.dn.mask[idn] ? e + (ACC)op.U : e;
     }
+    // stage the C columns row-interleaved, xs[i*C + c] = x(i, idn0 + c)
+    for (int i0 = threadIdx.x; i0 < nup; i0 += kStage * kHvBlock) {
+        T v[C][kStage];
 #pragma unroll
-    for (int c = 0; c < C; ++c) {
-        const int idn = idn0 + c < ndn ? idn0 + c : ndn - 1;
-        const T *xc = x + (long long)idn * nup;
-        for (int i = threadIdx.x; i < nup; i += kHvBlock) xs[i * C + c] = xc[i];
+        for (int c = 0; c < C; ++c) {
+            const int idn = idn0 + c < ndn ? idn0 + c : ndn - 1;
+            const T *xc = x + (long long)idn * nup;
+#pragma unroll
+            for (int r = 0; r < kStage; ++r) {
+                const int i = i0 + r * kHvBlock;
+                v[c][r] = xc[i < nup ? i : nup - 1];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kStage; ++r) {
+            const int i = i0 + r * kHvBlock;
+            if (i < nup) {
+                Row<T, C> w;
+#pragma unroll
+                for (int c = 0; c < C; ++c) w.v[c] = v[c][r];
+                *reinterpret_cast<Row<T, C> *>(xs + i * C) = w;
+            }
+        }
     }
     __syncthreads();
 
@@ -602,53 +635,90 @@ __global__ __launch_bounds__(kHvBlock) void k_hv1(HvOp op, HvArgs a)
     }
     T *y = static_cast<T *>(a.y);
     const T *pv = static_cast<const T *>(a.p);
-    for (int iup = threadIdx.x; iup < nup; iup += kHvBlock) {
-        const uint32_t ib = (uint32_t)iup * (uint32_t)sizeof(T);
-        // the row's KH compressed hop records are contiguous: wide loads
-        const Row<uint16_t, KH> hc = *reinterpret_cast<const Row<uint16_t, KH> *>(op.up.ccol + (long long)iup * KH);
-        const Row<uint8_t, KH> hi = *reinterpret_cast<const Row<uint8_t, KH> *>(op.up.cidx + (long long)iup * KH);
-        T g[C][KH];
+    const bool has_p = MODE == HV_LANCZOS && pv != nullptr;
+    for (int i0 = threadIdx.x; i0 < nup; i0 += U * kHvBlock) {
+        int iu[U];
+        bool live[U];
 #pragma unroll
-        for (int c = 0; c < C; ++c)
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * kHvBlock;
+            live[u] = i < nup;
+            iu[u] = live[u] ? i : i0;  // a dead row recomputes row i0 and is not stored
+        }
+        // all loads of the U rows first
+        Row<uint32_t, KS> hr[U];
+        T g[U][C][KH];
+        ACC eu[U];
+        bool mu[U];
+        T old[U][C];
 #pragma unroll
-            for (int k = 0; k < KH; ++k) g[c][k] = *reinterpret_cast<const T *>(xb + (coff[c][k] + ib));
-        const ACC eu = energy_of<ACC>(op.up, iup);
-        const bool mu = op.up.mask[iup] != 0;
-        const Row<T, C> xr = *reinterpret_cast<const Row<T, C> *>(xs + iup * C);
-        ACC acc[C];
+        for (int u = 0; u < U; ++u)
+            hr[u] = *reinterpret_cast<const Row<uint32_t, KS> *>(op.up.hrec + (long long)iu[u] * KS);
 #pragma unroll
-        for (int c = 0; c < C; ++c) acc[c] = (eu + (mu ? edU[c] : ed0[c])) * (ACC)xr.v[c];
+        for (int u = 0; u < U; ++u) {
+            const uint32_t ib = (uint32_t)iu[u] * (uint32_t)sizeof(T);
 #pragma unroll
-        for (int k = 0; k < KH; ++k) {
-            const uint32_t ix = hi.v[k];
-            const ACC val = s_v[(ix & 0x7Fu) + ((ix >> 7) ? kMaxHops + 1 : 0)];
-            const Row<T, C> src = *reinterpret_cast<const Row<T, C> *>(xs + (uint32_t)hc.v[k] * C);
+            for (int c = 0; c < C; ++c)
 #pragma unroll
-            for (int c = 0; c < C; ++c) acc[c] += val * (ACC)src.v[c];
+                for (int k = 0; k < KH; ++k) g[u][c][k] = *reinterpret_cast<const T *>(xb + (coff[c][k] + ib));
         }
 #pragma unroll
-        for (int k = 0; k < KH; ++k) {
-            const Row<ACC, C> dv = s_dval[k];
+        for (int u = 0; u < U; ++u) {
+            eu[u] = energy_of<ACC>(op.up, iu[u]);
+            mu[u] = op.up.mask[iu[u]] != 0;
+        }
+        if constexpr (MODE == HV_ACCUM || MODE == HV_LANCZOS) {
 #pragma unroll
-            for (int c = 0; c < C; ++c) acc[c] += dv.v[c] * (ACC)g[c][k];
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const int idn = idn0 + c < ndn ? idn0 + c : ndn - 1;
+                    const long long e = (long long)idn * nup + iu[u];
+                    if constexpr (MODE == HV_ACCUM) old[u][c] = y[e];
+                    else old[u][c] = has_p ? pv[e] : T(0);
+                }
         }
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
-            if (idn0 + c >= ndn) break;
-            const long long e = (long long)(idn0 + c) * nup + iup;
-            if constexpr (MODE == HV_PLAIN) {
-                y[e] = a.accumulate ? (T)((ACC)y[e] + acc[c]) : (T)acc[c];
-            } else if constexpr (MODE == HV_LANCZOS) {
-                double r = sx * (double)acc[c];
-                if (pv) r -= bsp * (double)pv[e];
-                const T rt = (T)r;
-                y[e] = rt;
-                sums[0] += (double)xr.v[c] * (double)rt;
-            } else {
-                const double xv = (double)xr.v[c], hv = (double)acc[c];
-                sums[0] += xv * hv;
-                sums[1] += hv * hv;
-                sums[2] += xv * xv;
+        for (int u = 0; u < U; ++u) {
+            const Row<T, C> xr = *reinterpret_cast<const Row<T, C> *>(xs + iu[u] * C);
+            ACC acc[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) acc[c] = (eu[u] + (mu[u] ? edU[c] : ed0[c])) * (ACC)xr.v[c];
+#pragma unroll
+            for (int k = 0; k < KH; ++k) {
+                const uint32_t w = hr[u].v[k];
+                const ACC val = s_v[w >> 16];
+                const Row<T, C> src = *reinterpret_cast<const Row<T, C> *>(xs + (w & 0xFFFFu) * C);
+#pragma unroll
+                for (int c = 0; c < C; ++c) acc[c] += val * (ACC)src.v[c];
+            }
+#pragma unroll
+            for (int k = 0; k < KH; ++k) {
+                const Row<ACC, C> dv = s_dval[k];
+#pragma unroll
+                for (int c = 0; c < C; ++c) acc[c] += dv.v[c] * (ACC)g[u][c][k];
+            }
+            if (!live[u]) continue;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                if (idn0 + c >= ndn) break;
+                const long long e = (long long)(idn0 + c) * nup + iu[u];
+                if constexpr (MODE == HV_PLAIN) {
+                    y[e] = (T)acc[c];
+                } else if constexpr (MODE == HV_ACCUM) {
+                    y[e] = (T)((ACC)old[u][c] + acc[c]);
+                } else if constexpr (MODE == HV_LANCZOS) {
+                    double r = sx * (double)acc[c];
+                    if (has_p) r -= bsp * (double)old[u][c];
+                    const T rt = (T)r;
+                    y[e] = rt;
+                    sums[0] += (double)xr.v[c] * (double)rt;
+                } else {
+                    const double xv = (double)xr.v[c], hv = (double)acc[c];
+                    sums[0] += xv * hv;
+                    sums[1] += hv * hv;
+                    sums[2] += xv * xv;
+                }
             }
         }
     }
@@ -781,7 +851,9 @@ __global__ __launch_bounds__(kHvpBlock) void k_hvp(HvOp op, HvArgs a)
                 if (idn0 + c >= ndn) break;
                 const long long e = (long long)(idn0 + c) * nup + iup;
                 if constexpr (MODE == HV_PLAIN) {
-                    y[e] = a.accumulate ? (T)((ACC)y[e] + acc[c]) : (T)acc[c];
+                    y[e] = (T)acc[c];
+                } else if constexpr (MODE == HV_ACCUM) {
+                    y[e] = (T)((ACC)y[e] + acc[c]);
                 } else if constexpr (MODE == HV_LANCZOS) {
                     double r = sx * (double)acc[c];
                     if (pv) r -= bsp * (double)pv[e];
@@ -820,6 +892,7 @@ static int launch_hvp_c(hipStream_t s, const HvOp &op, int mode, const HvArgs &a
     const int grid = ngroups < op.ncu ? ngroups : op.ncu;
     void (*f)(HvOp, HvArgs);
     if (mode == HV_PLAIN) f = k_hvp<T, ACC, C, HV_PLAIN, KH>;
+    else if (mode == HV_ACCUM) f = k_hvp<T, ACC, C, HV_ACCUM, KH>;
     else if (mode == HV_LANCZOS) f = k_hvp<T, ACC, C, HV_LANCZOS, KH>;
     else f = k_hvp<T, double, C, HV_RQ, KH>;
     if (hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
@@ -851,6 +924,7 @@ static int launch_hv1_c(hipStream_t s, const HvOp &op, int mode, const HvArgs &a
     const int grid = (op.ndn + C - 1) / C;
     void (*f)(HvOp, HvArgs);
     if (mode == HV_PLAIN) f = k_hv1<T, ACC, C, HV_PLAIN, KH>;
+    else if (mode == HV_ACCUM) f = k_hv1<T, ACC, C, HV_ACCUM, KH>;
     else if (mode == HV_LANCZOS) f = k_hv1<T, ACC, C, HV_LANCZOS, KH>;
     else f = k_hv1<T, double, C, HV_RQ, KH>;
     if (lds > 64 * 1024) {
@@ -869,8 +943,8 @@ static int launch_hv1_k(hipStream_t s, const HvOp &op, int mode, int cols, const
     switch (cols) {
     case 1: return launch_hv1_c<T, ACC, 1, KH>(s, op, mode, a);
     case 2: return launch_hv1_c<T, ACC, 2, KH>(s, op, mode, a);
-    case 4: return launch_hv1_c<T, ACC, 4, KH>(s, op, mode, a);
-    case 8: return launch_hv1_c<T, ACC, 8, KH>(s, op, mode, a);
+    case 4:
+    case 8: return launch_hv1_c<T, ACC, 4, KH>(s, op, mode, a);  // 8 columns would spill
     default: return -2;
     }
 }
@@ -886,6 +960,7 @@ static int launch_hv_c(hipStream_t s, const HvOp &op, int mode, const HvArgs &a)
     const int grid = (op.ndn + C - 1) / C;
     void (*f)(HvOp, HvArgs);
     if (mode == HV_PLAIN) f = k_hv<T, ACC, C, HV_PLAIN, KAN, KH>;
+    else if (mode == HV_ACCUM) f = k_hv<T, ACC, C, HV_ACCUM, KAN, KH>;
     else if (mode == HV_LANCZOS) f = k_hv<T, ACC, C, HV_LANCZOS, KAN, KH>;
     else f = k_hv<T, double, C, HV_RQ, KAN, KH>;
     if (lds > 64 * 1024) {
@@ -921,6 +996,7 @@ static int launch_hv_a(hipStream_t s, const HvOp &op, int mode, int cols, const 
         const int pc = op.sched >> 4;
         if (hvp_lds(op, pc, 8, sizeof(T)) <= kLdsBudget) return launch_hvp_k<T, ACC, 8>(s, op, mode, pc, a);
     }
+    if (op.norb == 1 && op.up.KC == 8 && km <= 7) return launch_hv1_k<T, ACC, 7>(s, op, mode, cols, a);
     if (op.norb == 1 && op.up.KC == 8 && km <= 8) return launch_hv1_k<T, ACC, 8>(s, op, mode, cols, a);
     if (op.norb == 1 && op.up.KC == 16 && km <= 16) return launch_hv1_k<T, ACC, 16>(s, op, mode, cols, a);
     if (km <= 8) return launch_hv_k<T, ACC, false, 8>(s, op, mode, cols, a);
@@ -934,6 +1010,7 @@ int launch_hv(hipStream_t s, const HvOp &op_in, int mode, int cols, bool acc64, 
     // cols 1/2/4/8: one workgroup per column group; 16/32/64: persistent
     // workgroups with 1/2/4 columns (falls back to 1 column per workgroup)
     HvOp op = op_in;
+    if (mode == HV_PLAIN && a.accumulate) mode = HV_ACCUM;
     op.sched = cols >= 16 ? cols : 0;
     if (cols >= 16) cols = 1;
     while (cols > 1 && (size_t)op.nup * cols * sizeof(T) > kLdsBudget) cols >>= 1;
