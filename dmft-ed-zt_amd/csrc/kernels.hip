@@ -556,15 +556,17 @@ __device__ __forceinline__ double energy_of<double>(const SpinTab &t, int r) { r
 //   down spin: the C columns' neighbour-column byte offsets and elements are
 //              wave-uniform, read once into scalar registers / LDS, and each
 //              hop is one coalesced load of a neighbour column.
-// Latency: the column staging issues all its loads before the LDS stores,
-// and the row loop keeps U rows per thread in flight (all of their loads
-// are issued before any arithmetic), so a workgroup waits on about
-// 1 + nup / (U * 512) memory round trips.
+// The column staging issues all its loads before the LDS stores.  The row
+// loop is written for U rows per thread in flight; U = 1 measured fastest
+// on MI355X (Ns=14: 46 us against 50 us for U = 2 and 61 us for U = 4 at two
+// columns per workgroup): the kernel is bound by the L1/L2 request stream
+// of the neighbour-column loads, not by their latency, and fewer registers
+// keep more waves resident.
 template <typename T, typename ACC, int C, int MODE, int KH>
 __global__ __launch_bounds__(kHvBlock) void k_hv1(HvOp op, HvArgs a)
 {
     constexpr int KS = KH <= 8 ? 8 : 16;
-    constexpr int U = C * KH * sizeof(T) <= 64 ? 2 : 1;  // keeps the wide forms out of spills
+    constexpr int U = 1;
     constexpr int kStage = C <= 8 ? 8 / C : 1;
     if (a.stop && *a.stop) return;
     using Hop = typename HopOf<ACC>::type;
@@ -730,122 +732,170 @@ __global__ __launch_bounds__(kHvBlock) void k_hv1(HvOp op, HvArgs a)
     }
 }
 
-// Persistent single-orbital H.v: one 1024-thread workgroup per CU copies
-// the compressed up-spin hop table into LDS once and then walks column
-// groups g = blockIdx.x, blockIdx.x + gridDim.x, ...  The table then costs
-// no L2 requests per column; what each group still reads from L2 is its own
-// C columns (staged in LDS for the up hops) and the down-hop neighbour
-// columns.  Partial sums are accumulated per workgroup over its groups in a
-// fixed order, so the reductions stay deterministic.
-constexpr int kHvpBlock = 1024;
+// Persistent single-orbital H.v with the up-spin hop table resident in LDS.
+// One workgroup per CU copies the compressed table (16-bit target rank and
+// 8-bit element index per slot, 24 bytes per row) into LDS once; every hop
+// record then costs an LDS read instead of an L2 request.  The workgroup
+// walks its column groups with the next group's columns loaded into
+// registers while the current group is computed (double-buffered LDS
+// columns, one barrier per group).  Group order is XCD-aware: the
+// workgroups of XCD x (b % 8 == x) share XCD x's contiguous chunk of groups
+// and advance through it together, so the down-hop neighbour columns they
+// read stay close in the XCD's L2.  Partial sums are per workgroup over its
+// groups in a fixed order (deterministic).
+constexpr int kHvtSlots = 8;     // stored table slots per row
+constexpr int kHvtMaxRows = 4096; // nup bound of the register prefetch
 
-template <typename T, typename ACC, int C, int MODE, int KH>
-__global__ __launch_bounds__(kHvpBlock) void k_hvp(HvOp op, HvArgs a)
+__host__ __device__ constexpr size_t hvt_lds(int nup, int C, size_t esize)
 {
+    return 2 * align16((size_t)nup * C * esize) + align16((size_t)nup * kHvtSlots * 2) +
+           align16((size_t)nup * kHvtSlots);
+}
+
+template <typename T, typename ACC, int C, int MODE, int KH, int BS>
+__global__ __launch_bounds__(BS) void k_hv1t(HvOp op, HvArgs a)
+{
+    constexpr int KS = kHvtSlots, NP = kHvtMaxRows / BS;
     if (a.stop && *a.stop) return;
+    using Hop = typename HopOf<ACC>::type;
     extern __shared__ __align__(64) unsigned char smem[];
     const int nup = op.nup, ndn = op.ndn;
-    // LDS layout (each region 16-byte aligned):
-    //   xs[nup*C] (T) | tcol[nup*KH] (u16) | tidx[nup*KH] (u8)
     const size_t xsz = align16((size_t)nup * C * sizeof(T));
-    const size_t csz = align16((size_t)nup * KH * 2);
-    T *xs = reinterpret_cast<T *>(smem);
-    uint16_t *tcol = reinterpret_cast<uint16_t *>(smem + xsz);
-    uint8_t *tidx = reinterpret_cast<uint8_t *>(smem + xsz + csz);
-    __shared__ Row<ACC, C> s_dval[KH];
-    __shared__ ACC s_v[2 * (kMaxHops + 1)];
+    T *xbuf0 = reinterpret_cast<T *>(smem);
+    T *xbuf1 = reinterpret_cast<T *>(smem + xsz);
+    uint16_t *tcol = reinterpret_cast<uint16_t *>(smem + 2 * xsz);
+    uint8_t *tidx = reinterpret_cast<uint8_t *>(smem + 2 * xsz + align16((size_t)nup * KS * 2));
+    __shared__ ACC s_v[256];  // by 8-bit element index: [i] = +V_i, [128 + i] = -V_i, else 0
 
-    for (int i = threadIdx.x; i <= kMaxHops; i += kHvpBlock) {
-        const ACC v = i < op.nvup ? (ACC)op.up.vsrc[i] : ACC(0);
-        s_v[i] = v;
-        s_v[kMaxHops + 1 + i] = -v;
+    for (int i = threadIdx.x; i < 256; i += BS) {
+        const int j = i & 127;
+        const ACC v = (j < 64 && j < op.nvup) ? (ACC)op.up.vsrc[j] : ACC(0);
+        s_v[i] = i < 128 ? v : -v;
     }
     {   // table -> LDS: 16-byte chunks, then the element tail
-        const int nc = nup * KH;                 // entries of each array
+        const int nc = nup * KS;
         const int c16 = nc * 2 / 16, i16 = nc / 16;
         const uint4 *gc = reinterpret_cast<const uint4 *>(op.up.ccol);
         const uint4 *gi = reinterpret_cast<const uint4 *>(op.up.cidx);
         uint4 *lc = reinterpret_cast<uint4 *>(tcol);
         uint4 *li = reinterpret_cast<uint4 *>(tidx);
-        for (int i = threadIdx.x; i < c16; i += kHvpBlock) lc[i] = gc[i];
-        for (int i = threadIdx.x; i < i16; i += kHvpBlock) li[i] = gi[i];
-        for (int i = c16 * 8 + threadIdx.x; i < nc; i += kHvpBlock) tcol[i] = op.up.ccol[i];
-        for (int i = i16 * 16 + threadIdx.x; i < nc; i += kHvpBlock) tidx[i] = op.up.cidx[i];
+        for (int i = threadIdx.x; i < c16; i += BS) lc[i] = gc[i];
+        for (int i = threadIdx.x; i < i16; i += BS) li[i] = gi[i];
+        for (int i = c16 * 8 + threadIdx.x; i < nc; i += BS) tcol[i] = op.up.ccol[i];
+        for (int i = i16 * 16 + threadIdx.x; i < nc; i += BS) tidx[i] = op.up.cidx[i];
     }
-    const T *x = static_cast<const T *>(a.x);
+
+    // this workgroup's groups: XCD x's chunk [g, g1), stride = workgroups on x
+    const int ngroups = (ndn + C - 1) / C;
+    const int nx = gridDim.x < kXcds ? (int)gridDim.x : kXcds;
+    const int x = blockIdx.x % nx, wi = blockIdx.x / nx;
+    const int wstride = (gridDim.x - x + nx - 1) / nx;
+    const int q = ngroups / nx, r = ngroups - q * nx;
+    const int gbeg = x * q + (x < r ? x : r);
+    const int g1 = gbeg + q + (x < r ? 1 : 0);
+    int g = gbeg + wi;
+
+    const T *x_in = static_cast<const T *>(a.x);
     const unsigned char *xb = static_cast<const unsigned char *>(a.x);
-    const HopF *dtabf = op.dn.hf;
-    const HopD *dtabd = op.dn.hd;
+    const Hop *dtab = HopOf<ACC>::tab(op.dn);
     T *y = static_cast<T *>(a.y);
     const T *pv = static_cast<const T *>(a.p);
+
+    // column loads of group gg into registers (clamped indices, always in bounds)
+    T pf[C][NP];
+    auto load_cols = [&](int gg) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int idn = gg * C + c < ndn ? gg * C + c : ndn - 1;
+            const T *xc = x_in + (long long)idn * nup;
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                const int i = threadIdx.x + k * BS;
+                pf[c][k] = xc[i < nup ? i : nup - 1];
+            }
+        }
+    };
+    auto store_cols = [&](T *xs) {
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            const int i = threadIdx.x + k * BS;
+            if (i < nup) {
+                Row<T, C> w;
+#pragma unroll
+                for (int c = 0; c < C; ++c) w.v[c] = pf[c][k];
+                *reinterpret_cast<Row<T, C> *>(xs + i * C) = w;
+            }
+        }
+    };
+    if (g < g1) {
+        load_cols(g);
+        store_cols(xbuf0);
+    }
+    __syncthreads();
+
     double sums[3] = {0.0, 0.0, 0.0};
     double sx = 1.0, bsp = 0.0;
     if constexpr (MODE == HV_LANCZOS) {
-        if (!lanczos_scales<kHvpBlock>(a, sx, bsp)) return;
+        if (!lanczos_scales<BS>(a, sx, bsp)) return;
     }
-    const int ngroups = (ndn + C - 1) / C;
+    const bool has_p = MODE == HV_LANCZOS && pv != nullptr;
 
-    for (int g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    for (int it = 0; g < g1; ++it) {
+        const int gn = g + wstride;
+        if (gn < g1) load_cols(gn);  // in flight while group g is computed
+        const T *xs = (it & 1) ? xbuf1 : xbuf0;
         const int idn0 = g * C;
         uint32_t coff[C][KH];
-        ACC ed0[C], edU[C];
-        __syncthreads();  // previous group is done with xs and s_dval
+        ACC dval[C][KH], ed0[C], edU[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             const int idn = idn0 + c < ndn ? idn0 + c : ndn - 1;
+            const Hop *hr = dtab + (long long)idn * op.dn.K;
 #pragma unroll
             for (int k = 0; k < KH; ++k) {
-                uint32_t col;
-                ACC val;
-                if constexpr (sizeof(ACC) == 8) {
-                    const HopD h = dtabd[(long long)idn * op.dn.K + k];
-                    col = h.col;
-                    val = h.val;
-                } else {
-                    const HopF h = dtabf[(long long)idn * op.dn.K + k];
-                    col = h.col;
-                    val = h.val;
-                }
-                coff[c][k] = col * (uint32_t)nup * (uint32_t)sizeof(T);
-                if (threadIdx.x == 0) s_dval[k].v[c] = val;
+                const Hop h = hr[k];
+                coff[c][k] = h.col * (uint32_t)nup * (uint32_t)sizeof(T);
+                dval[c][k] = (ACC)h.val;
             }
             const ACC e = energy_of<ACC>(op.dn, idn);
             ed0[c] = e;
             edU[c] = op.dn.mask[idn] ? e + (ACC)op.U : e;
-            const T *xc = x + (long long)idn * nup;
-            for (int i = threadIdx.x; i < nup; i += kHvpBlock) xs[i * C + c] = xc[i];
         }
-        __syncthreads();
-
-        for (int iup = threadIdx.x; iup < nup; iup += kHvpBlock) {
+        for (int iup = threadIdx.x; iup < nup; iup += BS) {
             const uint32_t ib = (uint32_t)iup * (uint32_t)sizeof(T);
             T gv[C][KH];
 #pragma unroll
             for (int c = 0; c < C; ++c)
 #pragma unroll
                 for (int k = 0; k < KH; ++k) gv[c][k] = *reinterpret_cast<const T *>(xb + (coff[c][k] + ib));
-            const Row<uint16_t, KH> hc = *reinterpret_cast<const Row<uint16_t, KH> *>(tcol + iup * KH);
-            const Row<uint8_t, KH> hi = *reinterpret_cast<const Row<uint8_t, KH> *>(tidx + iup * KH);
             const ACC eu = energy_of<ACC>(op.up, iup);
             const bool mu = op.up.mask[iup] != 0;
+            T old[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int idn = idn0 + c < ndn ? idn0 + c : ndn - 1;
+                const long long e = (long long)idn * nup + iup;
+                if constexpr (MODE == HV_ACCUM) old[c] = y[e];
+                else if constexpr (MODE == HV_LANCZOS) old[c] = has_p ? pv[e] : T(0);
+                else old[c] = T(0);
+            }
+            const Row<uint16_t, KS> hc = *reinterpret_cast<const Row<uint16_t, KS> *>(tcol + iup * KS);
+            const Row<uint8_t, KS> hi = *reinterpret_cast<const Row<uint8_t, KS> *>(tidx + iup * KS);
             const Row<T, C> xr = *reinterpret_cast<const Row<T, C> *>(xs + iup * C);
             ACC acc[C];
 #pragma unroll
             for (int c = 0; c < C; ++c) acc[c] = (eu + (mu ? edU[c] : ed0[c])) * (ACC)xr.v[c];
 #pragma unroll
             for (int k = 0; k < KH; ++k) {
-                const uint32_t ix = hi.v[k];
-                const ACC val = s_v[(ix & 0x7Fu) + ((ix >> 7) ? kMaxHops + 1 : 0)];
+                const ACC val = s_v[hi.v[k]];
                 const Row<T, C> src = *reinterpret_cast<const Row<T, C> *>(xs + (uint32_t)hc.v[k] * C);
 #pragma unroll
                 for (int c = 0; c < C; ++c) acc[c] += val * (ACC)src.v[c];
             }
 #pragma unroll
-            for (int k = 0; k < KH; ++k) {
-                const Row<ACC, C> dv = s_dval[k];
+            for (int k = 0; k < KH; ++k)
 #pragma unroll
-                for (int c = 0; c < C; ++c) acc[c] += dv.v[c] * (ACC)gv[c][k];
-            }
+                for (int c = 0; c < C; ++c) acc[c] += dval[c][k] * (ACC)gv[c][k];
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 if (idn0 + c >= ndn) break;
@@ -853,11 +903,11 @@ __global__ __launch_bounds__(kHvpBlock) void k_hvp(HvOp op, HvArgs a)
                 if constexpr (MODE == HV_PLAIN) {
                     y[e] = (T)acc[c];
                 } else if constexpr (MODE == HV_ACCUM) {
-                    y[e] = (T)((ACC)y[e] + acc[c]);
+                    y[e] = (T)((ACC)old[c] + acc[c]);
                 } else if constexpr (MODE == HV_LANCZOS) {
-                    double r = sx * (double)acc[c];
-                    if (pv) r -= bsp * (double)pv[e];
-                    const T rt = (T)r;
+                    double rr = sx * (double)acc[c];
+                    if (has_p) rr -= bsp * (double)old[c];
+                    const T rt = (T)rr;
                     y[e] = rt;
                     sums[0] += (double)xr.v[c] * (double)rt;
                 } else {
@@ -868,50 +918,42 @@ __global__ __launch_bounds__(kHvpBlock) void k_hvp(HvOp op, HvArgs a)
                 }
             }
         }
+        if (gn < g1) store_cols((it & 1) ? xbuf0 : xbuf1);
+        __syncthreads();  // next buffer written; this group's buffer free again
+        g = gn;
     }
     if constexpr (MODE == HV_LANCZOS) {
         double mine[1] = {sums[0]};
-        block_partial<1, kHvpBlock>(mine, a.partial);  // alpha_j = s_j * sum (finisher)
+        block_partial<1, BS>(mine, a.partial);  // alpha_j = s_j * sum (next kernel)
     } else if constexpr (MODE == HV_RQ) {
-        block_partial<3, kHvpBlock>(sums, a.partial);
+        block_partial<3, BS>(sums, a.partial);
     }
 }
 
-// LDS bytes of the persistent kernel (dynamic part).
-static inline size_t hvp_lds(const HvOp &op, int C, int KH, size_t esize)
+// Usable when the table, two column buffers and the register prefetch fit.
+static inline bool hv1t_fits(const HvOp &op, int C, size_t esize, size_t budget)
 {
-    return align16((size_t)op.nup * C * esize) + align16((size_t)op.nup * KH * 2) +
-           align16((size_t)op.nup * KH);
+    return op.ncu > 0 && op.up.KC == kHvtSlots && op.nup <= kHvtMaxRows &&
+           hvt_lds(op.nup, C, esize) <= budget;
 }
 
-template <typename T, typename ACC, int C, int KH>
-static int launch_hvp_c(hipStream_t s, const HvOp &op, int mode, const HvArgs &a)
+template <typename T, typename ACC, int C, int KH, int BS>
+static int launch_hv1t_c(hipStream_t s, const HvOp &op, int mode, const HvArgs &a)
 {
-    const size_t lds = hvp_lds(op, C, KH, sizeof(T));
+    const size_t lds = hvt_lds(op.nup, C, sizeof(T));
     const int ngroups = (op.ndn + C - 1) / C;
     const int grid = ngroups < op.ncu ? ngroups : op.ncu;
     void (*f)(HvOp, HvArgs);
-    if (mode == HV_PLAIN) f = k_hvp<T, ACC, C, HV_PLAIN, KH>;
-    else if (mode == HV_ACCUM) f = k_hvp<T, ACC, C, HV_ACCUM, KH>;
-    else if (mode == HV_LANCZOS) f = k_hvp<T, ACC, C, HV_LANCZOS, KH>;
-    else f = k_hvp<T, double, C, HV_RQ, KH>;
+    if (mode == HV_PLAIN) f = k_hv1t<T, ACC, C, HV_PLAIN, KH, BS>;
+    else if (mode == HV_ACCUM) f = k_hv1t<T, ACC, C, HV_ACCUM, KH, BS>;
+    else if (mode == HV_LANCZOS) f = k_hv1t<T, ACC, C, HV_LANCZOS, KH, BS>;
+    else f = k_hv1t<T, double, C, HV_RQ, KH, BS>;
     if (hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
         hipSuccess)
         return -1;
-    hipLaunchKernelGGL(f, dim3(grid), dim3(kHvpBlock), lds, s, op, a);
+    hipLaunchKernelGGL(f, dim3(grid), dim3(BS), lds, s, op, a);
     if (hipGetLastError() != hipSuccess) return -1;
     return finish_hv(s, mode, grid, a);
-}
-
-template <typename T, typename ACC, int KH>
-static int launch_hvp_k(hipStream_t s, const HvOp &op, int mode, int cols, const HvArgs &a)
-{
-    switch (cols) {
-    case 1: return launch_hvp_c<T, ACC, 1, KH>(s, op, mode, a);
-    case 2: return launch_hvp_c<T, ACC, 2, KH>(s, op, mode, a);
-    case 4: return launch_hvp_c<T, ACC, 4, KH>(s, op, mode, a);
-    default: return -2;
-    }
 }
 
 template <typename T, typename ACC, int C, int KH>
@@ -991,10 +1033,24 @@ static int launch_hv_a(hipStream_t s, const HvOp &op, int mode, int cols, const 
     const bool kan = op.norb >= 2;
     const int km = op.up.kmax > op.dn.kmax ? op.up.kmax : op.dn.kmax;
     if (kan) return launch_hv_k<T, ACC, true, 0>(s, op, mode, cols, a);
-    // persistent kernel: schedule 16/32/64 = C 1/2/4 with the table in LDS
-    if (op.norb == 1 && op.up.KC == 8 && km <= 8 && op.sched >= 16 && op.ncu > 0) {
-        const int pc = op.sched >> 4;
-        if (hvp_lds(op, pc, 8, sizeof(T)) <= kLdsBudget) return launch_hvp_k<T, ACC, 8>(s, op, mode, pc, a);
+    // persistent table-in-LDS kernel: schedule 16 = 1 column per group and
+    // 512 threads, 32 = 1 column and 1024 threads, 64 = 2 columns and 1024
+    // threads (1 column when 2 do not fit in LDS)
+    if (op.norb == 1 && km <= 8 && op.sched >= 16) {
+        int pc = op.sched >= 64 ? 2 : 1;
+        if (!hv1t_fits(op, pc, sizeof(T), kLdsBudget)) pc = 1;
+        if (hv1t_fits(op, pc, sizeof(T), kLdsBudget)) {
+            if (pc == 2) {
+                return km <= 7 ? launch_hv1t_c<T, ACC, 2, 7, 1024>(s, op, mode, a)
+                               : launch_hv1t_c<T, ACC, 2, 8, 1024>(s, op, mode, a);
+            }
+            if (op.sched >= 32) {
+                return km <= 7 ? launch_hv1t_c<T, ACC, 1, 7, 1024>(s, op, mode, a)
+                               : launch_hv1t_c<T, ACC, 1, 8, 1024>(s, op, mode, a);
+            }
+            return km <= 7 ? launch_hv1t_c<T, ACC, 1, 7, 512>(s, op, mode, a)
+                           : launch_hv1t_c<T, ACC, 1, 8, 512>(s, op, mode, a);
+        }
     }
     if (op.norb == 1 && op.up.KC == 8 && km <= 7) return launch_hv1_k<T, ACC, 7>(s, op, mode, cols, a);
     if (op.norb == 1 && op.up.KC == 8 && km <= 8) return launch_hv1_k<T, ACC, 8>(s, op, mode, cols, a);

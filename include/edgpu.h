@@ -73,8 +73,9 @@ int edg_set_memory_budget(edg_ctx *ctx, unsigned long long bytes);
 int edg_device_count(int *n);
 /* Kernel schedule of the H.v kernel: sector columns per workgroup (1, 2, 4
  * or 8, one workgroup per column group) or 16, 32, 64 for persistent
- * workgroups (one per compute unit, hop table resident in LDS) with 1, 2 or
- * 4 columns; and FP64 accumulation of the row sums (1) or accumulation in
+ * workgroups (one per compute unit, single-orbital hop table resident in
+ * LDS): 16 = 1 column and 512 threads, 32 = 1 column and 1024 threads,
+ * 64 = 2 columns and 1024 threads; and FP64 accumulation of the row sums (1) or accumulation in
  * the storage precision (0).  No counterpart in the reference (its hx is one
  * scalar loop); set by the autotuner or a caller that has measured. */
 int edg_set_tuning(edg_ctx *ctx, int cols_per_block, int fp64_accumulate);
