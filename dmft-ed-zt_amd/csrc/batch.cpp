@@ -1,0 +1,976 @@
+// Batched single-orbital ED solver and request loop (C ABI: edg_batch_* in
+// include/edgpu.h).
+//
+// Each problem moves through the stages of edg_ground_state +
+// edg_cluster_green:
+//   QUEUED   waiting for device memory (admission by bytes)
+//   GS       ground-state Lanczos chain (stored Krylov basis), batched
+//   GS_READ  alpha/beta on their way to the host; then the tridiagonal
+//            eigenvector, gs = sum_j c_j q_j, the FP64 Rayleigh quotient and
+//            the c+/c start vectors (per-problem launches)
+//   CHAINS   the c+/c Lanczos chains (ring of 3 vectors each), batched
+//   CH_READ  chain coefficients on their way to the host; then the continued
+//            fractions on the device
+//   CF_READ  G(i w) on its way to the host; then DONE and memory released.
+// Every Lanczos chain in flight (ground state or c+/c, any problem, any
+// step) is advanced by the same three launches per step (batch.hip); the
+// stages between chains are short per-problem launches on the same stream.
+// Host-side waits use events polled without blocking, so new problems are
+// admitted while others run (continuous batching).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <string>
+#include <vector>
+
+#include "../../include/edgpu.h"
+#include "edgpu_internal.h"
+
+namespace edg {
+int tridiag_lowest(int n, const double *d, const double *e, double *lambda, double *z);
+}
+
+using namespace edg;
+
+namespace {
+
+constexpr long long kPadB = 64;       // element alignment of vectors
+constexpr int kPartialCap = 1 << 17;  // doubles of the shared partial-sum scratch
+constexpr size_t kLdsMax = 152 * 1024;
+
+long long rup(long long n, long long m) { return (n + m - 1) / m * m; }
+
+long long binom_ll(int n, int k)
+{
+    if (k < 0 || k > n) return 0;
+    long long r = 1;
+    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+    return r;
+}
+
+// Largest hop count of one spin half with ne electrons, single orbital:
+// impurity occupied -> the empty bath sites, impurity empty -> the occupied
+// ones (zero hybridizations only lower it).
+int kmax_single(int nbath, int ne)
+{
+    int k = 0;
+    if (ne >= 1 && ne - 1 <= nbath) k = std::max(k, nbath - (ne - 1));
+    if (ne <= nbath) k = std::max(k, ne);
+    return k;
+}
+
+enum Stage { ST_QUEUED, ST_GS, ST_GS_READ, ST_CHAINS, ST_CH_READ, ST_CF_READ, ST_DONE };
+
+struct Chain {
+    BInst bi{};
+    int launched = 0;  // steps launched
+    bool exists = false;
+    bool active = false;
+    int ispin = 0, pm = 0;
+    long long D = 0;
+};
+
+// Bump allocator over one device block per problem.  A request past the
+// block's end is refused (returns the block start and sets overflow), and
+// callers check overflow before any kernel uses what they took.
+struct Arena {
+    char *base = nullptr;
+    size_t used = 0, cap = 0;
+    bool overflow = false;
+    void *take(size_t bytes)
+    {
+        const size_t off = (used + 255) & ~size_t(255);
+        if (off + bytes > cap) {
+            overflow = true;
+            return base;
+        }
+        used = off + bytes;
+        return base + off;
+    }
+};
+
+struct Job {
+    long long id = 0;
+    edg_problem p{};
+    std::vector<double> ek, vmk;
+    Stage st = ST_QUEUED;
+    size_t bytes = 0;
+    Arena ar;
+    double *dparams = nullptr;  // em(1,2) | ek(nbath,2) | vmk(1,nbath,2)
+    SpinTab tab[2][3];          // [spin][ne - N_spin + 1]
+    bool has_tab[2][3] = {};
+    void *gs = nullptr;
+    double *misc = nullptr;     // [1..3] RQ sums, [4] |combine|, [5] 1/|combine|, [8+k] |v_init| of chain k
+    double *coef = nullptr;     // device coefficient staging
+    double *gdev = nullptr;     // 2 spins x 2 nw
+    uint32_t *map[4] = {};
+    Chain gsc;
+    Chain ch[4];                // (up,+), (up,-), (dn,+), (dn,-)
+    double *host = nullptr;     // pinned readback / upload area
+    size_t host_n = 0;
+    hipEvent_t ev = nullptr;
+    double e0 = 0.0;
+    int gs_nstep = 0;
+    std::vector<double> G;
+    std::vector<double> coeffs;  // (mg, 4, 2) column-major: ap bp am bm per spin
+};
+
+}  // namespace
+
+struct edg_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool fp64 = false;
+    size_t es = 4;
+    int nspin = 1;
+    int nw = 0;
+    int ncu = 0;
+    double *omega = nullptr;
+    unsigned long long budget = 0, in_use = 0;
+    std::string err;
+    std::deque<Job *> queue;
+    std::vector<Job *> running;
+    std::deque<Job *> finished;
+    long long next_id = 1;
+    double *partial = nullptr;
+    unsigned *ticket = nullptr;
+    // chains in the batched launches
+    std::vector<Chain *> active;
+    bool dirty = true;
+    char *ddesc = nullptr;
+    size_t ddesc_cap = 0;
+    char *hdesc[2] = {nullptr, nullptr};
+    size_t hdesc_cap[2] = {0, 0};
+    hipEvent_t hdesc_ev[2] = {nullptr, nullptr};
+    bool hdesc_used[2] = {false, false};
+    int flip = 0;
+    const BInst *d_inst = nullptr;
+    const int *d_hvoff = nullptr, *d_axoff = nullptr;
+    int ntiles = 0, nax = 0, lds_elems = 0;
+    bool use_graphs = true;
+    hipGraphExec_t gexec = nullptr;
+    bool graph_valid = false;
+    // replaced graphs may still have launches queued: destroyed once the
+    // stream has passed the event recorded after their last launch
+    std::vector<std::pair<hipGraphExec_t, hipEvent_t>> retired;
+    std::vector<double *> pinned_free;
+    size_t pinned_n = 0;
+    double st_steps = 0, st_chain_steps = 0, st_done = 0, st_graph = 0, st_rebuild = 0;
+};
+
+namespace {
+
+int fail(edg_batch *b, int code, const std::string &msg)
+{
+    b->err = msg;
+    return code;
+}
+
+#define BCHK(b, call)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) return fail(b, EDG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int nsite_of(const edg_problem &p) { return 1 + p.nbath; }
+int green_len(const edg_problem &p) { return p.maxnstep_green > 0 ? p.maxnstep_green : p.maxnstep; }
+
+// Chains a problem runs: spin-down chains only when spin symmetry does not
+// give them (dmft_green.f90:76-80) or when both_spins is set.
+bool runs_spin2(const edg_batch *b, const edg_problem &p)
+{
+    const bool even = !(b->nspin == 2 || p.nup != p.ndn);
+    return !even || p.both_spins;
+}
+
+size_t tab_bytes(int n, int nbath)
+{
+    const size_t K = std::max(nbath, kMinSlots);
+    return 4 * (size_t)n + K * n * (sizeof(HopF) + sizeof(HopD)) + n + 4 + 8 * (size_t)n + 4 * (size_t)n +
+           n + 16 * (size_t)n * (2 + 1 + 4) + 16 * 256;
+}
+
+size_t state_bytes(int maxn, int hvn, int axn)
+{
+    return 256 * 8 + ((size_t)maxn + 1) * 3 * 8 + 16 + (size_t)(hvn + axn) * 8;
+}
+
+int chain_cb(int nup)
+{
+    int cb = kBatchTile / std::max(nup, 1);
+    return std::min(std::max(cb, 1), kBatchMaxCols);
+}
+
+int chain_axn(long long D)
+{
+    long long g = (D + kBlock * 8 - 1) / (kBlock * 8);
+    return (int)std::min<long long>(std::max<long long>(g, 1), 1024);
+}
+
+// Device bytes of a problem in flight.
+size_t problem_bytes(const edg_batch *b, const edg_problem &p)
+{
+    const int ns = nsite_of(p);
+    size_t bytes = 64 * 1024 + 8 * (size_t)(2 + 4 * p.nbath);
+    for (int s = 0; s < 2; ++s) {
+        const int n0 = s == 0 ? p.nup : p.ndn;
+        for (int d = -1; d <= 1; ++d)
+            if (n0 + d >= 0 && n0 + d <= ns) bytes += tab_bytes((int)binom_ll(ns, n0 + d), p.nbath);
+    }
+    const long long nu = binom_ll(ns, p.nup), nd = binom_ll(ns, p.ndn);
+    const long long D = nu * nd, Dp = rup(D, kPadB);
+    bytes += ((size_t)p.maxnstep + 1) * Dp * b->es + Dp * b->es;
+    bytes += state_bytes(p.maxnstep, (int)((nd + chain_cb((int)nu) - 1) / chain_cb((int)nu)), chain_axn(D));
+    const int mg = green_len(p);
+    bytes += 64 * 8 + 8 * (8 * (size_t)std::max(mg, p.maxnstep) + 64) + 8 * 4 * (size_t)b->nw;
+    for (int k = 0; k < 4; ++k) {
+        const int ispin = k < 2 ? 1 : 2, pm = (k & 1) ? 2 : 1;
+        const int cu = p.nup + (ispin == 1 ? (pm == 1 ? 1 : -1) : 0);
+        const int cd = p.ndn + (ispin == 2 ? (pm == 1 ? 1 : -1) : 0);
+        if (cu < 0 || cd < 0 || cu > ns || cd > ns) continue;
+        const long long cnu = binom_ll(ns, cu), cnd = binom_ll(ns, cd), Dc = cnu * cnd;
+        bytes += 3 * rup(Dc, kPadB) * b->es + 4 * (size_t)(ispin == 1 ? cnu : cnd) + 256;
+        bytes += state_bytes(mg, (int)((cnd + chain_cb((int)cnu) - 1) / chain_cb((int)cnu)), chain_axn(Dc));
+    }
+    return bytes + 64 * 256;
+}
+
+int validate(const edg_batch *b, const edg_problem &p)
+{
+    const int ns = nsite_of(p);
+    if (p.nbath < 1 || ns > 15) return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "nbath must be 1..14");
+    if (!p.ek || !p.vmk) return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "ek / vmk missing");
+    if (p.nup < 0 || p.ndn < 0 || p.nup > ns || p.ndn > ns)
+        return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "sector outside 0..nsite");
+    if (p.maxnstep < 1 || p.maxnstep > 1000 || green_len(p) > 1000)
+        return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "Lanczos lengths must be 1..1000");
+    for (int s = 0; s < 2; ++s) {
+        const int n0 = s == 0 ? p.nup : p.ndn;
+        for (int d = -1; d <= 1; ++d) {
+            const int ne = n0 + d;
+            if (ne < 0 || ne > ns) continue;
+            if (kmax_single(p.nbath, ne) > 16)
+                return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "more than 16 hops per configuration");
+            const long long n = binom_ll(ns, ne);
+            if ((size_t)std::max<long long>(n, kBatchTile) * b->es > kLdsMax)
+                return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "sector column exceeds LDS");
+        }
+    }
+    return 0;
+}
+
+double *pinned_get(edg_batch *b)
+{
+    if (!b->pinned_free.empty()) {
+        double *p = b->pinned_free.back();
+        b->pinned_free.pop_back();
+        return p;
+    }
+    double *p = nullptr;
+    if (hipHostMalloc((void **)&p, b->pinned_n * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return p;
+}
+
+int build_table(edg_batch *b, Job *J, int spin, int ne, SpinTab &t)
+{
+    const int ns = nsite_of(J->p), nbath = J->p.nbath;
+    t = SpinTab{};
+    t.n = (int)binom_ll(ns, ne);
+    t.K = std::max(nbath, kMinSlots);
+    const size_t n = t.n, K = t.K;
+    t.pattern = (uint32_t *)J->ar.take(4 * n);
+    t.hf = (HopF *)J->ar.take(K * n * sizeof(HopF));
+    t.hd = (HopD *)J->ar.take(K * n * sizeof(HopD));
+    t.cnt = (uint8_t *)J->ar.take(n);
+    t.maxcnt = (int *)J->ar.take(4);
+    t.energy = (double *)J->ar.take(8 * n);
+    t.energyf = (float *)J->ar.take(4 * n);
+    t.mask = (uint8_t *)J->ar.take(n);
+    t.kmax = kmax_single(nbath, ne);
+    t.KC = t.kmax <= 8 ? 8 : 16;
+    t.ccol = (uint16_t *)J->ar.take(16 * n * 2);
+    t.cidx = (uint8_t *)J->ar.take(16 * n);
+    t.hrec = (uint32_t *)J->ar.take(16 * n * 4);
+    if (J->ar.overflow) return fail(b, EDG_ERR_NOMEM, "problem arena overflow (tables)");
+    const double *em_s = J->dparams + (spin - 1);
+    const double *ek_s = J->dparams + 2 + (size_t)(spin - 1) * nbath;
+    const double *v_s = J->dparams + 2 + 2 * (size_t)nbath + (size_t)(spin - 1) * nbath;
+    t.vsrc = v_s;
+    BCHK(b, hipMemsetAsync(t.maxcnt, 0, sizeof(int), b->stream));
+    if (launch_spin_table(b->stream, ns, 1, nbath, ne, em_s, ek_s, v_s, J->p.mu, 0.0, t))
+        return fail(b, EDG_ERR_HIP, "spin table launch failed");
+    if (launch_compress(b->stream, t)) return fail(b, EDG_ERR_HIP, "compress launch failed");
+    return 0;
+}
+
+HvOp make_op(const edg_batch *b, const Job *J, int nu, int nd)
+{
+    HvOp op{};
+    const SpinTab &tu = J->tab[0][nu - J->p.nup + 1];
+    const SpinTab &td = J->tab[1][nd - J->p.ndn + 1];
+    op.nup = tu.n;
+    op.ndn = td.n;
+    op.norb = 1;
+    op.nvup = J->p.nbath;
+    op.ncu = b->ncu;
+    op.sched = 0;
+    op.up = tu;
+    op.dn = td;
+    op.U = J->p.U;
+    op.Up = 0.0;
+    op.Jp = 0.0;
+    return op;
+}
+
+// Lay out one chain's vectors and state; the start vector goes in slot 0.
+int init_chain(edg_batch *b, Job *J, Chain &C, const HvOp &op, int maxn, bool stored)
+{
+    const long long D = (long long)op.nup * op.ndn, Dp = rup(D, kPadB);
+    BInst &I = C.bi;
+    I = BInst{};
+    I.op = op;
+    I.nslot = stored ? maxn + 1 : 3;
+    I.stride = Dp;
+    I.base = J->ar.take((size_t)I.nslot * Dp * b->es);
+    I.maxn = maxn;
+    I.cb = chain_cb(op.nup);
+    I.hvn = (op.ndn + I.cb - 1) / I.cb;
+    I.axn = chain_axn(D);
+    if ((size_t)I.cb * op.nup * b->es > kLdsMax) return fail(b, EDG_ERR_ARG, "tile exceeds LDS");
+    char *st = (char *)J->ar.take(state_bytes(maxn, I.hvn, I.axn));
+    I.step = (int *)st;
+    I.flags = (int *)(st + 8);
+    I.alpha = (double *)(st + 256);
+    I.beta = I.alpha + maxn;
+    I.scale = I.beta + maxn + 1;
+    I.hvpart = I.scale + maxn + 1;
+    I.axpart = I.hvpart + I.hvn;
+    if (J->ar.overflow) return fail(b, EDG_ERR_NOMEM, "problem arena overflow (chain)");
+    BCHK(b, hipMemsetAsync(st, 0, state_bytes(maxn, I.hvn, I.axn), b->stream));
+    C.D = D;
+    C.exists = true;
+    C.launched = 0;
+    return 0;
+}
+
+// flags = {0, maxn}, scale[0] = 1/|slot 0| (norm into norm_out).  Chain
+// number slot (0 = ground state, 1..4 = c+/c) owns its own pinned flag pair,
+// so no upload waits for an earlier one.
+int start_chain(edg_batch *b, Job *J, Chain &C, double *norm_out, int slot)
+{
+    BInst &I = C.bi;
+    int *h = (int *)(J->host + J->host_n - 8) + 2 * slot;
+    h[0] = 0;
+    h[1] = I.maxn;
+    BCHK(b, hipMemcpyAsync(I.flags, h, 2 * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    int rc = b->fp64 ? launch_norm<double>(b->stream, (const double *)I.base, C.D, b->partial, b->ticket,
+                                           norm_out, I.scale, nullptr)
+                     : launch_norm<float>(b->stream, (const float *)I.base, C.D, b->partial, b->ticket,
+                                          norm_out, I.scale, nullptr);
+    if (rc) return fail(b, EDG_ERR_HIP, "norm launch failed");
+    C.active = true;
+    b->active.push_back(&C);
+    b->dirty = true;
+    return 0;
+}
+
+// Returns 1 (nothing changed) when the device cannot hold the problem now.
+int admit(edg_batch *b, Job *J)
+{
+    const edg_problem &p = J->p;
+    void *mem = nullptr;
+    const hipError_t ae = hipMallocAsync(&mem, J->bytes, b->stream);
+    if (ae == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    if (ae != hipSuccess) return fail(b, EDG_ERR_HIP, std::string("hipMallocAsync: ") + hipGetErrorString(ae));
+    J->ar.base = (char *)mem;
+    J->ar.used = 0;
+    J->ar.cap = J->bytes;
+    b->in_use += J->bytes;
+    J->host = pinned_get(b);
+    if (!J->host) return fail(b, EDG_ERR_NOMEM, "pinned host allocation failed");
+    J->host_n = b->pinned_n;
+    BCHK(b, hipEventCreateWithFlags(&J->ev, hipEventDisableTiming));
+    const int nbath = p.nbath, ns = nsite_of(p);
+    J->dparams = (double *)J->ar.take(8 * (2 + 4 * (size_t)nbath));
+    if (J->ar.overflow) return fail(b, EDG_ERR_NOMEM, "problem arena overflow");
+    double *h = J->host;
+    h[0] = p.em[0];
+    h[1] = p.em[1];
+    std::memcpy(h + 2, J->ek.data(), sizeof(double) * 2 * nbath);
+    std::memcpy(h + 2 + 2 * nbath, J->vmk.data(), sizeof(double) * 2 * nbath);
+    BCHK(b, hipMemcpyAsync(J->dparams, h, 8 * (2 + 4 * (size_t)nbath), hipMemcpyHostToDevice, b->stream));
+    for (int s = 0; s < 2; ++s) {
+        const int n0 = s == 0 ? p.nup : p.ndn;
+        for (int d = -1; d <= 1; ++d) {
+            const int ne = n0 + d;
+            J->has_tab[s][d + 1] = ne >= 0 && ne <= ns;
+            if (J->has_tab[s][d + 1])
+                if (int rc = build_table(b, J, s + 1, ne, J->tab[s][d + 1])) return rc;
+        }
+    }
+    J->misc = (double *)J->ar.take(64 * 8);
+    J->coef = (double *)J->ar.take(8 * (8 * (size_t)std::max(green_len(p), p.maxnstep) + 64));
+    J->gdev = (double *)J->ar.take(8 * 4 * (size_t)b->nw);
+    const HvOp op = make_op(b, J, p.nup, p.ndn);
+    const long long Dp = rup((long long)op.nup * op.ndn, kPadB);
+    J->gs = J->ar.take(Dp * b->es);
+    if (J->ar.overflow) return fail(b, EDG_ERR_NOMEM, "problem arena overflow");
+    if (int rc = init_chain(b, J, J->gsc, op, p.maxnstep, true)) return rc;
+    const long long D = J->gsc.D;
+    const int rc0 = b->fp64 ? launch_start<double>(b->stream, (double *)J->gsc.bi.base, D, p.seed)
+                            : launch_start<float>(b->stream, (float *)J->gsc.bi.base, D, p.seed);
+    if (rc0) return fail(b, EDG_ERR_HIP, "start launch failed");
+    if (int rc = start_chain(b, J, J->gsc, &J->misc[0], 0)) return rc;
+    J->st = ST_GS;
+    b->running.push_back(J);
+    return 0;
+}
+
+// Queue the device->host copy of a finished chain's alpha / beta / scale /
+// flags into the pinned area at off; returns the doubles used.
+size_t read_chain(edg_batch *b, Job *J, const Chain &C, size_t off)
+{
+    const int m = C.bi.maxn;
+    double *h = J->host + off;
+    (void)hipMemcpyAsync(h, C.bi.alpha, sizeof(double) * m, hipMemcpyDeviceToHost, b->stream);
+    (void)hipMemcpyAsync(h + m, C.bi.beta, sizeof(double) * (m + 1), hipMemcpyDeviceToHost, b->stream);
+    (void)hipMemcpyAsync(h + 2 * m + 1, C.bi.scale, sizeof(double) * (m + 1), hipMemcpyDeviceToHost,
+                         b->stream);
+    (void)hipMemcpyAsync(h + 3 * m + 2, C.bi.flags, sizeof(int) * 2, hipMemcpyDeviceToHost, b->stream);
+    return 3 * (size_t)m + 4;
+}
+
+// a, b of run_lanczos from the raw arrays at h (maxn m).
+int chain_ab(const double *h, int m, double *a, double *bb)
+{
+    const int *fl = (const int *)(h + 3 * m + 2);
+    const int ns = fl[1];
+    const double *ha = h, *hb = h + m;
+    for (int k = 0; k < m; ++k) {
+        a[k] = k < ns ? ha[k] : 0.0;
+        bb[k] = (k >= 1 && k <= ns) ? hb[k] : 0.0;
+    }
+    if (ns < m && ns >= 1) bb[ns] = hb[ns];
+    bb[0] = 0.0;
+    return ns;
+}
+
+// Ground-state chain done: eigenvector, E0 and the c+/c start vectors.
+int after_gs(edg_batch *b, Job *J)
+{
+    const edg_problem &p = J->p;
+    const int m = p.maxnstep;
+    std::vector<double> a(m), bb(m);
+    const int nst = chain_ab(J->host, m, a.data(), bb.data());
+    const double *s = J->host + 2 * m + 1;
+    J->gs_nstep = nst;
+    if (nst < 1) return fail(b, EDG_ERR_SOLVER, "ground-state chain broke down at its start");
+    std::vector<double> z(nst);
+    double lam;
+    if (tridiag_lowest(nst, a.data(), bb.data() + 1, &lam, z.data()))
+        return fail(b, EDG_ERR_SOLVER, "tridiagonal eigensolve failed");
+    double *hc = J->host + 4 * (size_t)m + 8;
+    for (int j = 0; j < nst; ++j) hc[j] = z[j] * s[j];
+    BCHK(b, hipMemcpyAsync(J->coef, hc, sizeof(double) * nst, hipMemcpyHostToDevice, b->stream));
+    const HvOp op = J->gsc.bi.op;
+    const long long D = J->gsc.D;
+    int rc;
+    if (b->fp64) {
+        rc = launch_combine<double>(b->stream, (double *)J->gs, (const double *)J->gsc.bi.base, J->gsc.bi.stride,
+                                    nst, J->coef, D, b->partial, b->ticket, &J->misc[4], &J->misc[5]);
+        rc |= launch_scale<double>(b->stream, (double *)J->gs, D, &J->misc[5]);
+    } else {
+        rc = launch_combine<float>(b->stream, (float *)J->gs, (const float *)J->gsc.bi.base, J->gsc.bi.stride,
+                                   nst, J->coef, D, b->partial, b->ticket, &J->misc[4], &J->misc[5]);
+        rc |= launch_scale<float>(b->stream, (float *)J->gs, D, &J->misc[5]);
+    }
+    if (rc) return fail(b, EDG_ERR_HIP, "combine launch failed");
+    HvArgs ha{};
+    ha.x = J->gs;
+    ha.partial = b->partial;
+    ha.result = &J->misc[1];
+    const int hrc = b->fp64 ? launch_hv<double>(b->stream, op, HV_RQ, 2, true, ha)
+                            : launch_hv<float>(b->stream, op, HV_RQ, 2, true, ha);
+    if (hrc < 0) return fail(b, EDG_ERR_HIP, "Rayleigh-quotient launch failed");
+    // c+/c start vectors (dmft_green.f90:202-257), one ring per chain
+    const int ns_site = nsite_of(p), mg = green_len(p);
+    const bool spin2 = runs_spin2(b, p);
+    for (int k = 0; k < 4; ++k) {
+        Chain &C = J->ch[k];
+        C.ispin = k < 2 ? 1 : 2;
+        C.pm = (k & 1) ? 2 : 1;
+        C.exists = false;
+        if (C.ispin == 2 && !spin2) continue;
+        const int nu = p.nup + (C.ispin == 1 ? (C.pm == 1 ? 1 : -1) : 0);
+        const int nd = p.ndn + (C.ispin == 2 ? (C.pm == 1 ? 1 : -1) : 0);
+        if (nu < 0 || nd < 0 || nu > ns_site || nd > ns_site) continue;
+        const HvOp cop = make_op(b, J, nu, nd);
+        if (int r = init_chain(b, J, C, cop, mg, false)) return r;
+        const SpinTab &tout = C.ispin == 1 ? cop.up : cop.dn;
+        J->map[k] = (uint32_t *)J->ar.take(4 * (size_t)tout.n);
+        if (J->ar.overflow) return fail(b, EDG_ERR_NOMEM, "problem arena overflow (map)");
+        const int ne_in = C.ispin == 1 ? p.nup : p.ndn;
+        if (launch_cmap(b->stream, tout.pattern, tout.n, ns_site, ne_in, 0, C.pm, J->map[k]))
+            return fail(b, EDG_ERR_HIP, "cmap launch failed");
+        const int godd = C.ispin == 2 ? (p.nup & 1) : 0;
+        const int arc = b->fp64 ? launch_apply_c<double>(b->stream, (const double *)J->gs, (double *)C.bi.base,
+                                                         cop.nup, cop.ndn, op.nup, C.ispin, J->map[k], godd)
+                                : launch_apply_c<float>(b->stream, (const float *)J->gs, (float *)C.bi.base,
+                                                        cop.nup, cop.ndn, op.nup, C.ispin, J->map[k], godd);
+        if (arc) return fail(b, EDG_ERR_HIP, "apply_c launch failed");
+        if (int r = start_chain(b, J, C, &J->misc[8 + k], 1 + k)) return r;
+    }
+    J->st = ST_CHAINS;
+    return 0;
+}
+
+bool chains_running(const Job *J)
+{
+    for (int k = 0; k < 4; ++k)
+        if (J->ch[k].exists && J->ch[k].active) return true;
+    return false;
+}
+
+// Chain coefficients -> continued fractions on the device.
+int after_chains(edg_batch *b, Job *J)
+{
+    const edg_problem &p = J->p;
+    const int mg = green_len(p);
+    const double *h = J->host;
+    const double *rq = h;  // misc[0..7] copied first
+    J->e0 = rq[1] / rq[3];
+    // per chain: ap/bp (k even) or am/bm (k odd), b[0] = |v_init|
+    std::vector<double> co[2];
+    int nsv[2][2] = {{0, 0}, {0, 0}};
+    for (int s = 0; s < 2; ++s) co[s].assign(4 * (size_t)mg, 0.0);
+    for (int k = 0; k < 4; ++k) {
+        const Chain &C = J->ch[k];
+        if (!C.exists) continue;
+        const double *raw = h + 16 + (size_t)k * (3 * mg + 4);
+        const int s = C.ispin - 1, q = C.pm == 1 ? 0 : 2;
+        const int ns = chain_ab(raw, mg, co[s].data() + q * mg, co[s].data() + (q + 1) * mg);
+        co[s][(q + 1) * mg] = rq[8 + k];
+        nsv[s][C.pm - 1] = ns;
+    }
+    const bool spin2 = runs_spin2(b, p);
+    if (!spin2) {  // the reference reuses the spin-1 coefficients (:102-109)
+        co[1] = co[0];
+        nsv[1][0] = nsv[0][0];
+        nsv[1][1] = nsv[0][1];
+    }
+    J->coeffs.resize(8 * (size_t)mg);
+    std::memcpy(J->coeffs.data(), co[0].data(), sizeof(double) * 4 * mg);
+    std::memcpy(J->coeffs.data() + 4 * mg, co[1].data(), sizeof(double) * 4 * mg);
+    double *hc = J->host + 16 + 4 * (size_t)(3 * mg + 4);
+    const int nspin_run = spin2 ? 2 : 1;
+    for (int s = 0; s < nspin_run; ++s) {
+        std::memcpy(hc + (size_t)s * 4 * mg, co[s].data(), sizeof(double) * 4 * mg);
+        const double *dc = J->coef + (size_t)s * 4 * mg;
+        BCHK(b, hipMemcpyAsync((void *)dc, hc + (size_t)s * 4 * mg, sizeof(double) * 4 * mg, hipMemcpyHostToDevice,
+                               b->stream));
+        if (launch_cf(b->stream, b->nw, b->omega, J->e0, nsv[s][0], dc, dc + mg, nsv[s][1], dc + 2 * mg, dc + 3 * mg,
+                      J->gdev + (size_t)s * 2 * b->nw, 0))
+            return fail(b, EDG_ERR_HIP, "continued fraction launch failed");
+    }
+    double *hg = J->host + 16 + 4 * (size_t)(3 * mg + 4) + 8 * (size_t)mg;
+    BCHK(b, hipMemcpyAsync(hg, J->gdev, sizeof(double) * 2 * b->nw * nspin_run, hipMemcpyDeviceToHost, b->stream));
+    BCHK(b, hipEventRecord(J->ev, b->stream));
+    J->st = ST_CF_READ;
+    return 0;
+}
+
+// G_cl assembly (dmft_green.f90:111-118) and release.
+int finish(edg_batch *b, Job *J)
+{
+    const edg_problem &p = J->p;
+    const int mg = green_len(p), nw = b->nw;
+    const double *hg = J->host + 16 + 4 * (size_t)(3 * mg + 4) + 8 * (size_t)mg;
+    const bool even = !(b->nspin == 2 || p.nup != p.ndn);
+    const bool spin2 = runs_spin2(b, p);
+    J->G.assign(2 * (size_t)nw * b->nspin, 0.0);
+    for (int iw = 0; iw < nw; ++iw) {
+        const double r1 = hg[2 * iw], i1 = hg[2 * iw + 1];
+        const double r2 = spin2 ? hg[2 * nw + 2 * iw] : r1, i2 = spin2 ? hg[2 * nw + 2 * iw + 1] : i1;
+        if (b->nspin == 1 && even) {
+            J->G[2 * iw] = r1;
+            J->G[2 * iw + 1] = i1;
+        } else if (b->nspin == 1) {
+            J->G[2 * iw] = 0.5 * (r1 + r2);
+            J->G[2 * iw + 1] = 0.5 * (i1 + i2);
+        } else {
+            J->G[2 * iw] = r1;
+            J->G[2 * iw + 1] = i1;
+            J->G[2 * (size_t)nw + 2 * iw] = r2;
+            J->G[2 * (size_t)nw + 2 * iw + 1] = i2;
+        }
+    }
+    BCHK(b, hipFreeAsync(J->ar.base, b->stream));
+    J->ar.base = nullptr;
+    b->in_use -= J->bytes;
+    b->pinned_free.push_back(J->host);
+    J->host = nullptr;
+    (void)hipEventDestroy(J->ev);
+    J->ev = nullptr;
+    J->st = ST_DONE;
+    b->st_done += 1;
+    return 0;
+}
+
+// Event-driven stage transitions of every running problem (no blocking).
+int progress(edg_batch *b)
+{
+    for (size_t i = 0; i < b->running.size();) {
+        Job *J = b->running[i];
+        if (J->st == ST_GS && !J->gsc.active) {
+            read_chain(b, J, J->gsc, 0);
+            BCHK(b, hipEventRecord(J->ev, b->stream));
+            J->st = ST_GS_READ;
+        } else if (J->st == ST_GS_READ && hipEventQuery(J->ev) == hipSuccess) {
+            if (int rc = after_gs(b, J)) return rc;
+        } else if (J->st == ST_CHAINS && !chains_running(J)) {
+            BCHK(b, hipMemcpyAsync(J->host, J->misc, 16 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+            const int mg = green_len(J->p);
+            for (int k = 0; k < 4; ++k)
+                if (J->ch[k].exists) read_chain(b, J, J->ch[k], 16 + (size_t)k * (3 * mg + 4));
+            BCHK(b, hipEventRecord(J->ev, b->stream));
+            J->st = ST_CH_READ;
+        } else if (J->st == ST_CH_READ && hipEventQuery(J->ev) == hipSuccess) {
+            if (int rc = after_chains(b, J)) return rc;
+        } else if (J->st == ST_CF_READ && hipEventQuery(J->ev) == hipSuccess) {
+            if (int rc = finish(b, J)) return rc;
+            b->finished.push_back(J);
+            b->running.erase(b->running.begin() + i);
+            continue;
+        }
+        ++i;
+    }
+    return 0;
+}
+
+// Upload the descriptor table of the active chains.
+int rebuild(edg_batch *b)
+{
+    const int n = (int)b->active.size();
+    std::vector<int> hv(n + 1), ax(n + 1);
+    hv[0] = ax[0] = 0;
+    int lds = 1;
+    for (int i = 0; i < n; ++i) {
+        const BInst &I = b->active[i]->bi;
+        hv[i + 1] = hv[i] + I.hvn;
+        ax[i + 1] = ax[i] + I.axn;
+        lds = std::max(lds, I.cb * I.op.nup);
+    }
+    const size_t bi_bytes = sizeof(BInst) * std::max(n, 1);
+    const size_t off_hv = (bi_bytes + 255) & ~size_t(255);
+    const size_t off_ax = off_hv + (((size_t)(n + 1) * 4 + 255) & ~size_t(255));
+    const size_t total = off_ax + (size_t)(n + 1) * 4;
+    const int f = b->flip;
+    if (b->hdesc_used[f]) BCHK(b, hipEventSynchronize(b->hdesc_ev[f]));
+    if (b->hdesc_cap[f] < total) {
+        if (b->hdesc[f]) (void)hipHostFree(b->hdesc[f]);
+        BCHK(b, hipHostMalloc((void **)&b->hdesc[f], total * 2, hipHostMallocDefault));
+        b->hdesc_cap[f] = total * 2;
+    }
+    if (b->ddesc_cap < total) {
+        if (b->ddesc) BCHK(b, hipFreeAsync(b->ddesc, b->stream));
+        BCHK(b, hipMallocAsync((void **)&b->ddesc, total * 2, b->stream));
+        b->ddesc_cap = total * 2;
+    }
+    char *h = b->hdesc[f];
+    for (int i = 0; i < n; ++i) std::memcpy(h + sizeof(BInst) * i, &b->active[i]->bi, sizeof(BInst));
+    std::memcpy(h + off_hv, hv.data(), 4 * (size_t)(n + 1));
+    std::memcpy(h + off_ax, ax.data(), 4 * (size_t)(n + 1));
+    BCHK(b, hipMemcpyAsync(b->ddesc, h, total, hipMemcpyHostToDevice, b->stream));
+    BCHK(b, hipEventRecord(b->hdesc_ev[f], b->stream));
+    b->hdesc_used[f] = true;
+    b->flip ^= 1;
+    b->d_inst = (const BInst *)b->ddesc;
+    b->d_hvoff = (const int *)(b->ddesc + off_hv);
+    b->d_axoff = (const int *)(b->ddesc + off_ax);
+    b->ntiles = hv[n];
+    b->nax = ax[n];
+    b->lds_elems = lds;
+    b->dirty = false;
+    b->st_rebuild += 1;
+    if (b->gexec) {
+        hipEvent_t ev;
+        BCHK(b, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        BCHK(b, hipEventRecord(ev, b->stream));
+        b->retired.emplace_back(b->gexec, ev);
+        b->gexec = nullptr;
+    }
+    b->graph_valid = false;
+    for (size_t i = 0; i < b->retired.size();) {
+        if (hipEventQuery(b->retired[i].second) == hipSuccess) {
+            (void)hipGraphExecDestroy(b->retired[i].first);
+            (void)hipEventDestroy(b->retired[i].second);
+            b->retired.erase(b->retired.begin() + i);
+            continue;
+        }
+        ++i;
+    }
+    return 0;
+}
+
+int enqueue_step(edg_batch *b)
+{
+    const int n = (int)b->active.size();
+    if (launch_batch_hv(b->stream, b->fp64, b->d_inst, b->d_hvoff, n, b->ntiles, b->lds_elems))
+        return fail(b, EDG_ERR_HIP, "batched H.v launch failed");
+    if (launch_batch_axpy(b->stream, b->fp64, b->d_inst, b->d_axoff, n, b->nax))
+        return fail(b, EDG_ERR_HIP, "batched axpy launch failed");
+    if (launch_batch_advance(b->stream, b->d_inst, n)) return fail(b, EDG_ERR_HIP, "advance launch failed");
+    return 0;
+}
+
+// Run k batched steps of the active chains (k <= every chain's remainder).
+int run_steps(edg_batch *b, int k)
+{
+    if (b->dirty)
+        if (int rc = rebuild(b)) return rc;
+    if (b->use_graphs && !b->graph_valid) {
+        hipGraph_t g = nullptr;
+        BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_step(b);
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(b, EDG_ERR_HIP, "graph capture failed");
+        e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(b, EDG_ERR_HIP, "graph instantiation failed");
+        b->graph_valid = true;
+    }
+    for (int s = 0; s < k; ++s) {
+        if (b->use_graphs) {
+            BCHK(b, hipGraphLaunch(b->gexec, b->stream));
+            b->st_graph += 1;
+        } else if (int rc = enqueue_step(b)) {
+            return rc;
+        }
+    }
+    b->st_steps += k;
+    b->st_chain_steps += (double)k * b->active.size();
+    for (size_t i = 0; i < b->active.size();) {
+        Chain *C = b->active[i];
+        C->launched += k;
+        if (C->launched >= C->bi.maxn) {
+            C->active = false;
+            b->active.erase(b->active.begin() + i);
+            b->dirty = true;
+            continue;
+        }
+        ++i;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int edg_batch_create(int device, int precision, int nspin, int nw, const double *omega,
+                     unsigned long long budget_bytes, edg_batch **out)
+{
+    if (!out) return EDG_ERR_ARG;
+    *out = nullptr;
+    if ((precision != EDG_FP32 && precision != EDG_FP64) || (nspin != 1 && nspin != 2) || nw < 1 || !omega)
+        return EDG_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return EDG_ERR_HIP;
+    edg_batch *b = new edg_batch;
+    b->device = device;
+    b->fp64 = precision == EDG_FP64;
+    b->es = b->fp64 ? 8 : 4;
+    b->nspin = nspin;
+    b->nw = nw;
+    b->pinned_n = 16 + 4 * (3 * 1000 + 4) + 8 * 1000 + 4 * (size_t)nw + 8 * 1000 + 64;
+    int rc = 0;
+    auto chk = [&](hipError_t e) {
+        if (e != hipSuccess && !rc) rc = EDG_ERR_HIP;
+    };
+    chk(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    chk(hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, device));
+    chk(hipMalloc((void **)&b->omega, sizeof(double) * nw));
+    chk(hipMalloc((void **)&b->partial, sizeof(double) * kPartialCap));
+    chk(hipMalloc((void **)&b->ticket, 256));
+    chk(hipEventCreateWithFlags(&b->hdesc_ev[0], hipEventDisableTiming));
+    chk(hipEventCreateWithFlags(&b->hdesc_ev[1], hipEventDisableTiming));
+    if (!rc) chk(hipMemcpy(b->omega, omega, sizeof(double) * nw, hipMemcpyHostToDevice));
+    if (!rc) {  // keep freed problem memory in the stream-ordered pool for the next admission
+        hipMemPool_t pool;
+        chk(hipDeviceGetDefaultMemPool(&pool, device));
+        uint64_t keep = ~0ull;
+        if (!rc) chk(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+    }
+    if (!rc && batch_set_lds(kLdsMax)) rc = EDG_ERR_HIP;
+    if (!rc && budget_bytes == 0) {
+        size_t fr = 0, tot = 0;
+        chk(hipMemGetInfo(&fr, &tot));
+        budget_bytes = (unsigned long long)(0.85 * (double)fr);
+    }
+    if (rc) {
+        (void)hipGetLastError();
+        edg_batch_destroy(b);
+        return rc;
+    }
+    b->budget = budget_bytes;
+    *out = b;
+    return EDG_OK;
+}
+
+int edg_batch_destroy(edg_batch *b)
+{
+    if (!b) return EDG_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    auto drop = [&](Job *J) {
+        if (J->ar.base) {
+            (void)hipFreeAsync(J->ar.base, b->stream);
+            (void)hipStreamSynchronize(b->stream);
+        }
+        if (J->host) (void)hipHostFree(J->host);
+        if (J->ev) (void)hipEventDestroy(J->ev);
+        delete J;
+    };
+    for (Job *J : b->queue) drop(J);
+    for (Job *J : b->running) drop(J);
+    for (Job *J : b->finished) drop(J);
+    for (double *p : b->pinned_free) (void)hipHostFree(p);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    for (auto &r : b->retired) {
+        (void)hipGraphExecDestroy(r.first);
+        (void)hipEventDestroy(r.second);
+    }
+    if (b->ddesc) (void)hipFree(b->ddesc);
+    for (int f = 0; f < 2; ++f) {
+        if (b->hdesc[f]) (void)hipHostFree(b->hdesc[f]);
+        if (b->hdesc_ev[f]) (void)hipEventDestroy(b->hdesc_ev[f]);
+    }
+    if (b->omega) (void)hipFree(b->omega);
+    if (b->partial) (void)hipFree(b->partial);
+    if (b->ticket) (void)hipFree(b->ticket);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return EDG_OK;
+}
+
+const char *edg_batch_last_error(const edg_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int edg_batch_set_graphs(edg_batch *b, int use_graphs)
+{
+    if (!b) return EDG_ERR_ARG;
+    b->use_graphs = use_graphs != 0;
+    return EDG_OK;
+}
+
+int edg_batch_problem_bytes(const edg_batch *b, const edg_problem *p, unsigned long long *bytes)
+{
+    if (!b || !p || !bytes) return EDG_ERR_ARG;
+    if (int rc = validate(b, *p)) return rc;
+    *bytes = problem_bytes(b, *p);
+    return EDG_OK;
+}
+
+int edg_batch_submit(edg_batch *b, const edg_problem *p, long long *id)
+{
+    if (!b || !p || !id) return EDG_ERR_ARG;
+    if (int rc = validate(b, *p)) return rc;
+    Job *J = new Job;
+    J->p = *p;
+    J->ek.assign(p->ek, p->ek + 2 * (size_t)p->nbath);
+    J->vmk.assign(p->vmk, p->vmk + 2 * (size_t)p->nbath);
+    J->p.ek = nullptr;
+    J->p.vmk = nullptr;
+    J->bytes = problem_bytes(b, *p);
+    if (J->bytes > b->budget) {
+        delete J;
+        return fail(b, EDG_ERR_ADMISSION, "problem needs " + std::to_string(problem_bytes(b, *p)) +
+                                              " device bytes, budget is " + std::to_string(b->budget));
+    }
+    J->id = b->next_id++;
+    *id = J->id;
+    b->queue.push_back(J);
+    return EDG_OK;
+}
+
+int edg_batch_step(edg_batch *b, int max_steps, int *pending)
+{
+    if (!b) return EDG_ERR_ARG;
+    (void)hipSetDevice(b->device);
+    if (int rc = progress(b)) return rc;
+    while (!b->queue.empty() && b->in_use + b->queue.front()->bytes <= b->budget) {
+        Job *J = b->queue.front();
+        const int rc = admit(b, J);
+        if (rc == 1) {  // device full: wait for running problems to release memory
+            if (b->running.empty())
+                return fail(b, EDG_ERR_NOMEM, "device memory exhausted with no problem in flight");
+            break;
+        }
+        if (rc) return rc;
+        b->queue.pop_front();
+    }
+    if (!b->active.empty() && max_steps > 0) {
+        int k = max_steps;
+        for (const Chain *C : b->active) k = std::min(k, C->bi.maxn - C->launched);
+        if (int rc = run_steps(b, k)) return rc;
+        if (int rc = progress(b)) return rc;
+    } else if (!b->running.empty()) {
+        // nothing to step: wait for the oldest outstanding copy, then move on
+        for (Job *J : b->running)
+            if (J->st == ST_GS_READ || J->st == ST_CH_READ || J->st == ST_CF_READ) {
+                BCHK(b, hipEventSynchronize(J->ev));
+                break;
+            }
+        if (int rc = progress(b)) return rc;
+    }
+    if (pending) *pending = (int)(b->queue.size() + b->running.size());
+    return EDG_OK;
+}
+
+int edg_batch_run(edg_batch *b)
+{
+    if (!b) return EDG_ERR_ARG;
+    int pending = 1;
+    while (pending > 0)
+        if (int rc = edg_batch_step(b, 1 << 20, &pending)) return rc;
+    return EDG_OK;
+}
+
+int edg_batch_poll(edg_batch *b, long long *id, double *e0, double *G, int *nstep, double *coeffs)
+{
+    if (!b) return EDG_ERR_ARG;
+    if (b->finished.empty()) return 0;
+    Job *J = b->finished.front();
+    b->finished.pop_front();
+    if (id) *id = J->id;
+    if (e0) *e0 = J->e0;
+    if (nstep) *nstep = J->gs_nstep;
+    if (G) std::memcpy(G, J->G.data(), sizeof(double) * J->G.size());
+    if (coeffs) std::memcpy(coeffs, J->coeffs.data(), sizeof(double) * J->coeffs.size());
+    delete J;
+    return 1;
+}
+
+int edg_batch_stats(const edg_batch *b, double *stats)
+{
+    if (!b || !stats) return EDG_ERR_ARG;
+    stats[0] = b->st_steps;
+    stats[1] = b->st_chain_steps;
+    stats[2] = b->st_done;
+    stats[3] = (double)b->in_use;
+    stats[4] = (double)b->budget;
+    stats[5] = (double)b->running.size();
+    stats[6] = b->st_graph;
+    stats[7] = b->st_rebuild;
+    return EDG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,199 @@
+// Batched Lanczos steps over many independent chains (uneven sector sizes)
+// for the single-orbital impurity (norb == 1).  One launch of k_bhv runs the
+// H.v half of step j_i for every chain i, one launch of k_baxpy the axpy
+// half, one launch of k_badvance moves every chain to its next step; each
+// chain keeps its own step index, alpha/beta arrays and partial sums, so
+// chains at different steps (admitted at different times) share the
+// launches.  The recurrence is the one of lanczos_solver.f90:92-173 in the
+// unnormalized-vector form used by the single-chain path:
+//   H.v:   y = s_j H u_j - beta_j s_{j-1} u_{j-1},  partial <u_j, y>
+//   axpy:  alpha_j = s_j <u_j, y>,  y <- y - alpha_j s_j u_j,  partial |y|^2
+// with beta_{j+1} = |y| and s_{j+1} = 1/beta_{j+1} taken by the next H.v.
+// A chain's reductions depend only on its own tiles (fixed count and
+// order), so its result is the same whatever else is in the batch.
+#include "device_util.h"
+
+namespace edg {
+
+// Index of the chain owning workgroup b: largest i with off[i] <= b.
+__device__ __forceinline__ int owner(const int *off, int ninst, int b)
+{
+    int lo = 0, hi = ninst;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <typename T>
+__device__ __forceinline__ T *slot_of(const BInst &I, int k)
+{
+    return static_cast<T *>(I.base) + (long long)(k % I.nslot) * I.stride;
+}
+
+// H.v tile: columns [c0, c0 + ncol) of chain i, every row.  The tile's
+// columns are contiguous in memory and staged in LDS; up hops gather from
+// the staged column, down hops read the neighbour column at the same row
+// from global memory.  Hop records are the 32-bit compressed tables
+// (target rank | value index << 16, KC slots per row, self-padded).
+template <typename T, typename ACC>
+__global__ __launch_bounds__(kBlock) void k_bhv(const BInst *inst, const int *hv_off, int ninst)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *xs = reinterpret_cast<T *>(smem);
+    __shared__ uint32_t s_drec[kBatchMaxCols * 16];
+    __shared__ ACC s_ed[kBatchMaxCols];
+    __shared__ uint8_t s_md[kBatchMaxCols];
+    __shared__ ACC s_vu[2 * (kMaxHops + 1)], s_vd[2 * (kMaxHops + 1)];
+
+    const int i = owner(hv_off, ninst, blockIdx.x);
+    const BInst &I = inst[i];
+    if (I.flags[0]) return;
+    const int t = blockIdx.x - hv_off[i];
+    const HvOp &op = I.op;
+    const int nup = op.nup, ndn = op.ndn, cb = I.cb;
+    const int c0 = t * cb;
+    const int ncol = ndn - c0 < cb ? ndn - c0 : cb;
+    const int kcu = op.up.KC, kcd = op.dn.KC;
+    const int j = *I.step;
+    const T *x = slot_of<T>(I, j);
+    T *y = slot_of<T>(I, j + 1);
+    const T *pv = j > 0 ? slot_of<T>(I, j - 1) : nullptr;
+
+    for (int e = threadIdx.x; e <= kMaxHops; e += kBlock) {
+        const ACC vu = e < op.nvup ? (ACC)op.up.vsrc[e] : ACC(0);
+        const ACC vd = e < op.nvup ? (ACC)op.dn.vsrc[e] : ACC(0);
+        s_vu[e] = vu;
+        s_vu[kMaxHops + 1 + e] = -vu;
+        s_vd[e] = vd;
+        s_vd[kMaxHops + 1 + e] = -vd;
+    }
+    const int nel = ncol * nup;
+    const T *xt = x + (long long)c0 * nup;
+    for (int e = threadIdx.x; e < nel; e += kBlock) xs[e] = xt[e];
+    for (int e = threadIdx.x; e < ncol * kcd; e += kBlock) s_drec[e] = op.dn.hrec[(long long)c0 * kcd + e];
+    for (int c = threadIdx.x; c < ncol; c += kBlock) {
+        s_ed[c] = energy_of<ACC>(op.dn, c0 + c);
+        s_md[c] = op.dn.mask[c0 + c];
+    }
+    __syncthreads();
+
+    HvArgs la{};
+    la.step = j;
+    la.prev_part = I.axpart;
+    la.prev_n = I.axn;
+    la.scale = I.scale;
+    la.beta = I.beta;
+    la.stop = I.flags;
+    la.nstep = I.flags + 1;
+    double sx, bsp;
+    if (!lanczos_scales<kBlock>(la, sx, bsp, t == 0)) return;
+
+    const ACC U = (ACC)op.U;
+    const float inv = 1.0f / (float)nup;
+    double dot = 0.0;
+    for (int e = threadIdx.x; e < nel; e += kBlock) {
+        int c = (int)((float)e * inv);
+        if (c * nup > e) --c;
+        else if ((c + 1) * nup <= e) ++c;
+        const int iup = e - c * nup;
+        const ACC xv = (ACC)xs[e];
+        ACC acc = (energy_of<ACC>(op.up, iup) + s_ed[c] + (op.up.mask[iup] && s_md[c] ? U : ACC(0))) * xv;
+        const uint32_t *ur = op.up.hrec + (long long)iup * kcu;
+        const T *xc = xs + c * nup;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (k < kcu) {
+                const uint32_t w = ur[k];
+                acc += s_vu[w >> 16] * (ACC)xc[w & 0xFFFFu];
+            }
+        }
+        const uint32_t *dr = s_drec + c * kcd;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (k < kcd) {
+                const uint32_t w = dr[k];
+                acc += s_vd[w >> 16] * (ACC)x[(long long)(w & 0xFFFFu) * nup + iup];
+            }
+        }
+        const long long g = (long long)c0 * nup + e;
+        double r = sx * (double)acc;
+        if (pv) r -= bsp * (double)pv[g];
+        const T rt = (T)r;
+        y[g] = rt;
+        dot += (double)xv * (double)rt;
+    }
+    double mine[1] = {dot};
+    block_partial_at<1>(mine, I.hvpart, t);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_baxpy(const BInst *inst, const int *ax_off, int ninst)
+{
+    const int i = owner(ax_off, ninst, blockIdx.x);
+    const BInst &I = inst[i];
+    if (I.flags[0]) return;
+    const int t = blockIdx.x - ax_off[i];
+    const int j = *I.step;
+    const double sj = I.scale[j];
+    const double al = sj * sum_partials<kBlock>(I.hvpart, I.hvn);
+    if (t == 0 && threadIdx.x == 0) I.alpha[j] = al;
+    const double coef = al * sj;
+    T *r = slot_of<T>(I, j + 1);
+    const T *u = slot_of<T>(I, j);
+    const long long n = (long long)I.op.nup * I.op.ndn;
+    double v[1] = {0.0};
+    for (long long e = (long long)t * kBlock + threadIdx.x; e < n; e += (long long)I.axn * kBlock) {
+        const T w = (T)((double)r[e] - coef * (double)u[e]);
+        r[e] = w;
+        v[0] += (double)w * (double)w;
+    }
+    block_partial_at<1>(v, I.axpart, t);
+}
+
+__global__ void k_badvance(const BInst *inst, int ninst)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ninst) *inst[i].step += 1;
+}
+
+int batch_set_lds(size_t max_lds)
+{
+    if (hipFuncSetAttribute((const void *)k_bhv<float, float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)max_lds) != hipSuccess)
+        return -1;
+    if (hipFuncSetAttribute((const void *)k_bhv<double, double>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds) != hipSuccess)
+        return -1;
+    return 0;
+}
+
+int launch_batch_hv(hipStream_t s, bool fp64, const BInst *inst, const int *hv_off, int ninst,
+                    int ntiles, int lds_elems)
+{
+    if (ntiles <= 0) return 0;
+    const size_t lds = (size_t)lds_elems * (fp64 ? 8 : 4);
+    void (*f)(const BInst *, const int *, int) = fp64 ? k_bhv<double, double> : k_bhv<float, float>;
+    hipLaunchKernelGGL(f, dim3(ntiles), dim3(kBlock), lds, s, inst, hv_off, ninst);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_batch_axpy(hipStream_t s, bool fp64, const BInst *inst, const int *ax_off, int ninst,
+                      int nblocks)
+{
+    if (nblocks <= 0) return 0;
+    if (fp64) hipLaunchKernelGGL(k_baxpy<double>, dim3(nblocks), dim3(kBlock), 0, s, inst, ax_off, ninst);
+    else hipLaunchKernelGGL(k_baxpy<float>, dim3(nblocks), dim3(kBlock), 0, s, inst, ax_off, ninst);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_batch_advance(hipStream_t s, const BInst *inst, int ninst)
+{
+    if (ninst <= 0) return 0;
+    hipLaunchKernelGGL(k_badvance, dim3((ninst + 255) / 256), dim3(256), 0, s, inst, ninst);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace edg

@@ -1,0 +1,121 @@
+// Device helpers shared by the HIP translation units of the ED solver.
+#pragma once
+#include "edgpu_internal.h"
+
+namespace edg {
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum NS per-thread values over the block; result valid in thread 0.
+template <int NS, int BS = kBlock>
+__device__ __forceinline__ void block_sums(double (&v)[NS], double (&out)[NS])
+{
+    __shared__ double red[NS][BS / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        double t = wave_sum(v[s]);
+        if (lane == 0) red[s][w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < BS / 64; ++k) t += red[s][k];
+            out[s] = t;
+        }
+    }
+    __syncthreads();
+}
+
+// Write this block's NS partial sums to partial[blockIdx.x*NS + s].  A
+// separate one-workgroup finisher kernel combines them in block order: the
+// kernel boundary makes the partials visible across XCDs without the
+// device-scope release fences a last-block-done scheme would need in every
+// workgroup (each of which writes back the XCD's L2).
+template <int NS, int BS = kBlock>
+__device__ __forceinline__ void block_partial_at(double (&mine)[NS], double *partial, int slot)
+{
+    double blk[NS];
+    block_sums<NS, BS>(mine, blk);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) partial[(long long)slot * NS + s] = blk[s];
+}
+template <int NS, int BS = kBlock>
+__device__ __forceinline__ void block_partial(double (&mine)[NS], double *partial)
+{
+    block_partial_at<NS, BS>(mine, partial, blockIdx.x);
+}
+
+// Sum of n partials in a fixed order: thread t adds partials t, t+BS, ...,
+// then a fixed block tree; every workgroup of size BS gets the same value.
+template <int BS>
+__device__ __forceinline__ double sum_partials(const double *part, int n)
+{
+    __shared__ double s_sum;
+    double v[1] = {0.0}, out[1];
+    for (int i = threadIdx.x; i < n; i += BS) v[0] += part[i];
+    block_sums<1, BS>(v, out);
+    if (threadIdx.x == 0) s_sum = out[0];
+    __syncthreads();
+    return s_sum;
+}
+
+// Lanczos scales of step j for the H.v kernel: returns false when the chain
+// has broken down (beta_j < 1e-16); the lead workgroup records beta_j and s_j.
+template <int BS>
+__device__ __forceinline__ bool lanczos_scales(const HvArgs &a, double &sx, double &bsp,
+                                               bool lead_block)
+{
+    if (a.step == 0) {
+        sx = a.scale[0];
+        bsp = 0.0;
+        return true;
+    }
+    const double bj = sqrt(sum_partials<BS>(a.prev_part, a.prev_n));
+    const bool first = lead_block && threadIdx.x == 0;
+    if (bj < 1.0e-16) {
+        if (first) {
+            a.beta[a.step] = bj;
+            *a.stop = 1;
+            *a.nstep = a.step;
+        }
+        return false;
+    }
+    sx = 1.0 / bj;
+    bsp = bj * a.scale[a.step - 1];
+    if (first) {
+        a.beta[a.step] = bj;
+        a.scale[a.step] = sx;
+    }
+    return true;
+}
+
+
+template <typename T, int C>
+struct alignas(sizeof(T) * C) Row {
+    T v[C];
+};
+
+__host__ __device__ constexpr size_t align16(size_t n) { return (n + 15) & ~size_t(15); }
+
+
+template <typename ACC>
+__device__ __forceinline__ ACC energy_of(const SpinTab &t, int r);
+template <>
+__device__ __forceinline__ float energy_of<float>(const SpinTab &t, int r) { return t.energyf[r]; }
+template <>
+__device__ __forceinline__ double energy_of<double>(const SpinTab &t, int r) { return t.energy[r]; }
+
+
+}  // namespace edg

@@ -84,6 +84,29 @@ struct HvArgs {
     int *nstep;
 };
 
+// One Lanczos chain of the batched engine (batch.hip / batch.cpp): its
+// sector operator, its vectors and its device-resident step state.  Vector
+// slot k is base + (k % nslot) * stride (nslot = maxn + 1 for a stored
+// Krylov basis, 3 for a ring).  The step index lives on the device and is
+// advanced by a kernel, so a batched step is the same three launches every
+// time (capturable in a HIP graph).
+struct BInst {
+    HvOp op;
+    void *base;
+    long long stride;
+    int nslot;
+    int maxn;
+    int cb;            // sector columns per H.v tile
+    int hvn, axn;      // H.v tiles / axpy workgroups of this chain
+    int *step;         // current step j
+    int *flags;        // [0] stop, [1] nstep
+    double *alpha;     // [maxn]
+    double *beta;      // [maxn + 1]
+    double *scale;     // [maxn + 1]
+    double *hvpart;    // [hvn]
+    double *axpart;    // [axn]
+};
+
 // ---- launchers (kernels.hip) ----
 int launch_spin_table(hipStream_t s, int nsite, int norb, int nbath, int ne, const double *em_s,
                       const double *ek_s, const double *v_s, double mu, double UpmJ,
@@ -127,6 +150,23 @@ int launch_observables(hipStream_t s, const T *gs, const HvOp &op, const double 
 int launch_cf(hipStream_t s, int nw, const double *omega, double e0, int np, const double *ap,
               const double *bp, int nm, const double *am, const double *bm, double *g,
               int accumulate);
+
+// ---- batched Lanczos steps (batch.hip) ----
+// hv_off / ax_off: prefix sums (ninst + 1 entries) of the chains' H.v tiles
+// and axpy workgroups; the launches cover hv_off[ninst] / ax_off[ninst]
+// workgroups.  lds_elems: largest cb * nup over the chains.
+int launch_batch_hv(hipStream_t s, bool fp64, const BInst *inst, const int *hv_off, int ninst,
+                    int ntiles, int lds_elems);
+int launch_batch_axpy(hipStream_t s, bool fp64, const BInst *inst, const int *ax_off, int ninst,
+                      int nblocks);
+int launch_batch_advance(hipStream_t s, const BInst *inst, int ninst);
+// Allow max_lds bytes of dynamic LDS for the batched H.v (call once, outside
+// any graph capture).
+int batch_set_lds(size_t max_lds);
+// Elements per batched H.v tile: cb = clamp(kBatchTile / nup, 1, kBatchMaxCols)
+// columns; a tile stages cb * nup elements in LDS.
+constexpr int kBatchTile = 2048;
+constexpr int kBatchMaxCols = 64;
 
 // Number of per-block partial slots the reductions may need.
 long long max_blocks_needed(long long n);

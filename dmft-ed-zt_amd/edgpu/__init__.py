@@ -39,11 +39,22 @@ class _Params(C.Structure):
                 ("precision", C.c_int), ("both_spins", C.c_int), ("seed", C.c_ulonglong)]
 
 
+class _Problem(C.Structure):
+    _fields_ = [("nbath", C.c_int), ("U", C.c_double), ("mu", C.c_double),
+                ("em", C.c_double * 2), ("ek", C.POINTER(C.c_double)),
+                ("vmk", C.POINTER(C.c_double)), ("nup", C.c_int), ("ndn", C.c_int),
+                ("maxnstep", C.c_int), ("maxnstep_green", C.c_int), ("both_spins", C.c_int),
+                ("seed", C.c_ulonglong)]
+
+
 EXPORTS = ["edg_default_params", "edg_create", "edg_destroy", "edg_last_error",
            "edg_set_memory_budget", "edg_device_count", "edg_set_tuning", "edg_spin_table",
            "edg_hx", "edg_lanczos_iteration", "edg_ground_state", "edg_gs_info",
            "edg_get_gs_vector", "edg_set_gs", "edg_cluster_green", "edg_observables",
-           "edg_apply_c", "edg_time_hv", "edg_last_solve_stats"]
+           "edg_apply_c", "edg_time_hv", "edg_last_solve_stats",
+           "edg_batch_create", "edg_batch_destroy", "edg_batch_last_error",
+           "edg_batch_set_graphs", "edg_batch_problem_bytes", "edg_batch_submit",
+           "edg_batch_step", "edg_batch_run", "edg_batch_poll", "edg_batch_stats"]
 
 _lib = None
 
@@ -81,8 +92,21 @@ def load_library(path=LIB_PATH):
     L.edg_apply_c.argtypes = [ctx, C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_int, dp]
     L.edg_time_hv.argtypes = [ctx, dp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
     L.edg_last_solve_stats.argtypes = [ctx, dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    bh = C.c_void_p
+    L.edg_batch_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_ulonglong,
+                                   C.POINTER(C.c_void_p)]
+    L.edg_batch_destroy.argtypes = [bh]
+    L.edg_batch_last_error.argtypes = [bh]
+    L.edg_batch_last_error.restype = C.c_char_p
+    L.edg_batch_set_graphs.argtypes = [bh, C.c_int]
+    L.edg_batch_problem_bytes.argtypes = [bh, C.POINTER(_Problem), C.POINTER(C.c_ulonglong)]
+    L.edg_batch_submit.argtypes = [bh, C.POINTER(_Problem), C.POINTER(C.c_longlong)]
+    L.edg_batch_step.argtypes = [bh, C.c_int, ip]
+    L.edg_batch_run.argtypes = [bh]
+    L.edg_batch_poll.argtypes = [bh, C.POINTER(C.c_longlong), dp, dp, ip, dp]
+    L.edg_batch_stats.argtypes = [bh, dp]
     for name in EXPORTS:
-        if name not in ("edg_default_params", "edg_last_error"):
+        if name not in ("edg_default_params", "edg_last_error", "edg_batch_last_error"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -298,3 +322,111 @@ class Solver:
         nh, ns = C.c_longlong(0), C.c_longlong(0)
         self._chk(self._L.edg_last_solve_stats(self._h, ph, C.byref(nh), C.byref(ns)))
         return dict(gs_ms=ph[0], green_ms=ph[1], n_hv=nh.value, n_steps=ns.value)
+
+
+class Problem:
+    """One single-orbital impurity problem for Batch (edg_problem): bath of
+    nbath sites with ek(nbath,2), vmk(1,nbath,2), em(1,2), U, mu, sector."""
+
+    def __init__(self, nbath, U, mu, em, ek, vmk, nup, ndn, maxnstep=200, maxnstep_green=0,
+                 both_spins=False, seed=1):
+        self.nbath, self.U, self.mu = int(nbath), float(U), float(mu)
+        self.em = _fortran(em, (1, 2))
+        self.ek = _fortran(ek, (self.nbath, 2))
+        self.vmk = _fortran(vmk, (1, self.nbath, 2))
+        self.nup, self.ndn = int(nup), int(ndn)
+        self.maxnstep, self.maxnstep_green = int(maxnstep), int(maxnstep_green)
+        self.both_spins, self.seed = bool(both_spins), int(seed)
+
+    def _struct(self):
+        p = _Problem()
+        p.nbath, p.U, p.mu = self.nbath, self.U, self.mu
+        p.em[0], p.em[1] = self.em[0], self.em[1]
+        p.ek, p.vmk = _d(self.ek), _d(self.vmk)
+        p.nup, p.ndn = self.nup, self.ndn
+        p.maxnstep, p.maxnstep_green = self.maxnstep, self.maxnstep_green
+        p.both_spins, p.seed = int(self.both_spins), self.seed
+        return p
+
+
+class Batch:
+    """Batched solver / request loop over many single-orbital problems
+    (edg_batch_*).  submit() queues a Problem and returns its id; step()
+    admits queued problems as device memory allows and advances every
+    Lanczos chain in flight by batched launches; poll() returns finished
+    results as dicts (id, e0, G (nw, nspin) complex, nstep)."""
+
+    def __init__(self, omega, precision="fp32", nspin=1, budget_bytes=0, device=0, graphs=True):
+        L = load_library()
+        self._om = np.ascontiguousarray(omega, dtype=np.float64)
+        self.nw, self.nspin = self._om.size, int(nspin)
+        h = C.c_void_p()
+        rc = L.edg_batch_create(int(device), {"fp64": EDG_FP64, "fp32": EDG_FP32}[precision],
+                                self.nspin, self.nw, _d(self._om), int(budget_bytes), C.byref(h))
+        if rc != 0:
+            raise EdgpuError(rc, "edg_batch_create failed (no GPU, or invalid arguments)")
+        self._L, self._h = L, h
+        self._ng = {}
+        self._chk(L.edg_batch_set_graphs(h, int(bool(graphs))))
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise EdgpuError(rc, self._L.edg_batch_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.edg_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def problem_bytes(self, prob):
+        n = C.c_ulonglong(0)
+        p = prob._struct()
+        self._chk(self._L.edg_batch_problem_bytes(self._h, C.byref(p), C.byref(n)))
+        return n.value
+
+    def step(self, max_steps=16):
+        n = C.c_int(0)
+        self._chk(self._L.edg_batch_step(self._h, int(max_steps), C.byref(n)))
+        return n.value
+
+    def run(self):
+        self._chk(self._L.edg_batch_run(self._h))
+
+    def submit(self, prob):
+        i = C.c_longlong(0)
+        p = prob._struct()
+        self._chk(self._L.edg_batch_submit(self._h, C.byref(p), C.byref(i)))
+        self._ng[i.value] = prob.maxnstep_green or prob.maxnstep
+        return i.value
+
+    def poll(self):
+        """Finished problems: id, e0, G (nw, nspin) complex, nstep and
+        coeffs[ispin] = (ap, bp, am, bm) (g_coeffs.dat content)."""
+        out = []
+        G = np.zeros(2 * self.nw * self.nspin)
+        co = np.zeros(8 * 1000)
+        while True:
+            i, e0, ns = C.c_longlong(0), C.c_double(0), C.c_int(0)
+            r = self._L.edg_batch_poll(self._h, C.byref(i), C.byref(e0), _d(G), C.byref(ns), _d(co))
+            if r < 0 or r > 1:
+                self._chk(r)
+            if r == 0:
+                return out
+            ng = self._ng.pop(i.value)
+            Gc = (G[0::2] + 1j * G[1::2]).reshape(self.nw, self.nspin, order="F").copy()
+            c = co[:8 * ng].reshape(ng, 4, 2, order="F")
+            coeffs = {s + 1: tuple(c[:, q, s].copy() for q in range(4)) for s in range(2)}
+            out.append(dict(id=i.value, e0=e0.value, G=Gc, nstep=ns.value, coeffs=coeffs))
+
+    def stats(self):
+        st = (C.c_double * 8)()
+        self._chk(self._L.edg_batch_stats(self._h, st))
+        keys = ("steps", "chain_steps", "finished", "bytes_in_use", "budget", "in_flight",
+                "graph_launches", "rebuilds")
+        return dict(zip(keys, list(st)))

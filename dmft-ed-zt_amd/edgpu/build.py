@@ -13,7 +13,7 @@ PKG = os.path.dirname(HERE)
 REPO = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(HERE, "libedgpu.so")
-SOURCES = ["kernels.hip", "capi.cpp", "tridiag.cpp"]
+SOURCES = ["kernels.hip", "batch.hip", "capi.cpp", "batch.cpp", "tridiag.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -54,7 +54,7 @@ def build(verbose=False):
     """Compile every HIP/C++ source into one shared library (always rebuilds)."""
     objs, procs = [], []
     for src in SOURCES:  # translation units compile in parallel
-        obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
+        obj = os.path.join(CSRC, src + ".o")
         cmd = [HIPCC, *FLAGS, "-c", "-x", "hip", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)

@@ -46,3 +46,55 @@ def two_orbital_kanamori(nbath_per_orb, U, seed, eps=(-1.0, 1.0), v=(0.2, 0.5)):
 def matsubara(nw, beta):
     """omega_n = (2n+1) pi / beta, n = 0..nw-1 (dmft_grid.f90:42-53)."""
     return (2.0 * np.arange(nw) + 1.0) * np.pi / beta
+
+
+def batched_problems(edgpu, n=512, seed=7, ns_choices=(8, 10, 12), U=(0.0, 6.0), filling=(0.8, 1.2),
+                     eps=(-1.5, 1.5), v=(0.1, 0.7), maxnstep=200, maxnstep_green=150):
+    """Independent single-orbital problems of the batched workload: Ns
+    uniform over ns_choices, U ~ U[U], filling n ~ U[filling] with
+    N = round(n Ns), N_up = ceil(N/2), N_dn = floor(N/2) (the reference takes
+    sectors from its input), bath eps ~ U[eps], V ~ U[v], spin-symmetric."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(n):
+        ns = int(rng.choice(ns_choices))
+        u = float(rng.uniform(*U))
+        fill = float(rng.uniform(*filling))
+        nb = ns - 1
+        e = rng.uniform(eps[0], eps[1], nb)
+        vv = rng.uniform(v[0], v[1], nb)
+        ntot = int(round(fill * ns))
+        nup, ndn = (ntot + 1) // 2, ntot // 2
+        em = np.zeros((1, 2))
+        ek = np.stack([e, e], axis=1)
+        vmk = np.stack([vv, vv], axis=1)[None, :, :]
+        out.append(edgpu.Problem(nb, u, u / 2.0, em, ek, vmk, nup, ndn, maxnstep=maxnstep,
+                                 maxnstep_green=maxnstep_green, seed=seed * 100003 + i))
+    return out
+
+
+def served_requests(edgpu, rate=200.0, duration=60.0, seed=11, ns_weights=((10, 0.5), (12, 0.35), (14, 0.15)),
+                    U=(0.0, 8.0), maxnstep=200, maxnstep_green=150):
+    """Poisson arrivals at `rate` per second over `duration` seconds of
+    half-filled single-orbital problems: Ns drawn with ns_weights, U ~ U[U],
+    seeded baths eps ~ U[-1,1], V ~ U[0.2,0.6].  Returns [(t_arrival, Problem)]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ns_vals = np.array([w[0] for w in ns_weights])
+    p = np.array([w[1] for w in ns_weights], dtype=float)
+    p /= p.sum()
+    t, out, i = 0.0, [], 0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            return out
+        ns = int(rng.choice(ns_vals, p=p))
+        u = float(rng.uniform(*U))
+        nb = ns - 1
+        e = rng.uniform(-1.0, 1.0, nb)
+        vv = rng.uniform(0.2, 0.6, nb)
+        em = np.zeros((1, 2))
+        ek = np.stack([e, e], axis=1)
+        vmk = np.stack([vv, vv], axis=1)[None, :, :]
+        out.append((t, edgpu.Problem(nb, u, u / 2.0, em, ek, vmk, ns // 2, ns // 2, maxnstep=maxnstep,
+                                     maxnstep_green=maxnstep_green, seed=seed * 1000003 + i)))
+        i += 1

@@ -162,6 +162,71 @@ int edg_time_hv(edg_ctx *ctx, const double *em, const double *ek, const double *
  * phase_ms[1] chains) and the number of H.v applications performed. */
 int edg_last_solve_stats(edg_ctx *ctx, double *phase_ms, long long *n_hv, long long *n_steps);
 
+/* ---------------------------------------------------------------------
+ * Batched solver and request loop.  No counterpart in the reference (it
+ * solves one impurity problem per call); for callers with many independent
+ * single-orbital problems: the candidate sectors and parameter points of a
+ * DMFT step or scan, or a stream of requests.  Every problem runs the same
+ * algorithm as edg_ground_state + edg_cluster_green (stored-basis Lanczos
+ * ground state, FP64 Rayleigh-quotient E0, c+/c chains, continued
+ * fraction); the Lanczos steps of all problems in flight share batched
+ * launches (one H.v, one axpy and one step-advance kernel per step), and
+ * problems are admitted by device bytes as earlier ones finish.  A
+ * problem's result does not depend on what else is in the batch.
+ * --------------------------------------------------------------------- */
+typedef struct edg_batch edg_batch;
+
+/* One single-orbital impurity problem (norb = 1): bath of nbath sites with
+ * ek(nbath,2) and vmk(1,nbath,2) column-major as for edg_ground_state,
+ * em(1,2), U and mu, in the sector (nup, ndn). */
+typedef struct {
+    int nbath;
+    double U;
+    double mu;
+    double em[2];
+    const double *ek;
+    const double *vmk;
+    int nup;
+    int ndn;
+    int maxnstep;        /* ground-state Lanczos steps */
+    int maxnstep_green;  /* steps of each c+/c chain (0: maxnstep) */
+    int both_spins;      /* also run the spin-down chains when spin symmetry would skip them */
+    unsigned long long seed;
+} edg_problem;
+
+/* precision: EDG_FP64 / EDG_FP32 vector storage; nspin 1 or 2 (G_cl spins);
+ * omega: the nw Matsubara frequencies; budget_bytes: device bytes the
+ * problems in flight may hold (0 = 85% of the free device memory); a problem
+ * is also held back while the device cannot allocate it. */
+int edg_batch_create(int device, int precision, int nspin, int nw, const double *omega,
+                     unsigned long long budget_bytes, edg_batch **out);
+int edg_batch_destroy(edg_batch *b);
+const char *edg_batch_last_error(const edg_batch *b);
+/* Replay the batched step as a captured HIP graph (1, default) or launch
+ * its kernels one by one (0). */
+int edg_batch_set_graphs(edg_batch *b, int use_graphs);
+/* Device bytes one problem holds while in flight (its admission size). */
+int edg_batch_problem_bytes(const edg_batch *b, const edg_problem *p, unsigned long long *bytes);
+/* Queue a problem; *id receives its handle.  EDG_ERR_ADMISSION when it can
+ * never fit the budget, EDG_ERR_ARG for an invalid problem. */
+int edg_batch_submit(edg_batch *b, const edg_problem *p, long long *id);
+/* Admit queued problems while the budget allows, run the per-problem stages
+ * that are ready and up to max_steps batched Lanczos steps.  *pending
+ * receives the number of submitted problems not yet finished. */
+int edg_batch_step(edg_batch *b, int max_steps, int *pending);
+/* Run until every submitted problem is finished. */
+int edg_batch_run(edg_batch *b);
+/* Pop one finished problem: returns 1 and fills id, e0 (FP64 Rayleigh
+ * quotient), G (G_cl(nw,1,nspin) as interleaved re/im), nstep (ground-state
+ * Lanczos steps) and coeffs (ng x 4 x 2, ng = maxnstep_green or maxnstep:
+ * (ap, bp, am, bm) per spin as written to g_coeffs.dat, dmft_green.f90:89-109),
+ * or returns 0 when none is ready.  Pointers may be NULL. */
+int edg_batch_poll(edg_batch *b, long long *id, double *e0, double *G, int *nstep, double *coeffs);
+/* stats[0] batched steps, [1] chain-steps, [2] problems finished,
+ * [3] device bytes in use, [4] budget, [5] problems in flight, [6] graph
+ * launches, [7] descriptor rebuilds. */
+int edg_batch_stats(const edg_batch *b, double *stats);
+
 #ifdef __cplusplus
 }
 #endif

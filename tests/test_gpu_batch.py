@@ -1,0 +1,207 @@
+"""Batched solver (edg_batch_*): parity of every problem with the FP64 oracle
+and with the single-problem path, independence of a problem's result from
+the rest of the batch, uneven sizes, chain breakdown and admission.
+
+Tolerances (written per test):
+  E0 FP64 / FP32 vs oracle        1e-9 / 1e-6 (absolute)
+  G FP64 vs oracle                1e-8 (max abs over the grid); 1e-6 for
+                                  chains longer than their sector
+  chain coefficients FP64         1e-9 / 1e-12 on the leading steps
+  G FP32 vs oracle, Ns <= 8       1e-4: at beta = 100 the lowest Matsubara
+                                  points magnify the FP32 pole-position error
+                                  of small discrete baths (same bound as the
+                                  single-problem FP32 golden test)
+  alone vs in a mix               bit-identical (a chain's reductions depend
+                                  only on its own tiles)
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def problem(edgpu, nbath, U, seed, nup=None, ndn=None, maxnstep=80, maxnstep_green=0,
+            both_spins=False, eps=(-1.0, 1.0), v=(0.2, 0.6)):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    e = rng.uniform(eps[0], eps[1], nbath)
+    vv = rng.uniform(v[0], v[1], nbath)
+    em = np.zeros((1, 2))
+    ek = np.stack([e, e], axis=1)
+    vmk = np.stack([vv, vv], axis=1)[None, :, :]
+    ns = nbath + 1
+    nup = ns // 2 if nup is None else nup
+    ndn = ns // 2 if ndn is None else ndn
+    return edgpu.Problem(nbath, U, U / 2.0, em, ek, vmk, nup, ndn, maxnstep=maxnstep,
+                         maxnstep_green=maxnstep_green, both_spins=both_spins, seed=seed)
+
+
+def oracle_solve(oracle, pr, omega, nspin=1):
+    m = oracle.Model(1, pr.nbath, U=pr.U, mu=pr.mu)
+    em = pr.em.reshape(1, 2, order="F")
+    ek = pr.ek.reshape(pr.nbath, 2, order="F")
+    vmk = pr.vmk.reshape(1, pr.nbath, 2, order="F")
+    gs = oracle.ground_state(m, em, ek, vmk, [(pr.nup, pr.ndn)], pr.maxnstep, seed=pr.seed)
+    mg = pr.maxnstep_green or pr.maxnstep
+    G, _ = oracle.cluster_green_ftn(m, em, ek, vmk, gs, mg, omega, nspin=nspin,
+                                    both_spins=pr.both_spins)
+    return gs.val, G[:, 0, :]
+
+
+def run_batch(edgpu, probs, omega, precision="fp64", graphs=True, budget=0, nspin=1, step=16):
+    b = edgpu.Batch(omega, precision=precision, graphs=graphs, budget_bytes=budget, nspin=nspin)
+    ids = [b.submit(p) for p in probs]
+    peak = 0
+    pending = len(ids)
+    while pending:
+        pending = b.step(step)
+        peak = max(peak, b.stats()["bytes_in_use"])
+    res = {r["id"]: r for r in b.poll()}
+    st = b.stats()
+    b.close()
+    return [res[i] for i in ids], st, peak
+
+
+def oracle_coeffs(oracle, pr, omega):
+    m = oracle.Model(1, pr.nbath, U=pr.U, mu=pr.mu)
+    em = pr.em.reshape(1, 2, order="F")
+    ek = pr.ek.reshape(pr.nbath, 2, order="F")
+    vmk = pr.vmk.reshape(1, pr.nbath, 2, order="F")
+    gs = oracle.ground_state(m, em, ek, vmk, [(pr.nup, pr.ndn)], pr.maxnstep, seed=pr.seed)
+    mg = pr.maxnstep_green or pr.maxnstep
+    _, co = oracle.cluster_green_ftn(m, em, ek, vmk, gs, mg, omega, both_spins=pr.both_spins)
+    return co
+
+
+def test_batch_chain_coefficients_vs_oracle(edgpu, oracle):
+    """The g_coeffs.dat coefficients (ap, bp, am, bm) against the oracle's
+    green_diag, FP64 vectors.  Ns = 4 with 12 steps: every coefficient to
+    1e-9.  Ns = 6 with 30 steps: the first 6 to 1e-12.  Later coefficients
+    of a converging chain amplify the rounding difference of the two
+    reduction orders geometrically (measured: 1e-15 at step 4, 1e-11 at
+    step 8, 4e-9 at step 10), so they are checked through G(i w) instead."""
+    omega = oracle.matsubara(32, 100.0)
+    for pr, nexact, tol in ((problem(edgpu, 3, 2.0, 61, maxnstep=12), 12, 1e-9),
+                            (problem(edgpu, 5, 2.0, 3, maxnstep=30), 6, 1e-12)):
+        print(f"coefficients nbath={pr.nbath}", flush=True)
+        res, _, _ = run_batch(edgpu, [pr], omega)
+        co = oracle_coeffs(oracle, pr, omega)
+        for q, name in enumerate(("ap", "bp", "am", "bm")):
+            got, want = res[0]["coeffs"][1][q], co[(1, 1)][q]
+            d = np.abs(got - want)
+            assert d[:nexact].max() < tol, (name, d[:nexact])
+
+
+@pytest.mark.parametrize("precision,etol,gtol", [("fp64", 1e-9, 1e-8), ("fp32", 1e-6, 1e-4)])
+def test_batch_vs_oracle_mixed_sizes(edgpu, oracle, precision, etol, gtol):
+    """Ns = 4, 6, 8 and an off-half-filling sector in one batch; every
+    result against the oracle's ground_state + cluster_green_ftn."""
+    omega = oracle.matsubara(256, 100.0)
+    probs = [problem(edgpu, 5, 2.0, 3), problem(edgpu, 3, 1.0, 4), problem(edgpu, 7, 3.0, 5),
+             problem(edgpu, 5, 4.0, 6, nup=2, ndn=3), problem(edgpu, 7, 0.5, 7, nup=5, ndn=4)]
+    print(f"batch vs oracle {precision}", flush=True)
+    res, st, _ = run_batch(edgpu, probs, omega, precision=precision)
+    assert st["finished"] == len(probs)
+    for pr, r in zip(probs, res):
+        e0, G = oracle_solve(oracle, pr, omega)
+        assert abs(r["e0"] - e0) < etol, (pr.nbath, pr.nup, pr.ndn, r["e0"], e0)
+        assert np.abs(r["G"][:, 0] - G[:, 0]).max() < gtol
+
+
+def test_batch_matches_single_problem_path(edgpu, oracle):
+    """Same algorithm as edg_ground_state + edg_cluster_green: E0 and G agree
+    (FP64 vectors; the two paths tile their reductions differently)."""
+    omega = oracle.matsubara(256, 100.0)
+    pr = problem(edgpu, 7, 2.0, 11, maxnstep=100, maxnstep_green=60)
+    res, _, _ = run_batch(edgpu, [pr], omega)
+    s = edgpu.Solver(norb=1, nbath=7, U=2.0, mu=1.0, maxnstep=100, maxnstep_green=60, seed=11)
+    em = pr.em.reshape(1, 2, order="F")
+    ek = pr.ek.reshape(7, 2, order="F")
+    vmk = pr.vmk.reshape(1, 7, 2, order="F")
+    gs = s.ground_state(em, ek, vmk, [(pr.nup, pr.ndn)])
+    G, _ = s.cluster_green_ftn(em, ek, vmk, omega)
+    s.close()
+    assert abs(res[0]["e0"] - gs.val) < 1e-10
+    assert np.abs(res[0]["G"][:, 0] - G[:, 0, 0]).max() < 1e-9
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_result_independent_of_batch(edgpu, oracle, graphs):
+    """A problem solved alone and inside a mix of other sizes (admitted at
+    different times through a small memory budget) gives bit-identical E0
+    and G, with and without graph replay."""
+    omega = oracle.matsubara(128, 50.0)
+    target = problem(edgpu, 7, 2.5, 21, maxnstep=60, maxnstep_green=40)
+    alone, _, _ = run_batch(edgpu, [target], omega, precision="fp32", graphs=graphs)
+    others = [problem(edgpu, nb, 1.0 + 0.3 * k, 30 + k, maxnstep=40 + 10 * (k % 3),
+                      maxnstep_green=30) for k, nb in enumerate([5, 9, 3, 7, 9, 5])]
+    probe = edgpu.Batch(omega, precision="fp32")
+    need = max(probe.problem_bytes(p) for p in others + [target])
+    probe.close()
+    mix = others[:3] + [target] + others[3:]
+    res, st, peak = run_batch(edgpu, mix, omega, precision="fp32", graphs=graphs,
+                              budget=3 * need, step=7)
+    r = res[3]
+    assert r["e0"] == alone[0]["e0"]
+    assert np.array_equal(r["G"], alone[0]["G"])
+    assert peak <= 3 * need
+    if graphs:
+        assert st["graph_launches"] > 0
+
+
+def test_breakdown_and_missing_chains(edgpu, oracle, dense):
+    """A sector smaller than maxnstep (the chain runs past its dimension on
+    rounding noise, as the reference's does) and sectors whose c or c+ chain
+    does not exist (N = 0, N = nsite), against the oracle and, for the last,
+    the exact Lehmann sum of the dense Jordan-Wigner construction.  G to
+    1e-6: steps past the sector dimension continue from rounding noise once
+    beta falls near (not below) 1e-16, differently in each implementation;
+    their weight in G is about 1e-8.  (No chain sector here has dimension 1:
+    a chain of length 1 meets the reference's continued-fraction quirk,
+    which applies level 1 twice when n = 1, numeric_utils.f90:18-22, so a
+    breakdown after one step changes G by O(b_2^2) depending on rounding.)"""
+    omega = oracle.matsubara(64, 100.0)
+    probs = [problem(edgpu, 3, 2.0, 41, nup=2, ndn=2, maxnstep=60),   # D = 36 < 60
+             problem(edgpu, 3, 2.0, 42, nup=0, ndn=2, maxnstep=20),   # no c_up chain
+             problem(edgpu, 3, 2.0, 43, nup=4, ndn=2, maxnstep=20)]   # no c+_up chain
+    res, _, _ = run_batch(edgpu, probs, omega)
+    for pr, r in zip(probs, res):
+        e0, G = oracle_solve(oracle, pr, omega)
+        assert abs(r["e0"] - e0) < 1e-9
+        assert np.abs(r["G"][:, 0] - G[:, 0]).max() < 1e-6
+    pr, r = probs[2], res[2]
+    em, ek, vmk = pr.em.reshape(1, 2, order="F"), pr.ek.reshape(3, 2, order="F"), \
+        pr.vmk.reshape(1, 3, 2, order="F")
+    s = edgpu.Solver(norb=1, nbath=3, U=2.0, mu=1.0, maxnstep=20, seed=43)
+    gs = s.ground_state(em, ek, vmk, [(4, 2)])
+    Gs, _ = s.cluster_green_ftn(em, ek, vmk, omega)
+    s.close()
+    args = (1, 3, em, ek, vmk, 2.0, 0.0, 0.0, 0.0, 1.0, 4, 2, 1)
+    exact = 0.5 * (dense.lehmann_green(*args, 1, omega)[0] + dense.lehmann_green(*args, 2, omega)[0])
+    assert abs(gs.val - r["e0"]) < 1e-10
+    assert np.abs(r["G"][:, 0] - exact).max() < 1e-6
+    assert np.abs(Gs[:, 0, 0] - exact).max() < 1e-6
+
+
+def test_nspin2_and_both_spins(edgpu, oracle):
+    omega = oracle.matsubara(64, 100.0)
+    pr = problem(edgpu, 5, 2.0, 51, nup=3, ndn=2)
+    res, _, _ = run_batch(edgpu, [pr], omega, nspin=2)
+    e0, G = oracle_solve(oracle, pr, omega, nspin=2)
+    assert abs(res[0]["e0"] - e0) < 1e-9
+    assert np.abs(res[0]["G"] - G).max() < 1e-8
+    pr2 = problem(edgpu, 5, 2.0, 52, both_spins=True)
+    res2, _, _ = run_batch(edgpu, [pr2], omega)
+    e0, G = oracle_solve(oracle, pr2, omega)
+    assert np.abs(res2[0]["G"][:, 0] - G[:, 0]).max() < 1e-8
+
+
+def test_admission_errors(edgpu, oracle):
+    omega = oracle.matsubara(16, 10.0)
+    b = edgpu.Batch(omega, precision="fp32", budget_bytes=1 << 20)
+    with pytest.raises(edgpu.EdgpuError) as ei:
+        b.submit(problem(edgpu, 9, 2.0, 1, maxnstep=200))
+    assert ei.value.code == 2
+    with pytest.raises(edgpu.EdgpuError) as ei:
+        b.submit(problem(edgpu, 5, 2.0, 1, nup=7, ndn=3))
+    assert ei.value.code == 3
+    b.close()
