@@ -105,6 +105,68 @@ def profile_hv(edgpu, em, ek, vmk, U, mu):
                       "copy_bytes_written": nbytes}))
 
 
+def bench_batched(edgpu, W, S):
+    """512 independent single-orbital problems (Ns in {8,10,12}) through the
+    batched solver: whole-batch solves/s, after a warm-up batch of 32."""
+    om = W.matsubara(2048, 100.0)
+    probs = W.batched_problems(edgpu, n=512)
+    S.run_batched(edgpu, probs[:32], om)
+    res, wall, st = S.run_batched(edgpu, probs, om)
+    ns = [p.nbath + 1 for p in probs]
+    return dict(workload="512 single-orbital problems, Ns uniform in {8,10,12}, U~U[0,6], "
+                         "n~U[0.8,1.2] (N=round(n Ns), N_up=ceil(N/2)), eps~U[-1.5,1.5], "
+                         "V~U[0.1,0.7]; 200 GS steps + c+/c chains of 150 steps, FP32 vectors, "
+                         "2048 Matsubara points",
+                problems=len(probs), ns_counts={str(k): ns.count(k) for k in (8, 10, 12)},
+                solves_per_s=len(probs) / wall, wall_s=wall, batched_steps=st["steps"],
+                chain_steps_per_s=st["chain_steps"] / wall,
+                e0_finite=bool(np.all(np.isfinite([r["e0"] for r in res]))))
+
+
+def bench_served(edgpu, W, S, seconds):
+    """Poisson arrivals at 200 req/s for `seconds` (Ns 10/12/14 with weights
+    0.5/0.35/0.15, U~U[0,8]) replayed against the wall clock; latency =
+    result time - arrival time, solves/s over the whole run (arrivals + drain)."""
+    om = W.matsubara(2048, 100.0)
+    req = W.served_requests(edgpu, rate=200.0, duration=seconds)
+    r = S.serve(edgpu, req, om, drain_limit=300.0, log=log)
+    return dict(workload=f"Poisson 200 req/s for {seconds:g} s, Ns in {{10,12,14}} (0.5/0.35/0.15), "
+                         "U~U[0,8], half filling, 200 GS steps + 2 chains x 150 steps, FP32 vectors; "
+                         "admission by device bytes (stored Krylov basis + rings)",
+                submitted=r["submitted"], completed=r["completed"], wall_s=r["wall"],
+                solves_per_s=r["solves_per_s"], p50_s=r["p50"], p99_s=r["p99"],
+                offered_per_s=r["submitted"] / seconds if seconds else 0.0)
+
+
+def bench_kanamori(edgpu, W):
+    """Two-orbital Kanamori impurity, Ns=14 (6 bath sites per orbital),
+    sector (7,7), U=3, J=U/4, U'=U-2J; FP32-vector vs FP64 solve with the
+    2x2 G per spin from 8 chains."""
+    em, ek, vmk, mu, J, Up = W.two_orbital_kanamori(6, 3.0, seed=1)
+    om = W.matsubara(2048, 100.0)
+    out = {}
+    for prec in ("fp32", "fp64"):
+        s = edgpu.Solver(norb=2, nbath=12, U=3.0, J=J, Up=Up, Jp=J, mu=mu, maxnstep=200,
+                         maxnstep_green=150, precision=prec, both_spins=True, seed=1)
+        s.ground_state(em, ek, vmk, [(7, 7)])
+        t = time.perf_counter()
+        gs = s.ground_state(em, ek, vmk, [(7, 7)])
+        G, _ = s.cluster_green_ftn(em, ek, vmk, om)
+        out[prec] = (time.perf_counter() - t, gs.val, G, s.observables())
+        ms, by = s.time_hv(em, ek, vmk, 7, 7, warmup=3, reps=20)
+        out[prec + "_hv"] = (ms, by)
+        s.close()
+    t32, e32, g32, o32 = out["fp32"]
+    t64, e64, g64, o64 = out["fp64"]
+    return dict(workload="two-orbital Kanamori, Ns=14 (6 bath sites per orbital), sector (7,7), "
+                         "D=11778624, U=3, J=0.75, U'=1.5, Jp=J; 200 GS steps + 8 chains x 150",
+                fp32_solve_ms=t32 * 1e3, fp64_solve_ms=t64 * 1e3, fp32_vs_fp64_speedup=t64 / t32,
+                e0=e64, dE0=abs(e32 - e64), max_dG=float(np.abs(g32 - g64).max()),
+                max_d_occ=float(np.abs(o32[0] - o64[0]).max()),
+                hv_fp32_us=out["fp32_hv"][0] * 1e3,
+                hv_fp32_gbs=out["fp32_hv"][1] / (out["fp32_hv"][0] * 1e-3) / 1e9)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +176,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=3.0,
+                    help="seconds of Poisson arrivals for the served-stream figures (0: skip)")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-kanamori", action="store_true")
     ap.add_argument("--profile-hv", action="store_true",
                     help="only run H.v launches (FP32, Ns=14) and one device copy of the same "
                          "byte count, for hardware-counter profiles")
@@ -174,6 +240,18 @@ def main():
                    hv_fp64_gbs=hv64_bytes / (hv64_ms * 1e-3) / 1e9, hv_fp64_us=hv64_ms * 1e3)
         s64.close()
 
+    extra = {}
+    from edgpu import serve as S
+    if not args.no_batched:
+        log("batched problems")
+        extra["batched"] = bench_batched(edgpu, W, S)
+    if args.serve_seconds > 0:
+        log("served stream")
+        extra["served"] = bench_served(edgpu, W, S, args.serve_seconds)
+    if not args.no_kanamori:
+        log("two-orbital Kanamori")
+        extra["kanamori"] = bench_kanamori(edgpu, W)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle)")
@@ -213,6 +291,7 @@ def main():
         "cpu_baseline": cpu,
     }
     res.update(out)
+    res.update(extra)
     print(json.dumps(res))
 
 

@@ -147,9 +147,13 @@ struct edg_batch {
     hipEvent_t hdesc_ev[2] = {nullptr, nullptr};
     bool hdesc_used[2] = {false, false};
     int flip = 0;
+    // chains in launch classes {short, long columns} x {8, 16 hop slots}
     const BInst *d_inst = nullptr;
-    const int *d_hvoff = nullptr, *d_axoff = nullptr;
-    int ntiles = 0, nax = 0, lds_elems = 0;
+    const int *d_axoff = nullptr;
+    const int *d_hvoff[4] = {nullptr, nullptr, nullptr, nullptr};
+    int cls_start[4] = {0, 0, 0, 0}, cls_n[4] = {0, 0, 0, 0};
+    int cls_tiles[4] = {0, 0, 0, 0}, cls_lds[4] = {1, 1, 1, 1};
+    int nax = 0;
     bool use_graphs = true;
     hipGraphExec_t gexec = nullptr;
     bool graph_valid = false;
@@ -204,6 +208,15 @@ int chain_cb(int nup)
     return std::min(std::max(cb, 1), kBatchMaxCols);
 }
 
+// H.v tiles of a chain: long columns (nup >= kBatchWide) take one column per
+// 512-thread tile, short ones chain_cb columns per 256-thread tile.
+int chain_hvn(long long nup, long long ndn)
+{
+    if (nup >= kBatchWide) return (int)ndn;
+    const int cb = chain_cb((int)nup);
+    return (int)((ndn + cb - 1) / cb);
+}
+
 int chain_axn(long long D)
 {
     long long g = (D + kBlock * 8 - 1) / (kBlock * 8);
@@ -223,7 +236,7 @@ size_t problem_bytes(const edg_batch *b, const edg_problem &p)
     const long long nu = binom_ll(ns, p.nup), nd = binom_ll(ns, p.ndn);
     const long long D = nu * nd, Dp = rup(D, kPadB);
     bytes += ((size_t)p.maxnstep + 1) * Dp * b->es + Dp * b->es;
-    bytes += state_bytes(p.maxnstep, (int)((nd + chain_cb((int)nu) - 1) / chain_cb((int)nu)), chain_axn(D));
+    bytes += state_bytes(p.maxnstep, chain_hvn(nu, nd), chain_axn(D));
     const int mg = green_len(p);
     bytes += 64 * 8 + 8 * (8 * (size_t)std::max(mg, p.maxnstep) + 64) + 8 * 4 * (size_t)b->nw;
     for (int k = 0; k < 4; ++k) {
@@ -233,7 +246,7 @@ size_t problem_bytes(const edg_batch *b, const edg_problem &p)
         if (cu < 0 || cd < 0 || cu > ns || cd > ns) continue;
         const long long cnu = binom_ll(ns, cu), cnd = binom_ll(ns, cd), Dc = cnu * cnd;
         bytes += 3 * rup(Dc, kPadB) * b->es + 4 * (size_t)(ispin == 1 ? cnu : cnd) + 256;
-        bytes += state_bytes(mg, (int)((cnd + chain_cb((int)cnu) - 1) / chain_cb((int)cnu)), chain_axn(Dc));
+        bytes += state_bytes(mg, chain_hvn(cnu, cnd), chain_axn(Dc));
     }
     return bytes + 64 * 256;
 }
@@ -277,7 +290,9 @@ double *pinned_get(edg_batch *b)
     return p;
 }
 
-int build_table(edg_batch *b, Job *J, int spin, int ne, SpinTab &t)
+// Per-spin tables of one (spin, N) half; kc = hop slots per row, the same
+// for every table of the problem so that its chains share one launch class.
+int build_table(edg_batch *b, Job *J, int spin, int ne, SpinTab &t, int kc)
 {
     const int ns = nsite_of(J->p), nbath = J->p.nbath;
     t = SpinTab{};
@@ -293,7 +308,7 @@ int build_table(edg_batch *b, Job *J, int spin, int ne, SpinTab &t)
     t.energyf = (float *)J->ar.take(4 * n);
     t.mask = (uint8_t *)J->ar.take(n);
     t.kmax = kmax_single(nbath, ne);
-    t.KC = t.kmax <= 8 ? 8 : 16;
+    t.KC = kc;
     t.ccol = (uint16_t *)J->ar.take(16 * n * 2);
     t.cidx = (uint8_t *)J->ar.take(16 * n);
     t.hrec = (uint32_t *)J->ar.take(16 * n * 4);
@@ -339,8 +354,8 @@ int init_chain(edg_batch *b, Job *J, Chain &C, const HvOp &op, int maxn, bool st
     I.stride = Dp;
     I.base = J->ar.take((size_t)I.nslot * Dp * b->es);
     I.maxn = maxn;
-    I.cb = chain_cb(op.nup);
-    I.hvn = (op.ndn + I.cb - 1) / I.cb;
+    I.cb = op.nup >= kBatchWide ? 1 : chain_cb(op.nup);
+    I.hvn = chain_hvn(op.nup, op.ndn);
     I.axn = chain_axn(D);
     if ((size_t)I.cb * op.nup * b->es > kLdsMax) return fail(b, EDG_ERR_ARG, "tile exceeds LDS");
     char *st = (char *)J->ar.take(state_bytes(maxn, I.hvn, I.axn));
@@ -408,13 +423,19 @@ int admit(edg_batch *b, Job *J)
     std::memcpy(h + 2, J->ek.data(), sizeof(double) * 2 * nbath);
     std::memcpy(h + 2 + 2 * nbath, J->vmk.data(), sizeof(double) * 2 * nbath);
     BCHK(b, hipMemcpyAsync(J->dparams, h, 8 * (2 + 4 * (size_t)nbath), hipMemcpyHostToDevice, b->stream));
+    int kc = 8;
+    for (int s = 0; s < 2; ++s) {
+        const int n0 = s == 0 ? p.nup : p.ndn;
+        for (int d = -1; d <= 1; ++d)
+            if (n0 + d >= 0 && n0 + d <= ns && kmax_single(nbath, n0 + d) > 8) kc = 16;
+    }
     for (int s = 0; s < 2; ++s) {
         const int n0 = s == 0 ? p.nup : p.ndn;
         for (int d = -1; d <= 1; ++d) {
             const int ne = n0 + d;
             J->has_tab[s][d + 1] = ne >= 0 && ne <= ns;
             if (J->has_tab[s][d + 1])
-                if (int rc = build_table(b, J, s + 1, ne, J->tab[s][d + 1])) return rc;
+                if (int rc = build_table(b, J, s + 1, ne, J->tab[s][d + 1], kc)) return rc;
         }
     }
     J->misc = (double *)J->ar.take(64 * 8);
@@ -656,23 +677,43 @@ int progress(edg_batch *b)
     return 0;
 }
 
-// Upload the descriptor table of the active chains.
+int chain_class(const BInst &I) { return (I.op.nup >= kBatchWide ? 2 : 0) + (I.op.up.KC == 16 ? 1 : 0); }
+
+// Upload the descriptor table of the active chains grouped by launch class
+// (short / long columns x 8 / 16 hop slots), with the tile prefix sums of
+// each class and the axpy prefix sums over all chains.
 int rebuild(edg_batch *b)
 {
-    const int n = (int)b->active.size();
-    std::vector<int> hv(n + 1), ax(n + 1);
-    hv[0] = ax[0] = 0;
-    int lds = 1;
-    for (int i = 0; i < n; ++i) {
-        const BInst &I = b->active[i]->bi;
-        hv[i + 1] = hv[i] + I.hvn;
-        ax[i + 1] = ax[i] + I.axn;
-        lds = std::max(lds, I.cb * I.op.nup);
+    std::vector<Chain *> order;
+    int start[5] = {0, 0, 0, 0, 0};
+    for (int c = 0; c < 4; ++c) {
+        start[c] = (int)order.size();
+        for (Chain *C : b->active)
+            if (chain_class(C->bi) == c) order.push_back(C);
     }
-    const size_t bi_bytes = sizeof(BInst) * std::max(n, 1);
-    const size_t off_hv = (bi_bytes + 255) & ~size_t(255);
-    const size_t off_ax = off_hv + (((size_t)(n + 1) * 4 + 255) & ~size_t(255));
-    const size_t total = off_ax + (size_t)(n + 1) * 4;
+    start[4] = (int)order.size();
+    const int n = start[4];
+    std::vector<int> hv[4], ax(n + 1);
+    int lds[4] = {1, 1, 1, 1};
+    ax[0] = 0;
+    for (int c = 0; c < 4; ++c) {
+        hv[c].assign(start[c + 1] - start[c] + 1, 0);
+        for (int i = start[c]; i < start[c + 1]; ++i) {
+            const BInst &I = order[i]->bi;
+            hv[c][i - start[c] + 1] = hv[c][i - start[c]] + I.hvn;
+            lds[c] = std::max(lds[c], (c >= 2 ? 1 : I.cb) * I.op.nup);
+        }
+    }
+    for (int i = 0; i < n; ++i) ax[i + 1] = ax[i] + order[i]->bi.axn;
+    auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+    size_t off[4];
+    size_t total = al(sizeof(BInst) * std::max(n, 1));
+    for (int c = 0; c < 4; ++c) {
+        off[c] = total;
+        total += al(hv[c].size() * 4);
+    }
+    const size_t off_ax = total;
+    total += (size_t)(n + 1) * 4;
     const int f = b->flip;
     if (b->hdesc_used[f]) BCHK(b, hipEventSynchronize(b->hdesc_ev[f]));
     if (b->hdesc_cap[f] < total) {
@@ -686,19 +727,23 @@ int rebuild(edg_batch *b)
         b->ddesc_cap = total * 2;
     }
     char *h = b->hdesc[f];
-    for (int i = 0; i < n; ++i) std::memcpy(h + sizeof(BInst) * i, &b->active[i]->bi, sizeof(BInst));
-    std::memcpy(h + off_hv, hv.data(), 4 * (size_t)(n + 1));
+    for (int i = 0; i < n; ++i) std::memcpy(h + sizeof(BInst) * i, &order[i]->bi, sizeof(BInst));
+    for (int c = 0; c < 4; ++c) std::memcpy(h + off[c], hv[c].data(), 4 * hv[c].size());
     std::memcpy(h + off_ax, ax.data(), 4 * (size_t)(n + 1));
     BCHK(b, hipMemcpyAsync(b->ddesc, h, total, hipMemcpyHostToDevice, b->stream));
     BCHK(b, hipEventRecord(b->hdesc_ev[f], b->stream));
     b->hdesc_used[f] = true;
     b->flip ^= 1;
     b->d_inst = (const BInst *)b->ddesc;
-    b->d_hvoff = (const int *)(b->ddesc + off_hv);
     b->d_axoff = (const int *)(b->ddesc + off_ax);
-    b->ntiles = hv[n];
+    for (int c = 0; c < 4; ++c) {
+        b->d_hvoff[c] = (const int *)(b->ddesc + off[c]);
+        b->cls_start[c] = start[c];
+        b->cls_n[c] = start[c + 1] - start[c];
+        b->cls_tiles[c] = hv[c].back();
+        b->cls_lds[c] = lds[c];
+    }
     b->nax = ax[n];
-    b->lds_elems = lds;
     b->dirty = false;
     b->st_rebuild += 1;
     if (b->gexec) {
@@ -724,8 +769,12 @@ int rebuild(edg_batch *b)
 int enqueue_step(edg_batch *b)
 {
     const int n = (int)b->active.size();
-    if (launch_batch_hv(b->stream, b->fp64, b->d_inst, b->d_hvoff, n, b->ntiles, b->lds_elems))
-        return fail(b, EDG_ERR_HIP, "batched H.v launch failed");
+    for (int c = 0; c < 4; ++c) {
+        if (!b->cls_n[c]) continue;
+        if (launch_batch_hv(b->stream, b->fp64, c >= 2, (c & 1) ? 16 : 8, b->d_inst + b->cls_start[c],
+                            b->d_hvoff[c], b->cls_n[c], b->cls_tiles[c], b->cls_lds[c]))
+            return fail(b, EDG_ERR_HIP, "batched H.v launch failed");
+    }
     if (launch_batch_axpy(b->stream, b->fp64, b->d_inst, b->d_axoff, n, b->nax))
         return fail(b, EDG_ERR_HIP, "batched axpy launch failed");
     if (launch_batch_advance(b->stream, b->d_inst, n)) return fail(b, EDG_ERR_HIP, "advance launch failed");

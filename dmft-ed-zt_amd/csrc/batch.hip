@@ -1,6 +1,8 @@
 // Batched Lanczos steps over many independent chains (uneven sector sizes)
-// for the single-orbital impurity (norb == 1).  One launch of k_bhv runs the
-// H.v half of step j_i for every chain i, one launch of k_baxpy the axpy
+// for the single-orbital impurity (norb == 1).  One launch of k_bhv (chains
+// with short columns, several columns per tile) and one of k_bhvw (long
+// columns, one column per tile) run the H.v half of step j_i for every
+// chain i, one launch of k_baxpy the axpy
 // half, one launch of k_badvance moves every chain to its next step; each
 // chain keeps its own step index, alpha/beta arrays and partial sums, so
 // chains at different steps (admitted at different times) share the
@@ -38,7 +40,7 @@ __device__ __forceinline__ T *slot_of(const BInst &I, int k)
 // the staged column, down hops read the neighbour column at the same row
 // from global memory.  Hop records are the 32-bit compressed tables
 // (target rank | value index << 16, KC slots per row, self-padded).
-template <typename T, typename ACC>
+template <typename T, typename ACC, int KS>
 __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *inst, const int *hv_off, int ninst)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -56,7 +58,7 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *inst, const int *hv
     const int nup = op.nup, ndn = op.ndn, cb = I.cb;
     const int c0 = t * cb;
     const int ncol = ndn - c0 < cb ? ndn - c0 : cb;
-    const int kcu = op.up.KC, kcd = op.dn.KC;
+    constexpr int kcd = KS;
     const int j = *I.step;
     const T *x = slot_of<T>(I, j);
     T *y = slot_of<T>(I, j + 1);
@@ -101,23 +103,16 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *inst, const int *hv
         const int iup = e - c * nup;
         const ACC xv = (ACC)xs[e];
         ACC acc = (energy_of<ACC>(op.up, iup) + s_ed[c] + (op.up.mask[iup] && s_md[c] ? U : ACC(0))) * xv;
-        const uint32_t *ur = op.up.hrec + (long long)iup * kcu;
+        const Row<uint32_t, KS> ur = *reinterpret_cast<const Row<uint32_t, KS> *>(op.up.hrec + (long long)iup * KS);
         const T *xc = xs + c * nup;
+        const uint32_t *dr = s_drec + c * KS;
+        T gn[KS];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            if (k < kcu) {
-                const uint32_t w = ur[k];
-                acc += s_vu[w >> 16] * (ACC)xc[w & 0xFFFFu];
-            }
-        }
-        const uint32_t *dr = s_drec + c * kcd;
+        for (int k = 0; k < KS; ++k) gn[k] = x[(long long)(dr[k] & 0xFFFFu) * nup + iup];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            if (k < kcd) {
-                const uint32_t w = dr[k];
-                acc += s_vd[w >> 16] * (ACC)x[(long long)(w & 0xFFFFu) * nup + iup];
-            }
-        }
+        for (int k = 0; k < KS; ++k) acc += s_vu[ur.v[k] >> 16] * (ACC)xc[ur.v[k] & 0xFFFFu];
+#pragma unroll
+        for (int k = 0; k < KS; ++k) acc += s_vd[dr[k] >> 16] * (ACC)gn[k];
         const long long g = (long long)c0 * nup + e;
         double r = sx * (double)acc;
         if (pv) r -= bsp * (double)pv[g];
@@ -127,6 +122,99 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *inst, const int *hv
     }
     double mine[1] = {dot};
     block_partial_at<1>(mine, I.hvpart, t);
+}
+
+// H.v tile for chains with long sector columns: one column idn = t per
+// 512-thread workgroup (as in the single-chain kernel).  The column's
+// down-hop targets and elements are the same for every row, so they are
+// decoded once into uniform registers and each down hop is one coalesced
+// load of a neighbour column; up hops gather from the staged column.
+template <typename T, typename ACC, int KS>
+__global__ __launch_bounds__(kHvBlock) void k_bhvw(const BInst *inst, const int *hv_off, int ninst)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *xs = reinterpret_cast<T *>(smem);
+    __shared__ ACC s_vu[2 * (kMaxHops + 1)];
+
+    const int i = owner(hv_off, ninst, blockIdx.x);
+    const BInst &I = inst[i];
+    if (I.flags[0]) return;
+    const int t = blockIdx.x - hv_off[i];
+    const HvOp &op = I.op;
+    const int nup = op.nup, idn = t;
+    const int j = *I.step;
+    const T *x = slot_of<T>(I, j);
+    T *y = slot_of<T>(I, j + 1);
+    const T *pv = j > 0 ? slot_of<T>(I, j - 1) : nullptr;
+    const T *xc = x + (long long)idn * nup;
+
+    for (int e = threadIdx.x; e <= kMaxHops; e += kHvBlock) {
+        const ACC vu = e < op.nvup ? (ACC)op.up.vsrc[e] : ACC(0);
+        s_vu[e] = vu;
+        s_vu[kMaxHops + 1 + e] = -vu;
+    }
+    for (int i0 = threadIdx.x; i0 < nup; i0 += 8 * kHvBlock) {
+        T v[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int e = i0 + r * kHvBlock;
+            v[r] = xc[e < nup ? e : nup - 1];
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+            if (i0 + r * kHvBlock < nup) xs[i0 + r * kHvBlock] = v[r];
+    }
+    __syncthreads();
+
+    HvArgs la{};
+    la.step = j;
+    la.prev_part = I.axpart;
+    la.prev_n = I.axn;
+    la.scale = I.scale;
+    la.beta = I.beta;
+    la.stop = I.flags;
+    la.nstep = I.flags + 1;
+    double sx, bsp;
+    if (!lanczos_scales<kHvBlock>(la, sx, bsp, t == 0)) return;
+
+    // down hops of this column (uniform): byte offsets and elements
+    uint32_t coff[KS];
+    ACC dval[KS];
+    const uint32_t *dr = op.dn.hrec + (long long)idn * KS;
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+        const uint32_t w = dr[k];
+        const uint32_t vi = w >> 16;
+        const int l = vi > (uint32_t)kMaxHops ? (int)vi - (kMaxHops + 1) : (int)vi;
+        const ACC v = l < op.nvup ? (ACC)op.dn.vsrc[l] : ACC(0);
+        dval[k] = vi > (uint32_t)kMaxHops ? -v : v;
+        coff[k] = (w & 0xFFFFu) * (uint32_t)nup * (uint32_t)sizeof(T);
+    }
+    const ACC ed = energy_of<ACC>(op.dn, idn);
+    const ACC edU = op.dn.mask[idn] ? ed + (ACC)op.U : ed;
+    const unsigned char *xb = reinterpret_cast<const unsigned char *>(x);
+    double dot = 0.0;
+    for (int iup = threadIdx.x; iup < nup; iup += kHvBlock) {
+        const uint32_t ib = (uint32_t)iup * (uint32_t)sizeof(T);
+        T g[KS];
+#pragma unroll
+        for (int k = 0; k < KS; ++k) g[k] = *reinterpret_cast<const T *>(xb + (coff[k] + ib));
+        const Row<uint32_t, KS> ur = *reinterpret_cast<const Row<uint32_t, KS> *>(op.up.hrec + (long long)iup * KS);
+        const ACC xv = (ACC)xs[iup];
+        ACC acc = (energy_of<ACC>(op.up, iup) + (op.up.mask[iup] ? edU : ed)) * xv;
+#pragma unroll
+        for (int k = 0; k < KS; ++k) acc += s_vu[ur.v[k] >> 16] * (ACC)xs[ur.v[k] & 0xFFFFu];
+#pragma unroll
+        for (int k = 0; k < KS; ++k) acc += dval[k] * (ACC)g[k];
+        const long long gi = (long long)idn * nup + iup;
+        double r = sx * (double)acc;
+        if (pv) r -= bsp * (double)pv[gi];
+        const T rt = (T)r;
+        y[gi] = rt;
+        dot += (double)xv * (double)rt;
+    }
+    double mine[1] = {dot};
+    block_partial_at<1, kHvBlock>(mine, I.hvpart, t);
 }
 
 template <typename T>
@@ -159,24 +247,33 @@ __global__ void k_badvance(const BInst *inst, int ninst)
     if (i < ninst) *inst[i].step += 1;
 }
 
-int batch_set_lds(size_t max_lds)
+template <int KS>
+static int set_lds_ks(size_t max_lds)
 {
-    if (hipFuncSetAttribute((const void *)k_bhv<float, float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)max_lds) != hipSuccess)
-        return -1;
-    if (hipFuncSetAttribute((const void *)k_bhv<double, double>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds) != hipSuccess)
-        return -1;
+    const void *fs[4] = {(const void *)k_bhv<float, float, KS>, (const void *)k_bhv<double, double, KS>,
+                         (const void *)k_bhvw<float, float, KS>, (const void *)k_bhvw<double, double, KS>};
+    for (const void *f : fs)
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds) != hipSuccess)
+            return -1;
     return 0;
 }
 
-int launch_batch_hv(hipStream_t s, bool fp64, const BInst *inst, const int *hv_off, int ninst,
-                    int ntiles, int lds_elems)
+int batch_set_lds(size_t max_lds) { return set_lds_ks<8>(max_lds) || set_lds_ks<16>(max_lds) ? -1 : 0; }
+
+int launch_batch_hv(hipStream_t s, bool fp64, bool wide, int ks, const BInst *inst, const int *hv_off,
+                    int ninst, int ntiles, int lds_elems)
 {
     if (ntiles <= 0) return 0;
     const size_t lds = (size_t)lds_elems * (fp64 ? 8 : 4);
-    void (*f)(const BInst *, const int *, int) = fp64 ? k_bhv<double, double> : k_bhv<float, float>;
-    hipLaunchKernelGGL(f, dim3(ntiles), dim3(kBlock), lds, s, inst, hv_off, ninst);
+    void (*f)(const BInst *, const int *, int);
+    if (wide) {
+        if (ks == 8) f = fp64 ? k_bhvw<double, double, 8> : k_bhvw<float, float, 8>;
+        else f = fp64 ? k_bhvw<double, double, 16> : k_bhvw<float, float, 16>;
+    } else {
+        if (ks == 8) f = fp64 ? k_bhv<double, double, 8> : k_bhv<float, float, 8>;
+        else f = fp64 ? k_bhv<double, double, 16> : k_bhv<float, float, 16>;
+    }
+    hipLaunchKernelGGL(f, dim3(ntiles), dim3(wide ? kHvBlock : kBlock), lds, s, inst, hv_off, ninst);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

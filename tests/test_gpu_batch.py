@@ -124,6 +124,29 @@ def test_batch_matches_single_problem_path(edgpu, oracle):
     assert np.abs(res[0]["G"][:, 0] - G[:, 0, 0]).max() < 1e-9
 
 
+def test_long_columns_with_short_ones(edgpu, oracle):
+    """Ns = 13 (nup = C(13,6) = 1716, long-column tiles) in the same batched
+    steps as Ns = 6 and 8 problems (short-column tiles); E0 and G against the
+    single-problem path, FP64 vectors."""
+    omega = oracle.matsubara(128, 100.0)
+    big = problem(edgpu, 12, 2.0, 71, nup=6, ndn=6, maxnstep=60, maxnstep_green=40)
+    probs = [problem(edgpu, 5, 1.5, 72, maxnstep=60, maxnstep_green=40), big,
+             problem(edgpu, 7, 2.5, 73, maxnstep=60, maxnstep_green=40)]
+    res, _, _ = run_batch(edgpu, probs, omega)
+    for pr, r in zip(probs, res):
+        s = edgpu.Solver(norb=1, nbath=pr.nbath, U=pr.U, mu=pr.mu, maxnstep=60,
+                         maxnstep_green=40, seed=pr.seed)
+        em, ek = pr.em.reshape(1, 2, order="F"), pr.ek.reshape(pr.nbath, 2, order="F")
+        vmk = pr.vmk.reshape(1, pr.nbath, 2, order="F")
+        gs = s.ground_state(em, ek, vmk, [(pr.nup, pr.ndn)])
+        G, _ = s.cluster_green_ftn(em, ek, vmk, omega)
+        s.close()
+        print(f"Ns={pr.nbath + 1}: dE0 {abs(r['e0'] - gs.val):.2e} dG {np.abs(r['G'][:, 0] - G[:, 0, 0]).max():.2e}",
+              flush=True)
+        assert abs(r["e0"] - gs.val) < 1e-10
+        assert np.abs(r["G"][:, 0] - G[:, 0, 0]).max() < 1e-8
+
+
 @pytest.mark.parametrize("graphs", [True, False])
 def test_result_independent_of_batch(edgpu, oracle, graphs):
     """A problem solved alone and inside a mix of other sizes (admitted at
