@@ -18,6 +18,15 @@ stream.  cpu_baseline = the FP64 C restatement of the reference algorithm
 (oracle/, OpenMP) timed on the host for a bounded number of Lanczos steps
 at the same size and scaled to one solve.
 
+The same JSON line also carries:
+  batched   512 independent single-orbital problems (Ns in {8,10,12}, random
+            U, filling and bath) through the batched solver: solves/s;
+  served    a Poisson stream of 200 requests/s (Ns in {10,12,14}) replayed
+            against the wall clock for --serve-seconds: solves/s, p50/p99
+            latency (result time - arrival time);
+  kanamori  the two-orbital Kanamori Ns=14 problem, FP32 vs FP64 vectors:
+            solve times, |dE0|, max|dG|, H.v time.
+
 Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1]
 """
 import argparse
