@@ -201,7 +201,7 @@ int get_tab(edg_ctx *c, int spin, int ne, const SpinTab **out)
         HIPCHK(c, hipStreamSynchronize(c->stream));
         te.t.kmax = km;
         te.t.vsrc = v_s;
-        if (norb == 1 && km <= 16 && te.t.n <= 65536) {  // compressed tables (16-bit ranks)
+        if (norb <= 2 && km <= 16 && te.t.n <= 65536) {  // compressed tables (16-bit ranks)
             te.t.KC = km <= 8 ? 8 : 16;
             if (!te.t.ccol) {
                 auto al2 = [&](void **p, size_t bytes) -> int {

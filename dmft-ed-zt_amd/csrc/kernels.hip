@@ -627,6 +627,173 @@ __global__ __launch_bounds__(kHvBlock) void k_hv1(HvOp op, HvArgs a)
     }
 }
 
+// Two-orbital (Kanamori) H.v, one sector column per 512-thread workgroup.
+// Besides the column idn itself, the workgroup stages in LDS the target
+// columns of the column's down-spin orbital moves: for each ordered orbital
+// pair (i, j) the spin-flip term moves the down electron i -> j and the
+// pair-hop term j -> i, and a down pattern allows at most one of the two,
+// so each pair contributes at most one staged column.  The spin-flip and
+// pair-hop elements +-Jp * s_up * s_dn (impurity_hamiltonian.f90:176-230)
+// then gather from LDS at the up-move target row.  Up hops use the 32-bit
+// records (KS slots stored, KH processed) gathered from the staged column;
+// down hops are coalesced loads of neighbour columns at wave-uniform
+// offsets.  The diagonal adds U n_m,up n_m,dn and U' for unlike orbitals of
+// opposite spin to the one-spin energies (which carry U' - J for like spins).
+constexpr int kNorb2 = 2;
+constexpr int kPairs2 = kNorb2 * (kNorb2 - 1);
+
+template <typename T, typename ACC, int MODE, int KH>
+__global__ __launch_bounds__(kHvBlock) void k_hv2(HvOp op, HvArgs a)
+{
+    constexpr int KS = KH <= 8 ? 8 : 16;
+    if (a.stop && *a.stop) return;
+    using Hop = typename HopOf<ACC>::type;
+    extern __shared__ __align__(64) unsigned char smem[];
+    const int nup = op.nup, ndn = op.ndn;
+    const size_t csz = align16((size_t)nup * sizeof(T));
+    T *xs = reinterpret_cast<T *>(smem);                    // the column itself
+    __shared__ ACC s_v[2 * (kMaxHops + 1)];
+    __shared__ int s_kcol[kPairs2];
+    __shared__ ACC s_kval[kPairs2];
+
+    const int idn = xcd_group(blockIdx.x, ndn);
+    const T *x = static_cast<const T *>(a.x);
+    const unsigned char *xb = static_cast<const unsigned char *>(a.x);
+    for (int e = threadIdx.x; e <= kMaxHops; e += kHvBlock) {
+        const ACC v = e < op.nvup ? (ACC)op.up.vsrc[e] : ACC(0);
+        s_v[e] = v;
+        s_v[kMaxHops + 1 + e] = -v;
+    }
+    // down orbital moves of this column: pair p = (i, j), up move j -> i;
+    // spin flip: down i -> j; pair hop: down j -> i (at most one is valid)
+    if (threadIdx.x < kPairs2) {
+        const int p = threadIdx.x;
+        const int i = p / (kNorb2 - 1), jj = p % (kNorb2 - 1), j = jj >= i ? jj + 1 : jj;
+        const uint32_t tsf = op.dn.orb[(long long)(j * kNorb2 + i) * ndn + idn];
+        const uint32_t tph = op.dn.orb[(long long)(i * kNorb2 + j) * ndn + idn];
+        const uint32_t t = tsf != kInvalid ? tsf : tph;
+        s_kcol[p] = t != kInvalid ? (int)(t & 0x7FFFFFFFu) : -1;
+        s_kval[p] = t != kInvalid ? ((t >> 31) ? -(ACC)op.Jp : (ACC)op.Jp) : ACC(0);
+    }
+    // down hops of this column (wave-uniform)
+    uint32_t coff[KH];
+    ACC dval[KH];
+    const Hop *hr = HopOf<ACC>::tab(op.dn) + (long long)idn * op.dn.K;
+#pragma unroll
+    for (int k = 0; k < KH; ++k) {
+        const Hop h = hr[k];
+        coff[k] = h.col * (uint32_t)nup * (uint32_t)sizeof(T);
+        dval[k] = (ACC)h.val;
+    }
+    const ACC ed = energy_of<ACC>(op.dn, idn);
+    const int md = op.dn.mask[idn];
+    __syncthreads();  // s_kcol
+    // stage the column and the (up to kPairs2) orbital-move target columns
+#pragma unroll
+    for (int q = 0; q <= kPairs2; ++q) {
+        const int col = q == 0 ? idn : s_kcol[q - 1];
+        if (col < 0) continue;  // uniform
+        T *dst = reinterpret_cast<T *>(smem + (size_t)q * csz);
+        const T *src = x + (long long)col * nup;
+        for (int i0 = threadIdx.x; i0 < nup; i0 += 8 * kHvBlock) {
+            T v[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const int e = i0 + r * kHvBlock;
+                v[r] = src[e < nup ? e : nup - 1];
+            }
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+                if (i0 + r * kHvBlock < nup) dst[i0 + r * kHvBlock] = v[r];
+        }
+    }
+    __syncthreads();
+
+    double sums[3] = {0.0, 0.0, 0.0};
+    double sx = 1.0, bsp = 0.0;
+    if constexpr (MODE == HV_LANCZOS) {
+        if (!lanczos_scales<kHvBlock>(a, sx, bsp, blockIdx.x == 0)) return;
+    }
+    T *y = static_cast<T *>(a.y);
+    const T *pv = static_cast<const T *>(a.p);
+    const ACC U = (ACC)op.U, Up = (ACC)op.Up;
+    for (int iup = threadIdx.x; iup < nup; iup += kHvBlock) {
+        const uint32_t ib = (uint32_t)iup * (uint32_t)sizeof(T);
+        T g[KH];
+#pragma unroll
+        for (int k = 0; k < KH; ++k) g[k] = *reinterpret_cast<const T *>(xb + (coff[k] + ib));
+        const Row<uint32_t, KS> hrow = *reinterpret_cast<const Row<uint32_t, KS> *>(op.up.hrec + (long long)iup * KS);
+        uint32_t tu[kPairs2];
+#pragma unroll
+        for (int p = 0; p < kPairs2; ++p) {
+            const int i = p / (kNorb2 - 1), jj = p % (kNorb2 - 1), j = jj >= i ? jj + 1 : jj;
+            tu[p] = op.up.orb[(long long)(i * kNorb2 + j) * nup + iup];  // up move j -> i
+        }
+        const int mu = op.up.mask[iup];
+        const int both = __popc(mu & md);
+        const ACC diag = energy_of<ACC>(op.up, iup) + ed + U * (ACC)both +
+                         Up * (ACC)(__popc(mu) * __popc(md) - both);
+        const ACC xv = (ACC)xs[iup];
+        ACC acc = diag * xv;
+#pragma unroll
+        for (int k = 0; k < KH; ++k) {
+            const uint32_t w = hrow.v[k];
+            acc += s_v[w >> 16] * (ACC)xs[w & 0xFFFFu];
+        }
+#pragma unroll
+        for (int k = 0; k < KH; ++k) acc += dval[k] * (ACC)g[k];
+#pragma unroll
+        for (int p = 0; p < kPairs2; ++p) {
+            if (s_kcol[p] < 0) continue;  // uniform
+            const bool ok = tu[p] != kInvalid;
+            const uint32_t ru = ok ? (tu[p] & 0x7FFFFFFFu) : 0u;
+            const ACC su = ok ? ((tu[p] >> 31) ? -s_kval[p] : s_kval[p]) : ACC(0);
+            const T *xk = reinterpret_cast<const T *>(smem + (size_t)(p + 1) * csz);
+            acc += su * (ACC)xk[ru];
+        }
+        const long long e = (long long)idn * nup + iup;
+        if constexpr (MODE == HV_PLAIN) {
+            y[e] = (T)acc;
+        } else if constexpr (MODE == HV_ACCUM) {
+            y[e] = (T)((ACC)y[e] + acc);
+        } else if constexpr (MODE == HV_LANCZOS) {
+            double r = sx * (double)acc;
+            if (pv) r -= bsp * (double)pv[e];
+            const T rt = (T)r;
+            y[e] = rt;
+            sums[0] += (double)xv * (double)rt;
+        } else {
+            const double hv = (double)acc;
+            sums[0] += (double)xv * hv;
+            sums[1] += hv * hv;
+            sums[2] += (double)xv * (double)xv;
+        }
+    }
+    if constexpr (MODE == HV_LANCZOS) {
+        double mine[1] = {sums[0]};
+        block_partial<1, kHvBlock>(mine, a.partial);
+    } else if constexpr (MODE == HV_RQ) {
+        block_partial<3, kHvBlock>(sums, a.partial);
+    }
+}
+
+template <typename T, typename ACC, int KH>
+static int launch_hv2_k(hipStream_t s, const HvOp &op, int mode, const HvArgs &a)
+{
+    const size_t lds = (size_t)(kPairs2 + 1) * align16((size_t)op.nup * sizeof(T));
+    void (*f)(HvOp, HvArgs);
+    if (mode == HV_PLAIN) f = k_hv2<T, ACC, HV_PLAIN, KH>;
+    else if (mode == HV_ACCUM) f = k_hv2<T, ACC, HV_ACCUM, KH>;
+    else if (mode == HV_LANCZOS) f = k_hv2<T, ACC, HV_LANCZOS, KH>;
+    else f = k_hv2<T, double, HV_RQ, KH>;
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return -1;
+    hipLaunchKernelGGL(f, dim3(op.ndn), dim3(kHvBlock), lds, s, op, a);
+    if (hipGetLastError() != hipSuccess) return -1;
+    return finish_hv(s, mode, op.ndn, a);
+}
+
 // Persistent single-orbital H.v with the up-spin hop table resident in LDS.
 // One workgroup per CU copies the compressed table (16-bit target rank and
 // 8-bit element index per slot, 24 bytes per row) into LDS once; every hop
@@ -927,6 +1094,15 @@ static int launch_hv_a(hipStream_t s, const HvOp &op, int mode, int cols, const 
 {
     const bool kan = op.norb >= 2;
     const int km = op.up.kmax > op.dn.kmax ? op.up.kmax : op.dn.kmax;
+    // two orbitals: the specialized kernel (tables with <= 16 slots, LDS for 3 columns)
+    if (op.norb == kNorb2 && op.up.KC >= 8 && op.up.KC == op.dn.KC && km <= 16 && op.up.hrec &&
+        (size_t)(kPairs2 + 1) * align16((size_t)op.nup * sizeof(T)) <= kLdsBudget && op.sched < 16) {
+        if (km <= 8 && op.up.KC == 8) return launch_hv2_k<T, ACC, 8>(s, op, mode, a);
+        if (op.up.KC == 16) {
+            if (km <= 12) return launch_hv2_k<T, ACC, 12>(s, op, mode, a);
+            return launch_hv2_k<T, ACC, 16>(s, op, mode, a);
+        }
+    }
     if (kan) return launch_hv_k<T, ACC, true, 0>(s, op, mode, cols, a);
     // persistent table-in-LDS kernel: schedule 16 = 1 column per group and
     // 512 threads, 32 = 1 column and 1024 threads, 64 = 2 columns and 1024
