@@ -117,16 +117,19 @@ def profile_hv(edgpu, em, ek, vmk, U, mu):
 def bench_batched(edgpu, W, S):
     """512 independent single-orbital problems (Ns in {8,10,12}) through the
     batched solver: whole-batch solves/s, after a warm-up batch of 32."""
+    from edgpu import autotune
     om = W.matsubara(2048, 100.0)
     probs = W.batched_problems(edgpu, n=512)
-    S.run_batched(edgpu, probs[:32], om)
-    res, wall, st = S.run_batched(edgpu, probs, om)
+    sched, timings = autotune.tune_batch(edgpu, probs, om)
+    S.run_batched(edgpu, probs[:32], om, schedule=sched)
+    res, wall, st = S.run_batched(edgpu, probs, om, schedule=sched)
     ns = [p.nbath + 1 for p in probs]
     return dict(workload="512 single-orbital problems, Ns uniform in {8,10,12}, U~U[0,6], "
                          "n~U[0.8,1.2] (N=round(n Ns), N_up=ceil(N/2)), eps~U[-1.5,1.5], "
                          "V~U[0.1,0.7]; 200 GS steps + c+/c chains of 150 steps, FP32 vectors, "
                          "2048 Matsubara points",
                 problems=len(probs), ns_counts={str(k): ns.count(k) for k in (8, 10, 12)},
+                schedule=[list(e) for e in sched],
                 solves_per_s=len(probs) / wall, wall_s=wall, batched_steps=st["steps"],
                 chain_steps_per_s=st["chain_steps"] / wall,
                 e0_finite=bool(np.all(np.isfinite([r["e0"] for r in res]))))
@@ -136,12 +139,15 @@ def bench_served(edgpu, W, S, seconds):
     """Poisson arrivals at 200 req/s for `seconds` (Ns 10/12/14 with weights
     0.5/0.35/0.15, U~U[0,8]) replayed against the wall clock; latency =
     result time - arrival time, solves/s over the whole run (arrivals + drain)."""
+    from edgpu import autotune
     om = W.matsubara(2048, 100.0)
     req = W.served_requests(edgpu, rate=200.0, duration=seconds)
-    r = S.serve(edgpu, req, om, drain_limit=300.0, log=log)
+    sched, _ = autotune.tune_batch(edgpu, [p for _, p in req], om, sample=8)
+    r = S.serve(edgpu, req, om, drain_limit=300.0, log=log, schedule=sched)
     return dict(workload=f"Poisson 200 req/s for {seconds:g} s, Ns in {{10,12,14}} (0.5/0.35/0.15), "
                          "U~U[0,8], half filling, 200 GS steps + 2 chains x 150 steps, FP32 vectors; "
                          "admission by device bytes (stored Krylov basis + rings)",
+                schedule=[list(e) for e in sched],
                 submitted=r["submitted"], completed=r["completed"], wall_s=r["wall"],
                 solves_per_s=r["solves_per_s"], p50_s=r["p50"], p99_s=r["p99"],
                 offered_per_s=r["submitted"] / seconds if seconds else 0.0)
@@ -212,8 +218,10 @@ def main():
     om = W.matsubara(2048, 100.0)
     D = 3432 * 3432
 
+    from edgpu import autotune
     s32 = make_solver(edgpu, "fp32", U, mu)
-    log("H.v kernel timing (FP32)")
+    log("H.v schedule autotuning and timing (FP32)")
+    best, sched_us = autotune.tune_single(s32, em, ek, vmk, 7, 7)
     hv_ms, hv_bytes = s32.time_hv(em, ek, vmk, 7, 7, warmup=20, reps=200)
     hv_gbs = hv_bytes / (hv_ms * 1e-3) / 1e9
 
@@ -236,6 +244,7 @@ def main():
     if not args.no_fp64:
         log("FP64 solves")
         s64 = make_solver(edgpu, "fp64", U, mu)
+        autotune.tune_single(s64, em, ek, vmk, 7, 7)
         solve(s64, em, ek, vmk, om)
         lat64 = []
         for _ in range(max(1, min(args.steps, 3))):
@@ -287,6 +296,8 @@ def main():
                    "parallelism": "single GPU"},
         "hv_gbs": hv_gbs,
         "hv_us": hv_ms * 1e3,
+        "hv_schedule": best,
+        "hv_schedule_us": {str(k): round(v, 2) for k, v in sched_us.items()},
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),
         "gs_phase_ms": st["gs_ms"],

@@ -163,6 +163,12 @@ struct edg_batch {
     std::vector<double *> pinned_free;
     size_t pinned_n = 0;
     double st_steps = 0, st_chain_steps = 0, st_done = 0, st_graph = 0, st_rebuild = 0;
+    // H.v tile schedule by size class: the first entry with nsite <= nsite_max
+    // applies to every chain of a problem
+    struct Sched {
+        int nsite_max, tile, wide;
+    };
+    std::vector<Sched> sched{{64, kBatchTile, -1}};  // wide = -1: nup >= kBatchWide
 };
 
 namespace {
@@ -202,19 +208,28 @@ size_t state_bytes(int maxn, int hvn, int axn)
     return 256 * 8 + ((size_t)maxn + 1) * 3 * 8 + 16 + (size_t)(hvn + axn) * 8;
 }
 
-int chain_cb(int nup)
+// Tile shape of a chain with columns of nup rows under the engine's
+// schedule: long-column tiles (one column per 512-thread workgroup) or
+// cb = clamp(tile / nup, 1, kBatchMaxCols) columns per 256-thread workgroup.
+struct Shape {
+    int wide, cb;
+};
+Shape chain_shape(const edg_batch *b, int nsite, long long nup)
 {
-    int cb = kBatchTile / std::max(nup, 1);
-    return std::min(std::max(cb, 1), kBatchMaxCols);
+    for (const auto &e : b->sched)
+        if (nsite <= e.nsite_max) {
+            const int wide = e.wide < 0 ? (nup >= kBatchWide) : e.wide;
+            int cb = e.tile / (int)std::max<long long>(nup, 1);
+            cb = std::min(std::max(cb, 1), kBatchMaxCols);
+            return {wide, wide ? 1 : cb};
+        }
+    return {nup >= kBatchWide ? 1 : 0, 1};
 }
 
-// H.v tiles of a chain: long columns (nup >= kBatchWide) take one column per
-// 512-thread tile, short ones chain_cb columns per 256-thread tile.
-int chain_hvn(long long nup, long long ndn)
+int chain_hvn(const edg_batch *b, int nsite, long long nup, long long ndn)
 {
-    if (nup >= kBatchWide) return (int)ndn;
-    const int cb = chain_cb((int)nup);
-    return (int)((ndn + cb - 1) / cb);
+    const Shape sh = chain_shape(b, nsite, nup);
+    return (int)((ndn + sh.cb - 1) / sh.cb);
 }
 
 int chain_axn(long long D)
@@ -236,7 +251,7 @@ size_t problem_bytes(const edg_batch *b, const edg_problem &p)
     const long long nu = binom_ll(ns, p.nup), nd = binom_ll(ns, p.ndn);
     const long long D = nu * nd, Dp = rup(D, kPadB);
     bytes += ((size_t)p.maxnstep + 1) * Dp * b->es + Dp * b->es;
-    bytes += state_bytes(p.maxnstep, chain_hvn(nu, nd), chain_axn(D));
+    bytes += state_bytes(p.maxnstep, chain_hvn(b, ns, nu, nd), chain_axn(D));
     const int mg = green_len(p);
     bytes += 64 * 8 + 8 * (8 * (size_t)std::max(mg, p.maxnstep) + 64) + 8 * 4 * (size_t)b->nw;
     for (int k = 0; k < 4; ++k) {
@@ -246,7 +261,7 @@ size_t problem_bytes(const edg_batch *b, const edg_problem &p)
         if (cu < 0 || cd < 0 || cu > ns || cd > ns) continue;
         const long long cnu = binom_ll(ns, cu), cnd = binom_ll(ns, cd), Dc = cnu * cnd;
         bytes += 3 * rup(Dc, kPadB) * b->es + 4 * (size_t)(ispin == 1 ? cnu : cnd) + 256;
-        bytes += state_bytes(mg, chain_hvn(cnu, cnd), chain_axn(Dc));
+        bytes += state_bytes(mg, chain_hvn(b, ns, cnu, cnd), chain_axn(Dc));
     }
     return bytes + 64 * 256;
 }
@@ -268,7 +283,8 @@ int validate(const edg_batch *b, const edg_problem &p)
             if (kmax_single(p.nbath, ne) > 16)
                 return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "more than 16 hops per configuration");
             const long long n = binom_ll(ns, ne);
-            if ((size_t)std::max<long long>(n, kBatchTile) * b->es > kLdsMax)
+            const Shape sh = chain_shape(b, ns, n);
+            if ((size_t)sh.cb * n * b->es > kLdsMax)
                 return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "sector column exceeds LDS");
         }
     }
@@ -354,8 +370,10 @@ int init_chain(edg_batch *b, Job *J, Chain &C, const HvOp &op, int maxn, bool st
     I.stride = Dp;
     I.base = J->ar.take((size_t)I.nslot * Dp * b->es);
     I.maxn = maxn;
-    I.cb = op.nup >= kBatchWide ? 1 : chain_cb(op.nup);
-    I.hvn = chain_hvn(op.nup, op.ndn);
+    const Shape sh = chain_shape(b, nsite_of(J->p), op.nup);
+    I.wide = sh.wide;
+    I.cb = sh.cb;
+    I.hvn = chain_hvn(b, nsite_of(J->p), op.nup, op.ndn);
     I.axn = chain_axn(D);
     if ((size_t)I.cb * op.nup * b->es > kLdsMax) return fail(b, EDG_ERR_ARG, "tile exceeds LDS");
     char *st = (char *)J->ar.take(state_bytes(maxn, I.hvn, I.axn));
@@ -677,7 +695,7 @@ int progress(edg_batch *b)
     return 0;
 }
 
-int chain_class(const BInst &I) { return (I.op.nup >= kBatchWide ? 2 : 0) + (I.op.up.KC == 16 ? 1 : 0); }
+int chain_class(const BInst &I) { return (I.wide ? 2 : 0) + (I.op.up.KC == 16 ? 1 : 0); }
 
 // Upload the descriptor table of the active chains grouped by launch class
 // (short / long columns x 8 / 16 hop slots), with the tile prefix sums of
@@ -701,7 +719,7 @@ int rebuild(edg_batch *b)
         for (int i = start[c]; i < start[c + 1]; ++i) {
             const BInst &I = order[i]->bi;
             hv[c][i - start[c] + 1] = hv[c][i - start[c]] + I.hvn;
-            lds[c] = std::max(lds[c], (c >= 2 ? 1 : I.cb) * I.op.nup);
+            lds[c] = std::max(lds[c], I.cb * I.op.nup);
         }
     }
     for (int i = 0; i < n; ++i) ax[i + 1] = ax[i] + order[i]->bi.axn;
@@ -917,6 +935,22 @@ int edg_batch_set_graphs(edg_batch *b, int use_graphs)
 {
     if (!b) return EDG_ERR_ARG;
     b->use_graphs = use_graphs != 0;
+    return EDG_OK;
+}
+
+int edg_batch_set_schedule(edg_batch *b, int n, const int *nsite_max, const int *tile_elems, const int *wide)
+{
+    if (!b || n < 1 || n > 16 || !nsite_max || !tile_elems || !wide) return EDG_ERR_ARG;
+    if (!b->queue.empty() || !b->running.empty())
+        return fail(b, EDG_ERR_STATE, "the schedule can change only while no problem is queued or in flight");
+    std::vector<edg_batch::Sched> s;
+    for (int i = 0; i < n; ++i) {
+        if (tile_elems[i] < 64 || tile_elems[i] > 16384 || (wide[i] != 0 && wide[i] != 1) || nsite_max[i] < 1 ||
+            (i > 0 && nsite_max[i] <= nsite_max[i - 1]))
+            return fail(b, EDG_ERR_ARG, "schedule entries: increasing nsite_max, tile 64..16384, wide 0/1");
+        s.push_back({nsite_max[i], tile_elems[i], wide[i]});
+    }
+    b->sched = s;
     return EDG_OK;
 }
 

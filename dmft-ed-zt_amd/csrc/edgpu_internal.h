@@ -96,6 +96,7 @@ struct BInst {
     long long stride;
     int nslot;
     int maxn;
+    int wide;          // 1: one long column per 512-thread tile; 0: cb columns per 256-thread tile
     int cb;            // sector columns per H.v tile
     int hvn, axn;      // H.v tiles / axpy workgroups of this chain
     int *step;         // current step j

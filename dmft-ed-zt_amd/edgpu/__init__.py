@@ -54,7 +54,8 @@ EXPORTS = ["edg_default_params", "edg_create", "edg_destroy", "edg_last_error",
            "edg_apply_c", "edg_time_hv", "edg_last_solve_stats",
            "edg_batch_create", "edg_batch_destroy", "edg_batch_last_error",
            "edg_batch_set_graphs", "edg_batch_problem_bytes", "edg_batch_submit",
-           "edg_batch_step", "edg_batch_run", "edg_batch_poll", "edg_batch_stats"]
+           "edg_batch_step", "edg_batch_run", "edg_batch_poll", "edg_batch_stats",
+           "edg_batch_set_schedule"]
 
 _lib = None
 
@@ -105,6 +106,7 @@ def load_library(path=LIB_PATH):
     L.edg_batch_run.argtypes = [bh]
     L.edg_batch_poll.argtypes = [bh, C.POINTER(C.c_longlong), dp, dp, ip, dp]
     L.edg_batch_stats.argtypes = [bh, dp]
+    L.edg_batch_set_schedule.argtypes = [bh, C.c_int, ip, ip, ip]
     for name in EXPORTS:
         if name not in ("edg_default_params", "edg_last_error", "edg_batch_last_error"):
             getattr(L, name).restype = C.c_int
@@ -383,6 +385,12 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+    def set_schedule(self, entries):
+        """[(nsite_max, tile_elems, wide), ...] with increasing nsite_max."""
+        n = len(entries)
+        arr = [(C.c_int * n)(*[int(e[k]) for e in entries]) for k in range(3)]
+        self._chk(self._L.edg_batch_set_schedule(self._h, n, *arr))
 
     def problem_bytes(self, prob):
         n = C.c_ulonglong(0)

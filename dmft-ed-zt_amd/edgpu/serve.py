@@ -9,10 +9,14 @@ import time
 import numpy as np
 
 
-def run_batched(edgpu, problems, omega, precision="fp32", budget_bytes=0, max_steps=32, graphs=True):
+def run_batched(edgpu, problems, omega, precision="fp32", budget_bytes=0, max_steps=32, graphs=True,
+                schedule=None):
     """Solve a list of problems submitted at once; returns (results, wall
-    seconds, engine stats)."""
+    seconds, engine stats).  schedule: tile schedule by size class
+    (edgpu.autotune.tune_batch), default the engine's."""
     b = edgpu.Batch(omega, precision=precision, budget_bytes=budget_bytes, graphs=graphs)
+    if schedule:
+        b.set_schedule(schedule)
     t0 = time.perf_counter()
     ids = [b.submit(p) for p in problems]
     pending = len(ids)
@@ -30,11 +34,13 @@ def run_batched(edgpu, problems, omega, precision="fp32", budget_bytes=0, max_st
 
 
 def serve(edgpu, requests, omega, precision="fp32", budget_bytes=0, max_steps=8, graphs=True,
-          drain_limit=120.0, log=None):
+          drain_limit=120.0, log=None, schedule=None):
     """Replay [(t_arrival, Problem)] against the wall clock.  Returns a dict
     with per-request latencies (seconds), solves/s over the run, completion
     counts and engine stats."""
     b = edgpu.Batch(omega, precision=precision, budget_bytes=budget_bytes, graphs=graphs)
+    if schedule:
+        b.set_schedule(schedule)
     arrival = {}
     lat = []
     nxt = 0

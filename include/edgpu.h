@@ -205,6 +205,13 @@ const char *edg_batch_last_error(const edg_batch *b);
 /* Replay the batched step as a captured HIP graph (1, default) or launch
  * its kernels one by one (0). */
 int edg_batch_set_graphs(edg_batch *b, int use_graphs);
+/* H.v tile schedule by size class (set while the engine is idle): entry i
+ * applies to every chain of problems with nsite <= nsite_max[i] (increasing);
+ * wide[i] = 1 gives one sector column per 512-thread tile, wide[i] = 0 gives
+ * clamp(tile_elems[i] / nup, 1, 64) columns of nup rows per 256-thread tile.
+ * The default is one class: tiles of about 2048 elements, one column per
+ * 512-thread tile from nup >= 1024.  Set by the autotuner (edgpu.autotune). */
+int edg_batch_set_schedule(edg_batch *b, int n, const int *nsite_max, const int *tile_elems, const int *wide);
 /* Device bytes one problem holds while in flight (its admission size). */
 int edg_batch_problem_bytes(const edg_batch *b, const edg_problem *p, unsigned long long *bytes);
 /* Queue a problem; *id receives its handle.  EDG_ERR_ADMISSION when it can

@@ -228,3 +228,31 @@ def test_admission_errors(edgpu, oracle):
         b.submit(problem(edgpu, 5, 2.0, 1, nup=7, ndn=3))
     assert ei.value.code == 3
     b.close()
+
+
+def test_schedule_classes_and_autotune(edgpu, oracle):
+    """Every tile shape of the per-size-class schedule gives the same results
+    (to the rounding of the tile's reduction order), and the autotuner
+    returns a valid schedule covering every class."""
+    from edgpu import autotune
+    omega = oracle.matsubara(64, 100.0)
+    probs = [problem(edgpu, 5, 2.0, 81, maxnstep=40, maxnstep_green=30),
+             problem(edgpu, 9, 2.0, 82, maxnstep=40, maxnstep_green=30)]
+    ref, _, _ = run_batch(edgpu, probs, omega)
+    for sched in ([(64, 1024, 0)], [(64, 4096, 0)], [(6, 512, 0), (64, 2048, 1)]):
+        b = edgpu.Batch(omega, precision="fp64")
+        b.set_schedule(sched)
+        ids = [b.submit(p) for p in probs]
+        b.run()
+        got = {r["id"]: r for r in b.poll()}
+        b.close()
+        for i, r0 in zip(ids, ref):
+            assert abs(got[i]["e0"] - r0["e0"]) < 1e-10, sched
+            assert np.abs(got[i]["G"] - r0["G"]).max() < 1e-8, sched
+    b = edgpu.Batch(omega, precision="fp32")
+    with pytest.raises(edgpu.EdgpuError):
+        b.set_schedule([(10, 2048, 0), (8, 2048, 0)])
+    b.close()
+    sched, timings = autotune.tune_batch(edgpu, probs, omega, sample=2, steps=(10, 8))
+    assert [e[0] for e in sched] == [6, 64]
+    assert all(e[1] in autotune.BATCH_TILES for e in sched)

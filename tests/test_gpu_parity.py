@@ -378,3 +378,23 @@ def test_schedules_same_solve(edgpu, oracle, precision, sched):
     tol_e, tol_g = (1e-10, 1e-9) if precision == "fp64" else (1e-6, 1e-5)
     assert abs(g.val - g0.val) < tol_e
     assert np.abs(G - G0).max() < tol_g
+
+
+def test_autotune_single_schedule(edgpu, oracle, tmp_path):
+    """tune_single times every schedule, installs the fastest and the cache
+    returns the same choice without measuring again."""
+    from edgpu import autotune
+    rng = np.random.default_rng(91)
+    em, ek, vmk = rand_params(rng, 1, 9)
+    s, m = make(edgpu, oracle, 1, 9, precision="fp32")
+    best, times = autotune.tune_single(s, em, ek, vmk, 5, 5, reps=3)
+    assert best in autotune.SINGLE_CANDIDATES and set(times) == set(autotune.SINGLE_CANDIDATES)
+    path = str(tmp_path / "tune.json")
+    b1, _ = autotune.cached_single(s, em, ek, vmk, 5, 5, path)
+    b2, _ = autotune.cached_single(s, em, ek, vmk, 5, 5, path)
+    assert b1 == b2
+    b = oracle.Basis(10, 5, 5)
+    x = rng.normal(size=b.ntot).astype(np.float32).astype(np.float64)
+    y = s.hx(em, ek, vmk, 5, 5, x)
+    ref = oracle.hx(m, em, ek, vmk, b, x)
+    assert np.abs(y - ref).max() <= 2e-6 * np.abs(ref).max()
