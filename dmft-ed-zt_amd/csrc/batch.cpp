@@ -254,7 +254,7 @@ size_t problem_bytes(const edg_batch *b, const edg_problem &p)
     bytes += state_bytes(p.maxnstep, chain_hvn(b, ns, nu, nd), chain_axn(D));
     const int mg = green_len(p);
     bytes += 64 * 8 + 8 * (8 * (size_t)std::max(mg, p.maxnstep) + 64) + 8 * 4 * (size_t)b->nw;
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 4 && !p.gs_only; ++k) {
         const int ispin = k < 2 ? 1 : 2, pm = (k & 1) ? 2 : 1;
         const int cu = p.nup + (ispin == 1 ? (pm == 1 ? 1 : -1) : 0);
         const int cd = p.ndn + (ispin == 2 ? (pm == 1 ? 1 : -1) : 0);
@@ -273,6 +273,7 @@ int validate(const edg_batch *b, const edg_problem &p)
     if (!p.ek || !p.vmk) return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "ek / vmk missing");
     if (p.nup < 0 || p.ndn < 0 || p.nup > ns || p.ndn > ns)
         return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "sector outside 0..nsite");
+    if (p.gs_only != 0 && p.gs_only != 1) return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "gs_only must be 0 or 1");
     if (p.maxnstep < 1 || p.maxnstep > 1000 || green_len(p) > 1000)
         return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "Lanczos lengths must be 1..1000");
     for (int s = 0; s < 2; ++s) {
@@ -548,6 +549,7 @@ int after_gs(edg_batch *b, Job *J)
         C.ispin = k < 2 ? 1 : 2;
         C.pm = (k & 1) ? 2 : 1;
         C.exists = false;
+        if (p.gs_only) continue;
         if (C.ispin == 2 && !spin2) continue;
         const int nu = p.nup + (C.ispin == 1 ? (C.pm == 1 ? 1 : -1) : 0);
         const int nd = p.ndn + (C.ispin == 2 ? (C.pm == 1 ? 1 : -1) : 0);

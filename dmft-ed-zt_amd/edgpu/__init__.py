@@ -44,7 +44,7 @@ class _Problem(C.Structure):
                 ("em", C.c_double * 2), ("ek", C.POINTER(C.c_double)),
                 ("vmk", C.POINTER(C.c_double)), ("nup", C.c_int), ("ndn", C.c_int),
                 ("maxnstep", C.c_int), ("maxnstep_green", C.c_int), ("both_spins", C.c_int),
-                ("seed", C.c_ulonglong)]
+                ("seed", C.c_ulonglong), ("gs_only", C.c_int)]
 
 
 EXPORTS = ["edg_default_params", "edg_create", "edg_destroy", "edg_last_error",
@@ -331,7 +331,7 @@ class Problem:
     nbath sites with ek(nbath,2), vmk(1,nbath,2), em(1,2), U, mu, sector."""
 
     def __init__(self, nbath, U, mu, em, ek, vmk, nup, ndn, maxnstep=200, maxnstep_green=0,
-                 both_spins=False, seed=1):
+                 both_spins=False, seed=1, gs_only=False):
         self.nbath, self.U, self.mu = int(nbath), float(U), float(mu)
         self.em = _fortran(em, (1, 2))
         self.ek = _fortran(ek, (self.nbath, 2))
@@ -339,6 +339,7 @@ class Problem:
         self.nup, self.ndn = int(nup), int(ndn)
         self.maxnstep, self.maxnstep_green = int(maxnstep), int(maxnstep_green)
         self.both_spins, self.seed = bool(both_spins), int(seed)
+        self.gs_only = bool(gs_only)
 
     def _struct(self):
         p = _Problem()
@@ -348,6 +349,7 @@ class Problem:
         p.nup, p.ndn = self.nup, self.ndn
         p.maxnstep, p.maxnstep_green = self.maxnstep, self.maxnstep_green
         p.both_spins, p.seed = int(self.both_spins), self.seed
+        p.gs_only = int(self.gs_only)
         return p
 
 

@@ -192,6 +192,7 @@ typedef struct {
     int maxnstep_green;  /* steps of each c+/c chain (0: maxnstep) */
     int both_spins;      /* also run the spin-down chains when spin symmetry would skip them */
     unsigned long long seed;
+    int gs_only;         /* 1: ground state only (E0, no chains, G = 0), e.g. for a sector sweep */
 } edg_problem;
 
 /* precision: EDG_FP64 / EDG_FP32 vector storage; nspin 1 or 2 (G_cl spins);

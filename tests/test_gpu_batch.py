@@ -256,3 +256,27 @@ def test_schedule_classes_and_autotune(edgpu, oracle):
     sched, timings = autotune.tune_batch(edgpu, probs, omega, sample=2, steps=(10, 8))
     assert [e[0] for e in sched] == [6, 64]
     assert all(e[1] in autotune.BATCH_TILES for e in sched)
+
+
+def test_ground_state_sweep_matches_single_path(edgpu, oracle):
+    """ground_state.f90:30-73 sweep with every candidate sector in one batch
+    (ground state only): the same winning sector and E0 as the
+    single-problem sweep (same per-sector seeds), FP64."""
+    from edgpu import dmft
+    rng = np.random.default_rng(95)
+    e = rng.uniform(-1, 1, 5)
+    v = rng.uniform(0.2, 0.6, 5)
+    em = np.zeros((1, 2))
+    ek = np.stack([e, e], axis=1)
+    vmk = np.stack([v, v], axis=1)[None, :, :]
+    sectors = [(3, 3), (2, 3), (3, 4), (4, 2)]
+    omega = oracle.matsubara(16, 10.0)
+    isec, e0, per = dmft.ground_state_sweep(edgpu, omega, 5, 2.0, 1.0, em, ek, vmk, sectors,
+                                             maxnstep=60, seed=7, precision="fp64")
+    s = edgpu.Solver(norb=1, nbath=5, U=2.0, mu=1.0, maxnstep=60, seed=7)
+    gs = s.ground_state(em, ek, vmk, sectors)
+    s.close()
+    assert isec == gs.isector and abs(e0 - gs.val) < 1e-10
+    m = oracle.Model(1, 5, U=2.0, mu=1.0)
+    ogs = oracle.ground_state(m, em, ek, vmk, sectors, 60, seed=7)
+    assert ogs.isector == isec and abs(ogs.val - e0) < 1e-9
