@@ -32,6 +32,7 @@ FLAGS = [
 FORTRAN = os.path.join(PKG, "fortran")
 FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
 EXAMPLE = os.path.join(FORTRAN, "example_solve")
+EXAMPLE_BATCH = os.path.join(FORTRAN, "example_batch")
 
 
 def build_fortran(verbose=False):
@@ -39,10 +40,10 @@ def build_fortran(verbose=False):
     if not os.path.exists(FLANG):
         return None
     mod = os.path.join(FORTRAN, "edgpu_c.f90")
-    cmds = [[FLANG, "-c", mod, "-module-dir", FORTRAN, "-o", os.path.join(FORTRAN, "edgpu_c.o")],
-            [FLANG, "-I", FORTRAN, os.path.join(FORTRAN, "example_solve.f90"),
-             os.path.join(FORTRAN, "edgpu_c.o"), "-L", HERE, "-ledgpu",
-             "-Wl,-rpath,$ORIGIN/../edgpu", "-o", EXAMPLE]]
+    cmds = [[FLANG, "-c", mod, "-module-dir", FORTRAN, "-o", os.path.join(FORTRAN, "edgpu_c.o")]]
+    for src, exe in (("example_solve.f90", EXAMPLE), ("example_batch.f90", EXAMPLE_BATCH)):
+        cmds.append([FLANG, "-I", FORTRAN, os.path.join(FORTRAN, src), os.path.join(FORTRAN, "edgpu_c.o"),
+                     "-L", HERE, "-ledgpu", "-Wl,-rpath,$ORIGIN/../edgpu", "-o", exe])
     for cmd in cmds:
         if verbose:
             print(" ".join(cmd), flush=True)

@@ -10,6 +10,9 @@ dmft_green.f90), for callers that drive the self-consistency from Python:
   spectral_function          A(w) on the real axis from the chain coefficients,
                              taking the c+ chain from (ap, bp) and the c chain
                              from (am, bm)
+  self_energy, write_sigma   Sigma(i w) of dump_sigma and its sigma.dat.<m>.<s>
+                             files (post_process.f90:128-185)
+  quasiparticle_weight       Z = 1 / (1 - Im Sigma(i w_1) / w_1) (post_process.f90:40-56)
   ground_state_sweep         the sector sweep of ground_state.f90:30-73 with
                              every candidate sector in one batched solve
 
@@ -135,3 +138,39 @@ def ground_state_sweep(edgpu, omega, nbath, U, mu, em, ek, vmk, sectors, maxnste
         if v < best:
             best, bi = v, i + 1
     return bi, best, e
+
+
+def self_energy(em, D_cl, G_cl, omega, mu):
+    """Sigma(nw, norb, nspin) = i w + mu - em - D_cl - 1/G_cl (dump_sigma)."""
+    em = np.asarray(em, dtype=np.float64)
+    iw = 1j * np.asarray(omega, dtype=np.float64)
+    G_cl = np.asarray(G_cl, dtype=np.complex128)
+    nw, norb, nspin = G_cl.shape
+    S = np.zeros_like(G_cl)
+    for s in range(nspin):
+        for m in range(norb):
+            S[:, m, s] = iw + mu - em[m, s] - D_cl[:, m, s] - 1.0 / G_cl[:, m, s]
+    return S
+
+
+def quasiparticle_weight(Sigma, omega):
+    """Z(norb, nspin) from the first Matsubara point."""
+    zq = np.asarray(Sigma)[0].imag / float(np.asarray(omega)[0])
+    return 1.0 / (1.0 - zq)
+
+
+def write_sigma(omega, Sigma, directory="."):
+    """sigma.dat.<iorb>.<ispin>: header `nw iorb ispin`, then per frequency
+    w, Re Sigma, Im Sigma in F12.8 fields."""
+    import os
+    nw, norb, nspin = Sigma.shape
+    paths = []
+    for s in range(nspin):
+        for m in range(norb):
+            path = os.path.join(directory, f"sigma.dat.{m + 1}.{s + 1}")
+            with open(path, "w") as f:
+                f.write(f" {nw:11d} {m + 1:11d} {s + 1:11d}\n")
+                for iw in range(nw):
+                    f.write(f"{omega[iw]:12.8f}{Sigma[iw, m, s].real:12.8f}{Sigma[iw, m, s].imag:12.8f}\n")
+            paths.append(path)
+    return paths

@@ -19,6 +19,18 @@ module edgpu_c
         integer(c_long_long) :: seed
     end type edg_params
 
+    ! mirrors struct edg_problem (batched solver): ek, vmk are C pointers
+    ! (c_loc of the caller's ek(nbath,2), vmk(1,nbath,2))
+    type, bind(C) :: edg_problem
+        integer(c_int) :: nbath
+        real(c_double) :: U, mu
+        real(c_double) :: em(2)
+        type(c_ptr) :: ek, vmk
+        integer(c_int) :: nup, ndn, maxnstep, maxnstep_green, both_spins
+        integer(c_long_long) :: seed
+        integer(c_int) :: gs_only
+    end type edg_problem
+
     interface
         subroutine edg_default_params(p) bind(C, name="edg_default_params")
             import :: edg_params
@@ -120,6 +132,51 @@ module edgpu_c
             integer(c_int), value :: ne_up, ne_dn, pm, iorb, ispin
             real(c_double), intent(in) :: vec(*)
             real(c_double), intent(out) :: vec_out(*)
+        end function
+
+        ! ---- batched solver and request loop (edg_batch_* of include/edgpu.h)
+        integer(c_int) function edg_batch_create(device, precision, nspin, nw, omega, budget, b) &
+                bind(C, name="edg_batch_create")
+            import :: c_int, c_ptr, c_double, c_long_long
+            integer(c_int), value :: device, precision, nspin, nw
+            real(c_double), intent(in) :: omega(*)
+            integer(c_long_long), value :: budget
+            type(c_ptr), intent(out) :: b
+        end function
+
+        integer(c_int) function edg_batch_destroy(b) bind(C, name="edg_batch_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: b
+        end function
+
+        integer(c_int) function edg_batch_submit(b, p, id) bind(C, name="edg_batch_submit")
+            import :: c_int, c_ptr, c_long_long, edg_problem
+            type(c_ptr), value :: b
+            type(edg_problem), intent(in) :: p
+            integer(c_long_long), intent(out) :: id
+        end function
+
+        integer(c_int) function edg_batch_step(b, max_steps, pending) bind(C, name="edg_batch_step")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: b
+            integer(c_int), value :: max_steps
+            integer(c_int), intent(out) :: pending
+        end function
+
+        integer(c_int) function edg_batch_run(b) bind(C, name="edg_batch_run")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: b
+        end function
+
+        ! returns 1 when a finished problem was popped, 0 when none is ready
+        integer(c_int) function edg_batch_poll(b, id, e0, G, nstep, coeffs) bind(C, name="edg_batch_poll")
+            import :: c_int, c_ptr, c_long_long, c_double, c_double_complex
+            type(c_ptr), value :: b
+            integer(c_long_long), intent(out) :: id
+            real(c_double), intent(out) :: e0
+            complex(c_double_complex), intent(out) :: G(*)
+            integer(c_int), intent(out) :: nstep
+            real(c_double), intent(out) :: coeffs(*)
         end function
     end interface
 

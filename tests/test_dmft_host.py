@@ -10,6 +10,9 @@ construction rather than by restating its formula.
                              header, record fields) and a write/read round trip
   spectral_function          against the Lorentzian-broadened Lehmann sum of
                              the dense construction (exact chains, Ns = 4)
+  self_energy, Z             Sigma = 0 and Z = 1 for the U = 0 impurity (dense
+                             Lehmann G); Sigma -> U n_(-s) at high frequency
+                             for U > 0 (the Hartree limit); sigma.dat layout
 """
 import os
 
@@ -115,3 +118,34 @@ def test_spectral_function_vs_lehmann(oracle, dense):
     assert abs(gs.val - E0) < 1e-10
     assert np.abs(A - ref).max() < 1e-7
     assert abs(np.trapezoid(A, w) - (bp[0] ** 2 + bm[0] ** 2)) < 2e-2
+
+
+def test_self_energy_limits(oracle, dense, tmp_path):
+    em, ek, vmk = bath(7, 3)
+    em[:] = 0.0
+    ek[:, 1] = ek[:, 0]
+    vmk[:, :, 1] = vmk[:, :, 0]
+    om = oracle.matsubara(400, 2.0)
+    mu = 0.3
+    D = dmft.cluster_hybridization_ftn(ek, vmk, om, nspin=1)
+    # U = 0: Fermi sea over all sectors -> Sigma = 0, Z = 1
+    best = min(((np.linalg.eigvalsh(dense.dense_h(1, 3, em, ek, vmk, 0, 0, 0, 0, mu, nu, nd)[0])[0], nu, nd)
+                for nu in range(5) for nd in range(5)), key=lambda t: t[0])
+    G = dense.lehmann_green(1, 3, em, ek, vmk, 0, 0, 0, 0, mu, best[1], best[2], 1, 1, om)[0]
+    S = dmft.self_energy(em, D, G[:, None, None], om, mu)
+    assert np.abs(S).max() < 1e-9
+    assert abs(dmft.quasiparticle_weight(S, om)[0, 0] - 1.0) < 1e-9
+    # U > 0, sector (2,2): Sigma(i w) -> U <n_dn> as w -> infinity
+    U = 2.0
+    H0, st0 = dense.dense_h(1, 3, em, ek, vmk, U, 0, 0, 0, U / 2, 2, 2)
+    w0, v0 = np.linalg.eigh(H0)
+    n_dn = sum(abs(v0[i, 0]) ** 2 for i, s in enumerate(st0) if (s >> 4) & 1)
+    G = dense.lehmann_green(1, 3, em, ek, vmk, U, 0, 0, 0, U / 2, 2, 2, 1, 1, om)[0]
+    S = dmft.self_energy(em, D, G[:, None, None], om, U / 2)
+    assert abs(S[-1, 0, 0].real - U * n_dn) < 2e-3
+    assert 0.0 < dmft.quasiparticle_weight(S, om)[0, 0] < 1.0
+    paths = dmft.write_sigma(om[:3], S[:3], str(tmp_path))
+    lines = open(paths[0]).read().splitlines()
+    assert paths[0].endswith("sigma.dat.1.1")
+    assert [int(t) for t in lines[0].split()] == [3, 1, 1] and len(lines) == 4
+    assert len(lines[1]) == 36 and abs(float(lines[1][12:24]) - S[0, 0, 0].real) < 1e-8

@@ -35,3 +35,37 @@ def test_fortran_caller_solves_config1():
     assert abs(vals["n_up"][0] - float(g["n_up"])) < 1e-10
     assert abs(vals["docc"][0] - float(g["docc"])) < 1e-10
     assert abs(complex(*vals["G0"]) - g["G"][0]) < 1e-9
+
+
+@pytest.mark.gpu
+def test_fortran_batch_caller_vs_oracle():
+    """Three problems solved in one batch from Fortran (edg_batch_* through
+    iso_c_binding): E0 and G(i w_0) against the oracle, FP64 vectors."""
+    from oracle import oracle
+    oracle.build()
+    exe = os.path.join(FDIR, "example_batch")
+    if not os.path.exists(exe):
+        pytest.fail("example_batch not built: run __graft_entry__.build()")
+    rng = np.random.default_rng(12)
+    probs, inp = [], ""
+    for k in range(3):
+        U = 1.0 + k
+        e = rng.uniform(-1, 1, 5)
+        v = rng.uniform(0.2, 0.6, 5)
+        probs.append((U, e, v))
+        inp += f"{U:.17e}\n" + " ".join(f"{x:.17e}" for x in e) + "\n" + " ".join(f"{x:.17e}" for x in v) + "\n"
+    out = subprocess.run([exe], input=inp, capture_output=True, text=True, timeout=120, check=True).stdout
+    rows = [line.split() for line in out.splitlines() if line.startswith("P ")]
+    assert len(rows) == 3
+    om = oracle.matsubara(64, 100.0)
+    for row in rows:
+        k = int(row[1]) - 1
+        U, e, v = probs[k]
+        em = np.zeros((1, 2))
+        ek = np.stack([e, e], axis=1)
+        vmk = np.stack([v, v], axis=1)[None, :, :]
+        m = oracle.Model(1, 5, U=U, mu=U / 2)
+        gs = oracle.ground_state(m, em, ek, vmk, [(3, 3)], 80, seed=21 + k)
+        G, _ = oracle.cluster_green_ftn(m, em, ek, vmk, gs, 80, om)
+        assert abs(float(row[2]) - gs.val) < 1e-9
+        assert abs(complex(float(row[3]), float(row[4])) - G[0, 0, 0]) < 1e-8
