@@ -2,9 +2,9 @@
  * edgpu.h -- C ABI of the MI355X exact-diagonalization impurity solver.
  *
  * Plain extern "C" entry points with pointers and sizes only.  A Fortran
- * caller binds them through iso_c_binding; the Fortran module in
- * dmft-ed-zt_amd/fortran/edgpu_mod.f90 re-exposes them with the reference's
- * argument lists.  Each entry point names the reference routine it replaces
+ * caller binds them through iso_c_binding (dmft-ed-zt_amd/fortran/edgpu_c.f90
+ * declares every entry point with bind(C)); dmft-ed-zt_amd/fortran/
+ * edgpu_dropin.f90 re-exposes them with the reference's argument lists.  Each entry point names the reference routine it replaces
  * (paths relative to the yw-choi/DMFT-ED-ZT tree).
  *
  * Array conventions (identical to the reference):
