@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "../../include/edgpu.h"
+#include "devmem.h"
 #include "edgpu_internal.h"
 
 namespace edg {
@@ -40,6 +41,7 @@ namespace {
 constexpr long long kPadB = 64;       // element alignment of vectors
 constexpr int kPartialCap = 1 << 17;  // doubles of the shared partial-sum scratch
 constexpr size_t kLdsMax = 152 * 1024;
+constexpr int kClasses = 6;            // H.v launch classes, chain_class()
 
 long long rup(long long n, long long m) { return (n + m - 1) / m * m; }
 
@@ -130,6 +132,7 @@ struct edg_batch {
     int ncu = 0;
     double *omega = nullptr;
     unsigned long long budget = 0, in_use = 0;
+    DevHeap *heap = nullptr;  // device memory of the problems in flight
     std::string err;
     std::deque<Job *> queue;
     std::vector<Job *> running;
@@ -150,9 +153,9 @@ struct edg_batch {
     // chains in launch classes {short, long columns} x {8, 16 hop slots}
     const BInst *d_inst = nullptr;
     const int *d_axoff = nullptr;
-    const int *d_hvoff[4] = {nullptr, nullptr, nullptr, nullptr};
-    int cls_start[4] = {0, 0, 0, 0}, cls_n[4] = {0, 0, 0, 0};
-    int cls_tiles[4] = {0, 0, 0, 0}, cls_lds[4] = {1, 1, 1, 1};
+    const int *d_hvoff[kClasses] = {};
+    int cls_start[kClasses] = {}, cls_n[kClasses] = {};
+    int cls_tiles[kClasses] = {}, cls_lds[kClasses] = {};
     int nax = 0;
     bool use_graphs = true;
     hipGraphExec_t gexec = nullptr;
@@ -209,11 +212,16 @@ size_t state_bytes(int maxn, int hvn, int axn)
 }
 
 // Tile shape of a chain with columns of nup rows under the engine's
-// schedule: long-column tiles (one column per 512-thread workgroup) or
-// cb = clamp(tile / nup, 1, kBatchMaxCols) columns per 256-thread workgroup.
+// schedule: long-column tiles (kBatchWideCols columns per 512-thread
+// workgroup when they fit LDS, else one) or cb = clamp(tile / nup, 1,
+// kBatchMaxCols) columns per 256-thread workgroup.
 struct Shape {
     int wide, cb;
 };
+int wide_cols(const edg_batch *b, long long nup)
+{
+    return (size_t)kBatchWideCols * nup * b->es <= kLdsMax ? kBatchWideCols : 1;
+}
 Shape chain_shape(const edg_batch *b, int nsite, long long nup)
 {
     for (const auto &e : b->sched)
@@ -221,9 +229,10 @@ Shape chain_shape(const edg_batch *b, int nsite, long long nup)
             const int wide = e.wide < 0 ? (nup >= kBatchWide) : e.wide;
             int cb = e.tile / (int)std::max<long long>(nup, 1);
             cb = std::min(std::max(cb, 1), kBatchMaxCols);
-            return {wide, wide ? 1 : cb};
+            return {wide, wide ? wide_cols(b, nup) : cb};
         }
-    return {nup >= kBatchWide ? 1 : 0, 1};
+    const int wide = nup >= kBatchWide ? 1 : 0;
+    return {wide, wide ? wide_cols(b, nup) : 1};
 }
 
 int chain_hvn(const edg_batch *b, int nsite, long long nup, long long ndn)
@@ -419,12 +428,9 @@ int admit(edg_batch *b, Job *J)
 {
     const edg_problem &p = J->p;
     void *mem = nullptr;
-    const hipError_t ae = hipMallocAsync(&mem, J->bytes, b->stream);
-    if (ae == hipErrorOutOfMemory) {
-        (void)hipGetLastError();
-        return 1;
-    }
-    if (ae != hipSuccess) return fail(b, EDG_ERR_HIP, std::string("hipMallocAsync: ") + hipGetErrorString(ae));
+    const int ae = b->heap->alloc(J->bytes, &mem);
+    if (ae == 1) return 1;
+    if (ae != 0) return fail(b, EDG_ERR_HIP, "device memory allocation failed");
     J->ar.base = (char *)mem;
     J->ar.used = 0;
     J->ar.cap = J->bytes;
@@ -654,7 +660,7 @@ int finish(edg_batch *b, Job *J)
             J->G[2 * (size_t)nw + 2 * iw + 1] = i2;
         }
     }
-    BCHK(b, hipFreeAsync(J->ar.base, b->stream));
+    b->heap->free(J->ar.base);  // later users are ordered on b->stream
     J->ar.base = nullptr;
     b->in_use -= J->bytes;
     b->pinned_free.push_back(J->host);
@@ -697,38 +703,50 @@ int progress(edg_batch *b)
     return 0;
 }
 
-int chain_class(const BInst &I) { return (I.wide ? 2 : 0) + (I.op.up.KC == 16 ? 1 : 0); }
+// Launch class of a chain: short columns, long columns two per tile, long
+// columns one per tile (x 8 / 16 hop slots).
+int chain_class(const BInst &I)
+{
+    return (I.wide ? (I.cb > 1 ? 2 : 4) : 0) + (I.op.up.KC == 16 ? 1 : 0);
+}
+bool class_wide(int c) { return c >= 2; }
+int class_cols(int c) { return c >= 4 ? 1 : (c >= 2 ? kBatchWideCols : 0); }
 
-// Upload the descriptor table of the active chains grouped by launch class
-// (short / long columns x 8 / 16 hop slots), with the tile prefix sums of
-// each class and the axpy prefix sums over all chains.
+// Upload the descriptor table of the active chains grouped by launch class,
+// with the tile prefix sums of each class and the axpy prefix sums over all
+// chains.  In the long-column classes every chain's tiles start at a
+// multiple of 8 (kXcds): workgroup b runs on XCD b % 8, so tile t of a chain
+// runs on XCD t % 8 and the kernel can deal the chain's column groups to the
+// XCDs in contiguous ranges.  The padding tiles exit at once.
 int rebuild(edg_batch *b)
 {
     std::vector<Chain *> order;
-    int start[5] = {0, 0, 0, 0, 0};
-    for (int c = 0; c < 4; ++c) {
+    int start[kClasses + 1] = {};
+    for (int c = 0; c < kClasses; ++c) {
         start[c] = (int)order.size();
         for (Chain *C : b->active)
             if (chain_class(C->bi) == c) order.push_back(C);
     }
-    start[4] = (int)order.size();
-    const int n = start[4];
-    std::vector<int> hv[4], ax(n + 1);
-    int lds[4] = {1, 1, 1, 1};
+    start[kClasses] = (int)order.size();
+    const int n = start[kClasses];
+    std::vector<int> hv[kClasses], ax(n + 1);
+    int lds[kClasses];
     ax[0] = 0;
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < kClasses; ++c) {
+        lds[c] = 1;
         hv[c].assign(start[c + 1] - start[c] + 1, 0);
         for (int i = start[c]; i < start[c + 1]; ++i) {
             const BInst &I = order[i]->bi;
-            hv[c][i - start[c] + 1] = hv[c][i - start[c]] + I.hvn;
+            const int tiles = class_wide(c) ? (I.hvn + 7) / 8 * 8 : I.hvn;
+            hv[c][i - start[c] + 1] = hv[c][i - start[c]] + tiles;
             lds[c] = std::max(lds[c], I.cb * I.op.nup);
         }
     }
     for (int i = 0; i < n; ++i) ax[i + 1] = ax[i] + order[i]->bi.axn;
     auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
-    size_t off[4];
+    size_t off[kClasses];
     size_t total = al(sizeof(BInst) * std::max(n, 1));
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < kClasses; ++c) {
         off[c] = total;
         total += al(hv[c].size() * 4);
     }
@@ -748,7 +766,7 @@ int rebuild(edg_batch *b)
     }
     char *h = b->hdesc[f];
     for (int i = 0; i < n; ++i) std::memcpy(h + sizeof(BInst) * i, &order[i]->bi, sizeof(BInst));
-    for (int c = 0; c < 4; ++c) std::memcpy(h + off[c], hv[c].data(), 4 * hv[c].size());
+    for (int c = 0; c < kClasses; ++c) std::memcpy(h + off[c], hv[c].data(), 4 * hv[c].size());
     std::memcpy(h + off_ax, ax.data(), 4 * (size_t)(n + 1));
     BCHK(b, hipMemcpyAsync(b->ddesc, h, total, hipMemcpyHostToDevice, b->stream));
     BCHK(b, hipEventRecord(b->hdesc_ev[f], b->stream));
@@ -756,7 +774,7 @@ int rebuild(edg_batch *b)
     b->flip ^= 1;
     b->d_inst = (const BInst *)b->ddesc;
     b->d_axoff = (const int *)(b->ddesc + off_ax);
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < kClasses; ++c) {
         b->d_hvoff[c] = (const int *)(b->ddesc + off[c]);
         b->cls_start[c] = start[c];
         b->cls_n[c] = start[c + 1] - start[c];
@@ -789,9 +807,9 @@ int rebuild(edg_batch *b)
 int enqueue_step(edg_batch *b)
 {
     const int n = (int)b->active.size();
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < kClasses; ++c) {
         if (!b->cls_n[c]) continue;
-        if (launch_batch_hv(b->stream, b->fp64, c >= 2, (c & 1) ? 16 : 8, b->d_inst + b->cls_start[c],
+        if (launch_batch_hv(b->stream, b->fp64, class_cols(c), (c & 1) ? 16 : 8, b->d_inst + b->cls_start[c],
                             b->d_hvoff[c], b->cls_n[c], b->cls_tiles[c], b->cls_lds[c]))
             return fail(b, EDG_ERR_HIP, "batched H.v launch failed");
     }
@@ -873,17 +891,14 @@ int edg_batch_create(int device, int precision, int nspin, int nw, const double 
     chk(hipEventCreateWithFlags(&b->hdesc_ev[0], hipEventDisableTiming));
     chk(hipEventCreateWithFlags(&b->hdesc_ev[1], hipEventDisableTiming));
     if (!rc) chk(hipMemcpy(b->omega, omega, sizeof(double) * nw, hipMemcpyHostToDevice));
-    if (!rc) {  // keep freed problem memory in the stream-ordered pool for the next admission
-        hipMemPool_t pool;
-        chk(hipDeviceGetDefaultMemPool(&pool, device));
-        uint64_t keep = ~0ull;
-        if (!rc) chk(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
-    }
+    b->heap = new DevHeap(device);
     if (!rc && batch_set_lds(kLdsMax)) rc = EDG_ERR_HIP;
     if (!rc && budget_bytes == 0) {
+        // free device memory plus the idle chunks earlier queues left in the
+        // device cache (hipMemGetInfo counts those as used)
         size_t fr = 0, tot = 0;
         chk(hipMemGetInfo(&fr, &tot));
-        budget_bytes = (unsigned long long)(0.85 * (double)fr);
+        budget_bytes = (unsigned long long)(0.85 * (double)(fr + devmem_cached_bytes(device)));
     }
     if (rc) {
         (void)hipGetLastError();
@@ -901,10 +916,7 @@ int edg_batch_destroy(edg_batch *b)
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     auto drop = [&](Job *J) {
-        if (J->ar.base) {
-            (void)hipFreeAsync(J->ar.base, b->stream);
-            (void)hipStreamSynchronize(b->stream);
-        }
+        if (J->ar.base && b->heap) b->heap->free(J->ar.base);
         if (J->host) (void)hipHostFree(J->host);
         if (J->ev) (void)hipEventDestroy(J->ev);
         delete J;
@@ -918,7 +930,9 @@ int edg_batch_destroy(edg_batch *b)
         (void)hipGraphExecDestroy(r.first);
         (void)hipEventDestroy(r.second);
     }
-    if (b->ddesc) (void)hipFree(b->ddesc);
+    if (b->ddesc && b->stream) (void)hipFreeAsync(b->ddesc, b->stream);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    delete b->heap;  // its chunks go to the device cache for later queues
     for (int f = 0; f < 2; ++f) {
         if (b->hdesc[f]) (void)hipHostFree(b->hdesc[f]);
         if (b->hdesc_ev[f]) (void)hipEventDestroy(b->hdesc_ev[f]);
@@ -928,6 +942,14 @@ int edg_batch_destroy(edg_batch *b)
     if (b->ticket) (void)hipFree(b->ticket);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
+    return EDG_OK;
+}
+
+int edg_batch_release_memory(int device)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return EDG_ERR_ARG;
+    devmem_release_cache(device);
     return EDG_OK;
 }
 

@@ -1,8 +1,7 @@
 // Batched Lanczos steps over many independent chains (uneven sector sizes)
-// for the single-orbital impurity (norb == 1).  One launch of k_bhv (chains
-// with short columns, several columns per tile) and one of k_bhvw (long
-// columns, one column per tile) run the H.v half of step j_i for every
-// chain i, one launch of k_baxpy the axpy
+// for the single-orbital impurity (norb == 1).  Launches of k_bhv (chains
+// with short columns, several columns per tile) and of k_bhvw (long columns,
+// the single-chain tile) run the H.v half of step j_i for every chain i, one launch of k_baxpy the axpy
 // half, one launch of k_badvance moves every chain to its next step; each
 // chain keeps its own step index, alpha/beta arrays and partial sums, so
 // chains at different steps (admitted at different times) share the
@@ -13,7 +12,7 @@
 // with beta_{j+1} = |y| and s_{j+1} = 1/beta_{j+1} taken by the next H.v.
 // A chain's reductions depend only on its own tiles (fixed count and
 // order), so its result is the same whatever else is in the batch.
-#include "device_util.h"
+#include "hv1_tile.h"
 
 namespace edg {
 
@@ -41,7 +40,7 @@ __device__ __forceinline__ T *slot_of(const BInst &I, int k)
 // from global memory.  Hop records are the 32-bit compressed tables
 // (target rank | value index << 16, KC slots per row, self-padded).
 template <typename T, typename ACC, int KS>
-__global__ __launch_bounds__(kBlock) void k_bhv(const BInst *inst, const int *hv_off, int ninst)
+__global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, const int *__restrict__ hv_off, int ninst)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     T *xs = reinterpret_cast<T *>(smem);
@@ -54,19 +53,19 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *inst, const int *hv
     const BInst &I = inst[i];
     if (I.flags[0]) return;
     const int t = blockIdx.x - hv_off[i];
-    const HvOp &op = I.op;
+    const HvOp op = I.op;  // private copy: stores to y cannot alias it
     const int nup = op.nup, ndn = op.ndn, cb = I.cb;
     const int c0 = t * cb;
     const int ncol = ndn - c0 < cb ? ndn - c0 : cb;
     constexpr int kcd = KS;
-    const int j = *I.step;
-    const T *x = slot_of<T>(I, j);
-    T *y = slot_of<T>(I, j + 1);
-    const T *pv = j > 0 ? slot_of<T>(I, j - 1) : nullptr;
+    const int j = uniform(*I.step);
+    const T *x = uniform(slot_of<T>(I, j));
+    T *y = uniform(slot_of<T>(I, j + 1));
+    const T *pv = j > 0 ? uniform(slot_of<T>(I, j - 1)) : nullptr;
 
     for (int e = threadIdx.x; e <= kMaxHops; e += kBlock) {
-        const ACC vu = e < op.nvup ? (ACC)op.up.vsrc[e] : ACC(0);
-        const ACC vd = e < op.nvup ? (ACC)op.dn.vsrc[e] : ACC(0);
+        const ACC vu = e < op.nvup ? (ACC)gload(op.up.vsrc + e) : ACC(0);
+        const ACC vd = e < op.nvup ? (ACC)gload(op.dn.vsrc + e) : ACC(0);
         s_vu[e] = vu;
         s_vu[kMaxHops + 1 + e] = -vu;
         s_vd[e] = vd;
@@ -74,11 +73,11 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *inst, const int *hv
     }
     const int nel = ncol * nup;
     const T *xt = x + (long long)c0 * nup;
-    for (int e = threadIdx.x; e < nel; e += kBlock) xs[e] = xt[e];
-    for (int e = threadIdx.x; e < ncol * kcd; e += kBlock) s_drec[e] = op.dn.hrec[(long long)c0 * kcd + e];
+    for (int e = threadIdx.x; e < nel; e += kBlock) xs[e] = gload(xt + e);
+    for (int e = threadIdx.x; e < ncol * kcd; e += kBlock) s_drec[e] = gload(op.dn.hrec + (long long)c0 * kcd + e);
     for (int c = threadIdx.x; c < ncol; c += kBlock) {
         s_ed[c] = energy_of<ACC>(op.dn, c0 + c);
-        s_md[c] = op.dn.mask[c0 + c];
+        s_md[c] = gload(op.dn.mask + c0 + c);
     }
     __syncthreads();
 
@@ -102,71 +101,48 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *inst, const int *hv
         else if ((c + 1) * nup <= e) ++c;
         const int iup = e - c * nup;
         const ACC xv = (ACC)xs[e];
-        ACC acc = (energy_of<ACC>(op.up, iup) + s_ed[c] + (op.up.mask[iup] && s_md[c] ? U : ACC(0))) * xv;
-        const Row<uint32_t, KS> ur = *reinterpret_cast<const Row<uint32_t, KS> *>(op.up.hrec + (long long)iup * KS);
+        ACC acc = (energy_of<ACC>(op.up, iup) + s_ed[c] + (gload(op.up.mask + iup) && s_md[c] ? U : ACC(0))) * xv;
+        const Row<uint32_t, KS> ur = gload(reinterpret_cast<const Row<uint32_t, KS> *>(op.up.hrec + (long long)iup * KS));
         const T *xc = xs + c * nup;
         const uint32_t *dr = s_drec + c * KS;
         T gn[KS];
 #pragma unroll
-        for (int k = 0; k < KS; ++k) gn[k] = x[(long long)(dr[k] & 0xFFFFu) * nup + iup];
+        for (int k = 0; k < KS; ++k) gn[k] = gload(x + (long long)(dr[k] & 0xFFFFu) * nup + iup);
 #pragma unroll
         for (int k = 0; k < KS; ++k) acc += s_vu[ur.v[k] >> 16] * (ACC)xc[ur.v[k] & 0xFFFFu];
 #pragma unroll
         for (int k = 0; k < KS; ++k) acc += s_vd[dr[k] >> 16] * (ACC)gn[k];
         const long long g = (long long)c0 * nup + e;
         double r = sx * (double)acc;
-        if (pv) r -= bsp * (double)pv[g];
+        if (pv) r -= bsp * (double)gload(pv + g);
         const T rt = (T)r;
-        y[g] = rt;
+        gstore(y + g, rt);
         dot += (double)xv * (double)rt;
     }
     double mine[1] = {dot};
     block_partial_at<1>(mine, I.hvpart, t);
 }
 
-// H.v tile for chains with long sector columns: one column idn = t per
-// 512-thread workgroup (as in the single-chain kernel).  The column's
-// down-hop targets and elements are the same for every row, so they are
-// decoded once into uniform registers and each down hop is one coalesced
-// load of a neighbour column; up hops gather from the staged column.
-template <typename T, typename ACC, int KS>
-__global__ __launch_bounds__(kHvBlock) void k_bhvw(const BInst *inst, const int *hv_off, int ninst)
+// H.v tile for chains with long sector columns: the single-chain tile
+// (hv1_tile: staged columns for the up hops, wave-uniform neighbour-column
+// loads for the down hops) over C columns per 512-thread workgroup.  Every
+// chain's tiles start at a multiple of 8 in the launch, so tile t runs on
+// XCD t % 8 and xcd_group deals the chain's column groups to the XCDs in
+// contiguous ranges, as in the single-chain kernel; tiles past the chain's
+// hvn are padding and exit at once.
+template <typename T, int C, int KS>
+__global__ __launch_bounds__(kHvBlock) void k_bhvw(const BInst *__restrict__ inst, const int *__restrict__ hv_off, int ninst)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
-    T *xs = reinterpret_cast<T *>(smem);
-    __shared__ ACC s_vu[2 * (kMaxHops + 1)];
-
     const int i = owner(hv_off, ninst, blockIdx.x);
     const BInst &I = inst[i];
-    if (I.flags[0]) return;
     const int t = blockIdx.x - hv_off[i];
-    const HvOp &op = I.op;
-    const int nup = op.nup, idn = t;
-    const int j = *I.step;
-    const T *x = slot_of<T>(I, j);
-    T *y = slot_of<T>(I, j + 1);
-    const T *pv = j > 0 ? slot_of<T>(I, j - 1) : nullptr;
-    const T *xc = x + (long long)idn * nup;
-
-    for (int e = threadIdx.x; e <= kMaxHops; e += kHvBlock) {
-        const ACC vu = e < op.nvup ? (ACC)op.up.vsrc[e] : ACC(0);
-        s_vu[e] = vu;
-        s_vu[kMaxHops + 1 + e] = -vu;
-    }
-    for (int i0 = threadIdx.x; i0 < nup; i0 += 8 * kHvBlock) {
-        T v[8];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const int e = i0 + r * kHvBlock;
-            v[r] = xc[e < nup ? e : nup - 1];
-        }
-#pragma unroll
-        for (int r = 0; r < 8; ++r)
-            if (i0 + r * kHvBlock < nup) xs[i0 + r * kHvBlock] = v[r];
-    }
-    __syncthreads();
-
+    if (I.flags[0] || t >= I.hvn) return;
+    const int j = uniform(*I.step);
     HvArgs la{};
+    la.x = uniform(slot_of<T>(I, j));
+    la.y = uniform(slot_of<T>(I, j + 1));
+    la.p = j > 0 ? uniform(slot_of<T>(I, j - 1)) : nullptr;
+    la.partial = I.hvpart;
     la.step = j;
     la.prev_part = I.axpart;
     la.prev_n = I.axn;
@@ -174,68 +150,32 @@ __global__ __launch_bounds__(kHvBlock) void k_bhvw(const BInst *inst, const int 
     la.beta = I.beta;
     la.stop = I.flags;
     la.nstep = I.flags + 1;
-    double sx, bsp;
-    if (!lanczos_scales<kHvBlock>(la, sx, bsp, t == 0)) return;
-
-    // down hops of this column (uniform): byte offsets and elements
-    uint32_t coff[KS];
-    ACC dval[KS];
-    const uint32_t *dr = op.dn.hrec + (long long)idn * KS;
-#pragma unroll
-    for (int k = 0; k < KS; ++k) {
-        const uint32_t w = dr[k];
-        const uint32_t vi = w >> 16;
-        const int l = vi > (uint32_t)kMaxHops ? (int)vi - (kMaxHops + 1) : (int)vi;
-        const ACC v = l < op.nvup ? (ACC)op.dn.vsrc[l] : ACC(0);
-        dval[k] = vi > (uint32_t)kMaxHops ? -v : v;
-        coff[k] = (w & 0xFFFFu) * (uint32_t)nup * (uint32_t)sizeof(T);
-    }
-    const ACC ed = energy_of<ACC>(op.dn, idn);
-    const ACC edU = op.dn.mask[idn] ? ed + (ACC)op.U : ed;
-    const unsigned char *xb = reinterpret_cast<const unsigned char *>(x);
-    double dot = 0.0;
-    for (int iup = threadIdx.x; iup < nup; iup += kHvBlock) {
-        const uint32_t ib = (uint32_t)iup * (uint32_t)sizeof(T);
-        T g[KS];
-#pragma unroll
-        for (int k = 0; k < KS; ++k) g[k] = *reinterpret_cast<const T *>(xb + (coff[k] + ib));
-        const Row<uint32_t, KS> ur = *reinterpret_cast<const Row<uint32_t, KS> *>(op.up.hrec + (long long)iup * KS);
-        const ACC xv = (ACC)xs[iup];
-        ACC acc = (energy_of<ACC>(op.up, iup) + (op.up.mask[iup] ? edU : ed)) * xv;
-#pragma unroll
-        for (int k = 0; k < KS; ++k) acc += s_vu[ur.v[k] >> 16] * (ACC)xs[ur.v[k] & 0xFFFFu];
-#pragma unroll
-        for (int k = 0; k < KS; ++k) acc += dval[k] * (ACC)g[k];
-        const long long gi = (long long)idn * nup + iup;
-        double r = sx * (double)acc;
-        if (pv) r -= bsp * (double)pv[gi];
-        const T rt = (T)r;
-        y[gi] = rt;
-        dot += (double)xv * (double)rt;
-    }
-    double mine[1] = {dot};
-    block_partial_at<1, kHvBlock>(mine, I.hvpart, t);
+    // a private copy: the tile's stores to y cannot alias it, so its fields
+    // stay in registers across the row loop
+    const HvOp op = I.op;
+    hv1_tile<T, T, C, HV_LANCZOS, KS>(op, la, xcd_group(t, I.hvn) * C, t, t == 0);
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_baxpy(const BInst *inst, const int *ax_off, int ninst)
+__global__ __launch_bounds__(kBlock) void k_baxpy(const BInst *__restrict__ inst, const int *__restrict__ ax_off, int ninst)
 {
     const int i = owner(ax_off, ninst, blockIdx.x);
     const BInst &I = inst[i];
     if (I.flags[0]) return;
     const int t = blockIdx.x - ax_off[i];
-    const int j = *I.step;
+    const int j = uniform(*I.step);
     const double sj = I.scale[j];
     const double al = sj * sum_partials<kBlock>(I.hvpart, I.hvn);
     if (t == 0 && threadIdx.x == 0) I.alpha[j] = al;
     const double coef = al * sj;
-    T *r = slot_of<T>(I, j + 1);
-    const T *u = slot_of<T>(I, j);
+    T *r = uniform(slot_of<T>(I, j + 1));
+    const T *u = uniform(slot_of<T>(I, j));
     const long long n = (long long)I.op.nup * I.op.ndn;
+    const long long stride = (long long)I.axn * kBlock;
     double v[1] = {0.0};
-    for (long long e = (long long)t * kBlock + threadIdx.x; e < n; e += (long long)I.axn * kBlock) {
-        const T w = (T)((double)r[e] - coef * (double)u[e]);
-        r[e] = w;
+    for (long long e = (long long)t * kBlock + threadIdx.x; e < n; e += stride) {
+        const T w = (T)((double)gload(r + e) - coef * (double)gload(u + e));
+        gstore(r + e, w);
         v[0] += (double)w * (double)w;
     }
     block_partial_at<1>(v, I.axpart, t);
@@ -250,8 +190,10 @@ __global__ void k_badvance(const BInst *inst, int ninst)
 template <int KS>
 static int set_lds_ks(size_t max_lds)
 {
-    const void *fs[4] = {(const void *)k_bhv<float, float, KS>, (const void *)k_bhv<double, double, KS>,
-                         (const void *)k_bhvw<float, float, KS>, (const void *)k_bhvw<double, double, KS>};
+    const void *fs[6] = {(const void *)k_bhv<float, float, KS>,     (const void *)k_bhv<double, double, KS>,
+                         (const void *)k_bhvw<float, 1, KS>,        (const void *)k_bhvw<double, 1, KS>,
+                         (const void *)k_bhvw<float, kBatchWideCols, KS>,
+                         (const void *)k_bhvw<double, kBatchWideCols, KS>};
     for (const void *f : fs)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds) != hipSuccess)
             return -1;
@@ -260,15 +202,22 @@ static int set_lds_ks(size_t max_lds)
 
 int batch_set_lds(size_t max_lds) { return set_lds_ks<8>(max_lds) || set_lds_ks<16>(max_lds) ? -1 : 0; }
 
-int launch_batch_hv(hipStream_t s, bool fp64, bool wide, int ks, const BInst *inst, const int *hv_off,
+template <int C>
+static void (*wide_kernel(bool fp64, int ks))(const BInst *, const int *, int)
+{
+    if (ks == 8) return fp64 ? k_bhvw<double, C, 8> : k_bhvw<float, C, 8>;
+    return fp64 ? k_bhvw<double, C, 16> : k_bhvw<float, C, 16>;
+}
+
+int launch_batch_hv(hipStream_t s, bool fp64, int wide_cols, int ks, const BInst *inst, const int *hv_off,
                     int ninst, int ntiles, int lds_elems)
 {
     if (ntiles <= 0) return 0;
     const size_t lds = (size_t)lds_elems * (fp64 ? 8 : 4);
+    const bool wide = wide_cols > 0;
     void (*f)(const BInst *, const int *, int);
     if (wide) {
-        if (ks == 8) f = fp64 ? k_bhvw<double, double, 8> : k_bhvw<float, float, 8>;
-        else f = fp64 ? k_bhvw<double, double, 16> : k_bhvw<float, float, 16>;
+        f = wide_cols == 1 ? wide_kernel<1>(fp64, ks) : wide_kernel<kBatchWideCols>(fp64, ks);
     } else {
         if (ks == 8) f = fp64 ? k_bhv<double, double, 8> : k_bhv<float, float, 8>;
         else f = fp64 ? k_bhv<double, double, 16> : k_bhv<float, float, 16>;

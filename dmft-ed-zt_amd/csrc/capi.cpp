@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/edgpu.h"
+#include "devmem.h"
 #include "edgpu_internal.h"
 
 namespace edg {
@@ -102,6 +103,21 @@ int fail(edg_ctx *c, int code, const std::string &msg)
             return fail(c, EDG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// hipMalloc that, when the device is full, first returns the idle memory
+// chunks closed batched queues left on it and tries again.
+bool dev_malloc(void **p, size_t bytes, int device)
+{
+    *p = nullptr;
+    if (hipMalloc(p, bytes) == hipSuccess) return true;
+    (void)hipGetLastError();
+    *p = nullptr;
+    devmem_release_cache(device);
+    if (hipMalloc(p, bytes) == hipSuccess) return true;
+    (void)hipGetLastError();
+    *p = nullptr;
+    return false;
+}
+
 int ensure(edg_ctx *c, DevBuf &b, size_t bytes)
 {
     if (b.bytes >= bytes) return 0;
@@ -111,8 +127,7 @@ int ensure(edg_ctx *c, DevBuf &b, size_t bytes)
         b.p = nullptr;
         b.bytes = 0;
     }
-    if (hipMalloc(&b.p, bytes) != hipSuccess) {
-        (void)hipGetLastError();
+    if (!dev_malloc(&b.p, bytes, c->device)) {
         return fail(c, EDG_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
     }
     b.bytes = bytes;
@@ -167,10 +182,7 @@ int get_tab(edg_ctx *c, int spin, int ne, const SpinTab **out)
         t.n = (int)binom(ns, ne);
         t.K = norb * nbath > kMinSlots ? norb * nbath : kMinSlots;
         auto al = [&](void **p, size_t bytes) -> int {
-            if (hipMalloc(p, bytes < 16 ? 16 : bytes) != hipSuccess) {
-                (void)hipGetLastError();
-                return fail(c, EDG_ERR_NOMEM, "table allocation failed");
-            }
+            if (!dev_malloc(p, bytes < 16 ? 16 : bytes, c->device)) return fail(c, EDG_ERR_NOMEM, "table allocation failed");
             te.allocs.push_back(*p);
             return 0;
         };
@@ -205,10 +217,7 @@ int get_tab(edg_ctx *c, int spin, int ne, const SpinTab **out)
             te.t.KC = km <= 8 ? 8 : 16;
             if (!te.t.ccol) {
                 auto al2 = [&](void **p, size_t bytes) -> int {
-                    if (hipMalloc(p, bytes) != hipSuccess) {
-                        (void)hipGetLastError();
-                        return fail(c, EDG_ERR_NOMEM, "table allocation failed");
-                    }
+                    if (!dev_malloc(p, bytes, c->device)) return fail(c, EDG_ERR_NOMEM, "table allocation failed");
                     te.allocs.push_back(*p);
                     return 0;
                 };
@@ -706,11 +715,11 @@ int edg_create(const edg_params *p, int device, edg_ctx **out)
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bad();
     if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return bad();
-    if (hipMalloc((void **)&c->partial, sizeof(double) * 16 * 8192 + 64) != hipSuccess) return bad();
-    if (hipMalloc((void **)&c->ticket, 256) != hipSuccess) return bad();
+    if (!dev_malloc((void **)&c->partial, sizeof(double) * 16 * 8192 + 64, device)) return bad();
+    if (!dev_malloc((void **)&c->ticket, 256, device)) return bad();
     if (hipMemset(c->ticket, 0, 256) != hipSuccess) return bad();
     const int mx = (p->maxnstep > p->maxnstep_green ? p->maxnstep : p->maxnstep_green) + 8;
-    if (hipMalloc(&c->scal_mem, sizeof(double) * (3 * mx + 40) + 64) != hipSuccess) return bad();
+    if (!dev_malloc(&c->scal_mem, sizeof(double) * (3 * mx + 40) + 64, device)) return bad();
     double *s = (double *)c->scal_mem;
     c->sc.alpha = s;
     c->sc.beta = s + mx;

@@ -107,15 +107,92 @@ struct alignas(sizeof(T) * C) Row {
     T v[C];
 };
 
+// Loads and stores through the global address space.  Pointers that a
+// kernel reads out of a descriptor in memory (rather than receiving as
+// arguments) are generic to the compiler, which then emits flat accesses
+// with 64-bit vector addresses; these keep them global.
+// Aggregates (hop records, rows) are loaded as 32-bit vectors: clang copies
+// an aggregate through a generic pointer, which would undo the cast.
+template <int A>
+struct U32x2;
+template <>
+struct U32x2<4> {
+    typedef uint32_t type __attribute__((ext_vector_type(2), aligned(4)));
+};
+template <>
+struct U32x2<8> {
+    typedef uint32_t type __attribute__((ext_vector_type(2)));
+};
+template <typename T>
+__device__ __forceinline__ T gload(const T *p)
+{
+    if constexpr (__is_scalar(T)) {
+        return *(const __attribute__((address_space(1))) T *)p;
+    } else {
+        static_assert(sizeof(T) % 8 == 0 && alignof(T) >= 4, "gload: 8-byte multiple aggregates");
+        using u2 = typename U32x2<(alignof(T) < 8 ? alignof(T) : 8)>::type;
+        constexpr int n = (int)(sizeof(T) / 8);
+        const __attribute__((address_space(1))) u2 *q = (const __attribute__((address_space(1))) u2 *)p;
+        u2 w[n];
+#pragma unroll
+        for (int i = 0; i < n; ++i) w[i] = q[i];
+        T r;
+        __builtin_memcpy(&r, w, sizeof(T));
+        return r;
+    }
+}
+
+// A value or pointer the whole wave holds, moved to scalar registers.
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+template <typename T>
+__device__ __forceinline__ T *uniform(T *p)
+{
+    const unsigned long long a = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    return (T *)(((unsigned long long)hi << 32) | lo);
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T *p, const T &v)
+{
+    *(__attribute__((address_space(1))) T *)p = v;
+}
+
 __host__ __device__ constexpr size_t align16(size_t n) { return (n + 15) & ~size_t(15); }
 
 
 template <typename ACC>
 __device__ __forceinline__ ACC energy_of(const SpinTab &t, int r);
 template <>
-__device__ __forceinline__ float energy_of<float>(const SpinTab &t, int r) { return t.energyf[r]; }
+__device__ __forceinline__ float energy_of<float>(const SpinTab &t, int r) { return gload(t.energyf + r); }
 template <>
-__device__ __forceinline__ double energy_of<double>(const SpinTab &t, int r) { return t.energy[r]; }
+__device__ __forceinline__ double energy_of<double>(const SpinTab &t, int r) { return gload(t.energy + r); }
 
+// XCD-aware group order.  Workgroups of one launch are dispatched round-robin
+// over the 8 XCDs (workgroup b runs on XCD b % 8), each with its own 4 MiB L2.
+// Giving XCD x a contiguous range of column groups keeps neighbour columns of
+// the down hops in that XCD's L2: in increasing-pattern (colex) order the high
+// bath bits change slowly, so a contiguous rank range is closed under most
+// hops.  Bijective for any ngroups.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_group(int b, int ngroups)
+{
+    const int q = ngroups / kXcds, r = ngroups - q * kXcds;
+    const int x = b % kXcds, i = b / kXcds;
+    return x * q + (x < r ? x : r) + i;
+}
+
+template <typename ACC>
+struct HopOf;
+template <>
+struct HopOf<float> {
+    using type = HopF;
+    static __device__ __forceinline__ const HopF *tab(const SpinTab &t) { return t.hf; }
+};
+template <>
+struct HopOf<double> {
+    using type = HopD;
+    static __device__ __forceinline__ const HopD *tab(const SpinTab &t) { return t.hd; }
+};
 
 }  // namespace edg

@@ -156,10 +156,11 @@ int launch_cf(hipStream_t s, int nw, const double *omega, double e0, int np, con
 // hv_off / ax_off: prefix sums (ninst + 1 entries) of the chains' H.v tiles
 // and axpy workgroups; the launches cover hv_off[ninst] / ax_off[ninst]
 // workgroups.  lds_elems: largest staged elements per tile over the chains.
-// wide: long-column chains (nup >= kBatchWide), one column per 512-thread
-// tile; else chain_cb columns per 256-thread tile.  ks: hop slots per row of
-// every table of the launched chains (8 or 16).
-int launch_batch_hv(hipStream_t s, bool fp64, bool wide, int ks, const BInst *inst, const int *hv_off,
+// wide_cols: 0 for short-column chains (BInst::cb columns per 256-thread
+// tile); 1 or kBatchWideCols for long-column chains (that many columns per
+// 512-thread tile, tile offsets of every chain a multiple of 8).  ks: hop
+// slots per row of every table of the launched chains (8 or 16).
+int launch_batch_hv(hipStream_t s, bool fp64, int wide_cols, int ks, const BInst *inst, const int *hv_off,
                     int ninst, int ntiles, int lds_elems);
 int launch_batch_axpy(hipStream_t s, bool fp64, const BInst *inst, const int *ax_off, int ninst,
                       int nblocks);
@@ -172,6 +173,7 @@ int batch_set_lds(size_t max_lds);
 constexpr int kBatchTile = 2048;
 constexpr int kBatchMaxCols = 64;
 constexpr int kBatchWide = 1024;
+constexpr int kBatchWideCols = 2;  // columns per long-column tile when they fit LDS
 
 // Number of per-block partial slots the reductions may need.
 long long max_blocks_needed(long long n);

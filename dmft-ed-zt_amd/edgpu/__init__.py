@@ -55,7 +55,7 @@ EXPORTS = ["edg_default_params", "edg_create", "edg_destroy", "edg_last_error",
            "edg_batch_create", "edg_batch_destroy", "edg_batch_last_error",
            "edg_batch_set_graphs", "edg_batch_problem_bytes", "edg_batch_submit",
            "edg_batch_step", "edg_batch_run", "edg_batch_poll", "edg_batch_stats",
-           "edg_batch_set_schedule"]
+           "edg_batch_set_schedule", "edg_batch_release_memory"]
 
 _lib = None
 
@@ -107,6 +107,7 @@ def load_library(path=LIB_PATH):
     L.edg_batch_poll.argtypes = [bh, C.POINTER(C.c_longlong), dp, dp, ip, dp]
     L.edg_batch_stats.argtypes = [bh, dp]
     L.edg_batch_set_schedule.argtypes = [bh, C.c_int, ip, ip, ip]
+    L.edg_batch_release_memory.argtypes = [C.c_int]
     for name in EXPORTS:
         if name not in ("edg_default_params", "edg_last_error", "edg_batch_last_error"):
             getattr(L, name).restype = C.c_int
@@ -351,6 +352,14 @@ class Problem:
         p.both_spins, p.seed = int(self.both_spins), self.seed
         p.gs_only = int(self.gs_only)
         return p
+
+
+def release_batch_memory(device=0):
+    """Return to the device the idle memory chunks that closed Batch queues
+    left for later queues (edg_batch_release_memory)."""
+    rc = load_library().edg_batch_release_memory(int(device))
+    if rc != 0:
+        raise EdgpuError(rc, "edg_batch_release_memory failed")
 
 
 class Batch:

@@ -149,6 +149,11 @@ module edgpu_c
             type(c_ptr), value :: b
         end function
 
+        integer(c_int) function edg_batch_release_memory(device) bind(C, name="edg_batch_release_memory")
+            import :: c_int
+            integer(c_int), value :: device
+        end function
+
         integer(c_int) function edg_batch_submit(b, p, id) bind(C, name="edg_batch_submit")
             import :: c_int, c_ptr, c_long_long, edg_problem
             type(c_ptr), value :: b

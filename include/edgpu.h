@@ -197,11 +197,18 @@ typedef struct {
 
 /* precision: EDG_FP64 / EDG_FP32 vector storage; nspin 1 or 2 (G_cl spins);
  * omega: the nw Matsubara frequencies; budget_bytes: device bytes the
- * problems in flight may hold (0 = 85% of the free device memory); a problem
- * is also held back while the device cannot allocate it. */
+ * problems in flight may hold (0 = 85% of the free device memory, counting
+ * as free the idle chunks of earlier queues); a problem is also held back
+ * while the device cannot allocate it.  Problem memory is sub-allocated
+ * from large device chunks that are not unmapped while the process runs;
+ * edg_batch_destroy leaves a queue's chunks to later queues, and
+ * edg_batch_release_memory returns the idle ones to the device. */
 int edg_batch_create(int device, int precision, int nspin, int nw, const double *omega,
                      unsigned long long budget_bytes, edg_batch **out);
 int edg_batch_destroy(edg_batch *b);
+/* Return to the device the idle memory chunks that destroyed queues left
+ * on it (no counterpart in the reference). */
+int edg_batch_release_memory(int device);
 const char *edg_batch_last_error(const edg_batch *b);
 /* Replay the batched step as a captured HIP graph (1, default) or launch
  * its kernels one by one (0). */

@@ -125,13 +125,16 @@ def test_batch_matches_single_problem_path(edgpu, oracle):
 
 
 def test_long_columns_with_short_ones(edgpu, oracle):
-    """Ns = 13 (nup = C(13,6) = 1716, long-column tiles) in the same batched
-    steps as Ns = 6 and 8 problems (short-column tiles); E0 and G against the
-    single-problem path, FP64 vectors."""
+    """Ns = 13 (nup = C(13,6) = 1716, long-column tiles of two columns) in the
+    same batched steps as Ns = 6 and 8 problems (short-column tiles); E0 and
+    G against the single-problem path, FP64 vectors."""
     omega = oracle.matsubara(128, 100.0)
     big = problem(edgpu, 12, 2.0, 71, nup=6, ndn=6, maxnstep=60, maxnstep_green=40)
+    # ndn = C(13,5) = 1287 columns: an odd column count (the last two-column
+    # tile has one dead column) and a tile count that is not a multiple of 8
+    odd = problem(edgpu, 12, 3.0, 74, nup=6, ndn=5, maxnstep=60, maxnstep_green=40)
     probs = [problem(edgpu, 5, 1.5, 72, maxnstep=60, maxnstep_green=40), big,
-             problem(edgpu, 7, 2.5, 73, maxnstep=60, maxnstep_green=40)]
+             problem(edgpu, 7, 2.5, 73, maxnstep=60, maxnstep_green=40), odd]
     res, _, _ = run_batch(edgpu, probs, omega)
     for pr, r in zip(probs, res):
         s = edgpu.Solver(norb=1, nbath=pr.nbath, U=pr.U, mu=pr.mu, maxnstep=60,
@@ -216,6 +219,43 @@ def test_nspin2_and_both_spins(edgpu, oracle):
     res2, _, _ = run_batch(edgpu, [pr2], omega)
     e0, G = oracle_solve(oracle, pr2, omega)
     assert np.abs(res2[0]["G"][:, 0] - G[:, 0]).max() < 1e-8
+
+
+def test_budget_stable_across_queues(edgpu, oracle):
+    """A closed queue leaves its device memory chunks to later queues of the
+    process; their default budget (85% of free device memory) counts those
+    idle chunks as free, so it does not shrink from queue to queue, and the
+    next queue's problems reuse the chunks with the same results.  After
+    release_batch_memory the chunks are back with the device."""
+    omega = oracle.matsubara(16, 10.0)
+    probs = [problem(edgpu, 11, 2.0, 90 + k, maxnstep=200, maxnstep_green=20) for k in range(8)]
+
+    def run(b):
+        ids = [b.submit(p) for p in probs]
+        peak, pending = 0, len(ids)
+        while pending:
+            pending = b.step(32)
+            peak = max(peak, b.stats()["bytes_in_use"])
+        res = {r["id"]: r for r in b.poll()}
+        return [res[i]["e0"] for i in ids], peak
+
+    b0 = edgpu.Batch(omega, precision="fp32")
+    budget0 = b0.stats()["budget"]
+    e0, peak = run(b0)
+    b0.close()
+    assert peak > 1e9
+    b1 = edgpu.Batch(omega, precision="fp32")
+    budget1 = b1.stats()["budget"]
+    e1, _ = run(b1)
+    b1.close()
+    edgpu.release_batch_memory(0)
+    b2 = edgpu.Batch(omega, precision="fp32")
+    budget2 = b2.stats()["budget"]
+    b2.close()
+    print(f"budget {budget0:.4e} {budget1:.4e} {budget2:.4e}, peak in use {peak:.3e}", flush=True)
+    assert e1 == e0
+    assert abs(budget1 - budget0) < 0.05 * 0.85 * peak + 64e6
+    assert abs(budget2 - budget0) < 0.05 * 0.85 * peak + 64e6
 
 
 def test_admission_errors(edgpu, oracle):
