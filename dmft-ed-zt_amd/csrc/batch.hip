@@ -12,7 +12,7 @@
 // with beta_{j+1} = |y| and s_{j+1} = 1/beta_{j+1} taken by the next H.v.
 // A chain's reductions depend only on its own tiles (fixed count and
 // order), so its result is the same whatever else is in the batch.
-#include "hv1_tile.h"
+#include "device_util.h"
 
 namespace edg {
 
@@ -124,13 +124,13 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, 
 }
 
 // H.v tile for chains with long sector columns: the single-chain tile
-// (hv1_tile: staged columns for the up hops, wave-uniform neighbour-column
+// (hv1_body.inc: staged columns for the up hops, wave-uniform neighbour-column
 // loads for the down hops) over C columns per 512-thread workgroup.  Every
 // chain's tiles start at a multiple of 8 in the launch, so tile t runs on
 // XCD t % 8 and xcd_group deals the chain's column groups to the XCDs in
 // contiguous ranges, as in the single-chain kernel; tiles past the chain's
 // hvn are padding and exit at once.
-template <typename T, int C, int KS>
+template <typename T, int C, int SLOTS>
 __global__ __launch_bounds__(kHvBlock) void k_bhvw(const BInst *__restrict__ inst, const int *__restrict__ hv_off, int ninst)
 {
     const int i = owner(hv_off, ninst, blockIdx.x);
@@ -153,7 +153,13 @@ __global__ __launch_bounds__(kHvBlock) void k_bhvw(const BInst *__restrict__ ins
     // a private copy: the tile's stores to y cannot alias it, so its fields
     // stay in registers across the row loop
     const HvOp op = I.op;
-    hv1_tile<T, T, C, HV_LANCZOS, KS>(op, la, xcd_group(t, I.hvn) * C, t, t == 0);
+    const HvArgs &a = la;
+    using ACC = T;
+    constexpr int MODE = HV_LANCZOS, KH = SLOTS;
+    const int idn0 = xcd_group(t, I.hvn) * C;
+    const int slot = t;
+    const bool lead = t == 0;
+#include "hv1_body.inc"
 }
 
 template <typename T>

@@ -12,7 +12,7 @@
 // C columns, while down-hops read whole neighbour columns from global memory
 // with coalesced wave64 loads.  Reductions are FP64 and deterministic: each
 // block writes a partial, and the consuming kernel sums them in a fixed order.
-#include "hv1_tile.h"
+#include "device_util.h"
 
 namespace edg {
 
@@ -422,7 +422,9 @@ __global__ __launch_bounds__(kHvBlock) void k_hv1(HvOp op, HvArgs a)
 {
     if (a.stop && *a.stop) return;
     const int idn0 = xcd_group(blockIdx.x, (op.ndn + C - 1) / C) * C;
-    hv1_tile<T, ACC, C, MODE, KH>(op, a, idn0, blockIdx.x, blockIdx.x == 0);
+    const int slot = blockIdx.x;
+    const bool lead = blockIdx.x == 0;
+#include "hv1_body.inc"
 }
 
 // Two-orbital (Kanamori) H.v, one sector column per 512-thread workgroup.
