@@ -124,7 +124,16 @@ struct Job {
 
 struct edg_batch {
     int device = 0;
+    // stream: the batched Lanczos steps (and the reads of finished chains);
+    // side: per-problem stage work (tables, start vectors, combine, RQ,
+    // apply_c, continued fractions), which then overlaps the steps.  The
+    // steps wait on side_ev, recorded after stage work that started chains;
+    // the host moves a problem to its next stage only after an event shows
+    // the previous stage's work (on either stream) is done.
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;
+    hipEvent_t side_ev = nullptr;
+    bool side_pending = false;
     bool fp64 = false;
     size_t es = 4;
     int nspin = 1;
@@ -343,10 +352,10 @@ int build_table(edg_batch *b, Job *J, int spin, int ne, SpinTab &t, int kc)
     const double *ek_s = J->dparams + 2 + (size_t)(spin - 1) * nbath;
     const double *v_s = J->dparams + 2 + 2 * (size_t)nbath + (size_t)(spin - 1) * nbath;
     t.vsrc = v_s;
-    BCHK(b, hipMemsetAsync(t.maxcnt, 0, sizeof(int), b->stream));
-    if (launch_spin_table(b->stream, ns, 1, nbath, ne, em_s, ek_s, v_s, J->p.mu, 0.0, t))
+    BCHK(b, hipMemsetAsync(t.maxcnt, 0, sizeof(int), b->side));
+    if (launch_spin_table(b->side, ns, 1, nbath, ne, em_s, ek_s, v_s, J->p.mu, 0.0, t))
         return fail(b, EDG_ERR_HIP, "spin table launch failed");
-    if (launch_compress(b->stream, t)) return fail(b, EDG_ERR_HIP, "compress launch failed");
+    if (launch_compress(b->side, t)) return fail(b, EDG_ERR_HIP, "compress launch failed");
     return 0;
 }
 
@@ -395,7 +404,7 @@ int init_chain(edg_batch *b, Job *J, Chain &C, const HvOp &op, int maxn, bool st
     I.hvpart = I.scale + maxn + 1;
     I.axpart = I.hvpart + I.hvn;
     if (J->ar.overflow) return fail(b, EDG_ERR_NOMEM, "problem arena overflow (chain)");
-    BCHK(b, hipMemsetAsync(st, 0, state_bytes(maxn, I.hvn, I.axn), b->stream));
+    BCHK(b, hipMemsetAsync(st, 0, state_bytes(maxn, I.hvn, I.axn), b->side));
     C.D = D;
     C.exists = true;
     C.launched = 0;
@@ -411,15 +420,16 @@ int start_chain(edg_batch *b, Job *J, Chain &C, double *norm_out, int slot)
     int *h = (int *)(J->host + J->host_n - 8) + 2 * slot;
     h[0] = 0;
     h[1] = I.maxn;
-    BCHK(b, hipMemcpyAsync(I.flags, h, 2 * sizeof(int), hipMemcpyHostToDevice, b->stream));
-    int rc = b->fp64 ? launch_norm<double>(b->stream, (const double *)I.base, C.D, b->partial, b->ticket,
+    BCHK(b, hipMemcpyAsync(I.flags, h, 2 * sizeof(int), hipMemcpyHostToDevice, b->side));
+    int rc = b->fp64 ? launch_norm<double>(b->side, (const double *)I.base, C.D, b->partial, b->ticket,
                                            norm_out, I.scale, nullptr)
-                     : launch_norm<float>(b->stream, (const float *)I.base, C.D, b->partial, b->ticket,
+                     : launch_norm<float>(b->side, (const float *)I.base, C.D, b->partial, b->ticket,
                                           norm_out, I.scale, nullptr);
     if (rc) return fail(b, EDG_ERR_HIP, "norm launch failed");
     C.active = true;
     b->active.push_back(&C);
     b->dirty = true;
+    b->side_pending = true;  // the chain's first step waits for this stage work
     return 0;
 }
 
@@ -447,7 +457,7 @@ int admit(edg_batch *b, Job *J)
     h[1] = p.em[1];
     std::memcpy(h + 2, J->ek.data(), sizeof(double) * 2 * nbath);
     std::memcpy(h + 2 + 2 * nbath, J->vmk.data(), sizeof(double) * 2 * nbath);
-    BCHK(b, hipMemcpyAsync(J->dparams, h, 8 * (2 + 4 * (size_t)nbath), hipMemcpyHostToDevice, b->stream));
+    BCHK(b, hipMemcpyAsync(J->dparams, h, 8 * (2 + 4 * (size_t)nbath), hipMemcpyHostToDevice, b->side));
     int kc = 8;
     for (int s = 0; s < 2; ++s) {
         const int n0 = s == 0 ? p.nup : p.ndn;
@@ -472,8 +482,8 @@ int admit(edg_batch *b, Job *J)
     if (J->ar.overflow) return fail(b, EDG_ERR_NOMEM, "problem arena overflow");
     if (int rc = init_chain(b, J, J->gsc, op, p.maxnstep, true)) return rc;
     const long long D = J->gsc.D;
-    const int rc0 = b->fp64 ? launch_start<double>(b->stream, (double *)J->gsc.bi.base, D, p.seed)
-                            : launch_start<float>(b->stream, (float *)J->gsc.bi.base, D, p.seed);
+    const int rc0 = b->fp64 ? launch_start<double>(b->side, (double *)J->gsc.bi.base, D, p.seed)
+                            : launch_start<float>(b->side, (float *)J->gsc.bi.base, D, p.seed);
     if (rc0) return fail(b, EDG_ERR_HIP, "start launch failed");
     if (int rc = start_chain(b, J, J->gsc, &J->misc[0], 0)) return rc;
     J->st = ST_GS;
@@ -526,26 +536,26 @@ int after_gs(edg_batch *b, Job *J)
         return fail(b, EDG_ERR_SOLVER, "tridiagonal eigensolve failed");
     double *hc = J->host + 4 * (size_t)m + 8;
     for (int j = 0; j < nst; ++j) hc[j] = z[j] * s[j];
-    BCHK(b, hipMemcpyAsync(J->coef, hc, sizeof(double) * nst, hipMemcpyHostToDevice, b->stream));
+    BCHK(b, hipMemcpyAsync(J->coef, hc, sizeof(double) * nst, hipMemcpyHostToDevice, b->side));
     const HvOp op = J->gsc.bi.op;
     const long long D = J->gsc.D;
     int rc;
     if (b->fp64) {
-        rc = launch_combine<double>(b->stream, (double *)J->gs, (const double *)J->gsc.bi.base, J->gsc.bi.stride,
+        rc = launch_combine<double>(b->side, (double *)J->gs, (const double *)J->gsc.bi.base, J->gsc.bi.stride,
                                     nst, J->coef, D, b->partial, b->ticket, &J->misc[4], &J->misc[5]);
-        rc |= launch_scale<double>(b->stream, (double *)J->gs, D, &J->misc[5]);
+        rc |= launch_scale<double>(b->side, (double *)J->gs, D, &J->misc[5]);
     } else {
-        rc = launch_combine<float>(b->stream, (float *)J->gs, (const float *)J->gsc.bi.base, J->gsc.bi.stride,
+        rc = launch_combine<float>(b->side, (float *)J->gs, (const float *)J->gsc.bi.base, J->gsc.bi.stride,
                                    nst, J->coef, D, b->partial, b->ticket, &J->misc[4], &J->misc[5]);
-        rc |= launch_scale<float>(b->stream, (float *)J->gs, D, &J->misc[5]);
+        rc |= launch_scale<float>(b->side, (float *)J->gs, D, &J->misc[5]);
     }
     if (rc) return fail(b, EDG_ERR_HIP, "combine launch failed");
     HvArgs ha{};
     ha.x = J->gs;
     ha.partial = b->partial;
     ha.result = &J->misc[1];
-    const int hrc = b->fp64 ? launch_hv<double>(b->stream, op, HV_RQ, 2, true, ha)
-                            : launch_hv<float>(b->stream, op, HV_RQ, 2, true, ha);
+    const int hrc = b->fp64 ? launch_hv<double>(b->side, op, HV_RQ, 2, true, ha)
+                            : launch_hv<float>(b->side, op, HV_RQ, 2, true, ha);
     if (hrc < 0) return fail(b, EDG_ERR_HIP, "Rayleigh-quotient launch failed");
     // c+/c start vectors (dmft_green.f90:202-257), one ring per chain
     const int ns_site = nsite_of(p), mg = green_len(p);
@@ -566,12 +576,12 @@ int after_gs(edg_batch *b, Job *J)
         J->map[k] = (uint32_t *)J->ar.take(4 * (size_t)tout.n);
         if (J->ar.overflow) return fail(b, EDG_ERR_NOMEM, "problem arena overflow (map)");
         const int ne_in = C.ispin == 1 ? p.nup : p.ndn;
-        if (launch_cmap(b->stream, tout.pattern, tout.n, ns_site, ne_in, 0, C.pm, J->map[k]))
+        if (launch_cmap(b->side, tout.pattern, tout.n, ns_site, ne_in, 0, C.pm, J->map[k]))
             return fail(b, EDG_ERR_HIP, "cmap launch failed");
         const int godd = C.ispin == 2 ? (p.nup & 1) : 0;
-        const int arc = b->fp64 ? launch_apply_c<double>(b->stream, (const double *)J->gs, (double *)C.bi.base,
+        const int arc = b->fp64 ? launch_apply_c<double>(b->side, (const double *)J->gs, (double *)C.bi.base,
                                                          cop.nup, cop.ndn, op.nup, C.ispin, J->map[k], godd)
-                                : launch_apply_c<float>(b->stream, (const float *)J->gs, (float *)C.bi.base,
+                                : launch_apply_c<float>(b->side, (const float *)J->gs, (float *)C.bi.base,
                                                         cop.nup, cop.ndn, op.nup, C.ispin, J->map[k], godd);
         if (arc) return fail(b, EDG_ERR_HIP, "apply_c launch failed");
         if (int r = start_chain(b, J, C, &J->misc[8 + k], 1 + k)) return r;
@@ -623,14 +633,14 @@ int after_chains(edg_batch *b, Job *J)
         std::memcpy(hc + (size_t)s * 4 * mg, co[s].data(), sizeof(double) * 4 * mg);
         const double *dc = J->coef + (size_t)s * 4 * mg;
         BCHK(b, hipMemcpyAsync((void *)dc, hc + (size_t)s * 4 * mg, sizeof(double) * 4 * mg, hipMemcpyHostToDevice,
-                               b->stream));
-        if (launch_cf(b->stream, b->nw, b->omega, J->e0, nsv[s][0], dc, dc + mg, nsv[s][1], dc + 2 * mg, dc + 3 * mg,
+                               b->side));
+        if (launch_cf(b->side, b->nw, b->omega, J->e0, nsv[s][0], dc, dc + mg, nsv[s][1], dc + 2 * mg, dc + 3 * mg,
                       J->gdev + (size_t)s * 2 * b->nw, 0))
             return fail(b, EDG_ERR_HIP, "continued fraction launch failed");
     }
     double *hg = J->host + 16 + 4 * (size_t)(3 * mg + 4) + 8 * (size_t)mg;
-    BCHK(b, hipMemcpyAsync(hg, J->gdev, sizeof(double) * 2 * b->nw * nspin_run, hipMemcpyDeviceToHost, b->stream));
-    BCHK(b, hipEventRecord(J->ev, b->stream));
+    BCHK(b, hipMemcpyAsync(hg, J->gdev, sizeof(double) * 2 * b->nw * nspin_run, hipMemcpyDeviceToHost, b->side));
+    BCHK(b, hipEventRecord(J->ev, b->side));
     J->st = ST_CF_READ;
     return 0;
 }
@@ -660,7 +670,9 @@ int finish(edg_batch *b, Job *J)
             J->G[2 * (size_t)nw + 2 * iw + 1] = i2;
         }
     }
-    b->heap->free(J->ar.base);  // later users are ordered on b->stream
+    // every GPU user of the region is done: the host saw J->ev after the last
+    // stage, and the chains' steps before their coefficients were read
+    b->heap->free(J->ar.base);
     J->ar.base = nullptr;
     b->in_use -= J->bytes;
     b->pinned_free.push_back(J->host);
@@ -684,6 +696,10 @@ int progress(edg_batch *b)
         } else if (J->st == ST_GS_READ && hipEventQuery(J->ev) == hipSuccess) {
             if (int rc = after_gs(b, J)) return rc;
         } else if (J->st == ST_CHAINS && !chains_running(J)) {
+            // misc holds the RQ sums and start norms written on the side stream
+            // (a ground-state-only problem started no chain that waited for them)
+            BCHK(b, hipEventRecord(b->side_ev, b->side));
+            BCHK(b, hipStreamWaitEvent(b->stream, b->side_ev, 0));
             BCHK(b, hipMemcpyAsync(J->host, J->misc, 16 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
             const int mg = green_len(J->p);
             for (int k = 0; k < 4; ++k)
@@ -836,6 +852,11 @@ int run_steps(edg_batch *b, int k)
         if (e != hipSuccess) return fail(b, EDG_ERR_HIP, "graph instantiation failed");
         b->graph_valid = true;
     }
+    if (b->side_pending) {
+        BCHK(b, hipEventRecord(b->side_ev, b->side));
+        BCHK(b, hipStreamWaitEvent(b->stream, b->side_ev, 0));
+        b->side_pending = false;
+    }
     for (int s = 0; s < k; ++s) {
         if (b->use_graphs) {
             BCHK(b, hipGraphLaunch(b->gexec, b->stream));
@@ -884,6 +905,8 @@ int edg_batch_create(int device, int precision, int nspin, int nw, const double 
         if (e != hipSuccess && !rc) rc = EDG_ERR_HIP;
     };
     chk(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    chk(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+    chk(hipEventCreateWithFlags(&b->side_ev, hipEventDisableTiming));
     chk(hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, device));
     chk(hipMalloc((void **)&b->omega, sizeof(double) * nw));
     chk(hipMalloc((void **)&b->partial, sizeof(double) * kPartialCap));
@@ -915,6 +938,7 @@ int edg_batch_destroy(edg_batch *b)
     if (!b) return EDG_OK;
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->side) (void)hipStreamSynchronize(b->side);
     auto drop = [&](Job *J) {
         if (J->ar.base && b->heap) b->heap->free(J->ar.base);
         if (J->host) (void)hipHostFree(J->host);
@@ -940,6 +964,8 @@ int edg_batch_destroy(edg_batch *b)
     if (b->omega) (void)hipFree(b->omega);
     if (b->partial) (void)hipFree(b->partial);
     if (b->ticket) (void)hipFree(b->ticket);
+    if (b->side_ev) (void)hipEventDestroy(b->side_ev);
+    if (b->side) (void)hipStreamDestroy(b->side);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return EDG_OK;

@@ -1310,28 +1310,43 @@ __device__ __forceinline__ cplx cdiv(double num, cplx d)
 
 // numeric_utils.f90:11-44: f = b1^2/(z -+ a1 - b2^2/(z -+ a2 - ...)),
 // evaluated bottom-up; the last level is applied after the loop even for n=1.
-__device__ cplx cfrac(cplx z, int n, const double *a, const double *b, double sgn)
+// a and b2 = b*b are read from LDS.
+__device__ cplx cfrac(cplx z, int n, const double *a, const double *b2, double sgn)
 {
-    cplx f = cdiv(b[n - 1] * b[n - 1], {z.re + sgn * a[n - 1], z.im});
-    for (int i = n - 2; i >= 1; --i) f = cdiv(b[i] * b[i], {z.re + sgn * a[i] - f.re, z.im - f.im});
-    return cdiv(b[0] * b[0], {z.re + sgn * a[0] - f.re, z.im - f.im});
+    cplx f = cdiv(b2[n - 1], {z.re + sgn * a[n - 1], z.im});
+    for (int i = n - 2; i >= 1; --i) f = cdiv(b2[i], {z.re + sgn * a[i] - f.re, z.im - f.im});
+    return cdiv(b2[0], {z.re + sgn * a[0] - f.re, z.im - f.im});
 }
 
-// g[iw] (+)= CF_p(E0 + i w; ap, bp) + CF_m(-E0 + i w; am, bm)
-__global__ void k_cf(int nw, const double *omega, double e0, int np, const double *ap,
-                     const double *bp, int nm, const double *am, const double *bm, double *g,
-                     int accumulate)
+// g[iw] (+)= CF_p(E0 + i w; ap, bp) + CF_m(-E0 + i w; am, bm).  One thread
+// per frequency; the recursion is a serial chain of divisions per thread, so
+// the coefficients (and b^2) are staged in LDS first: read from global memory
+// level by level, every level would wait on an L2 round trip.
+__global__ __launch_bounds__(128) void k_cf(int nw, const double *omega, double e0, int np, const double *ap,
+                                            const double *bp, int nm, const double *am, const double *bm,
+                                            double *g, int accumulate)
 {
+    extern __shared__ double cfs[];  // ap[np] | bp^2[np] | am[nm] | bm^2[nm]
+    double *sap = cfs, *sbp = cfs + np, *sam = cfs + 2 * np, *sbm = cfs + 2 * np + nm;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) {
+        sap[i] = ap[i];
+        sbp[i] = bp[i] * bp[i];
+    }
+    for (int i = threadIdx.x; i < nm; i += blockDim.x) {
+        sam[i] = am[i];
+        sbm[i] = bm[i] * bm[i];
+    }
+    __syncthreads();
     const int iw = blockIdx.x * blockDim.x + threadIdx.x;
     if (iw >= nw) return;
     double re = 0.0, im = 0.0;
     if (np > 0) {
-        const cplx f = cfrac({e0, omega[iw]}, np, ap, bp, -1.0);
+        const cplx f = cfrac({e0, omega[iw]}, np, sap, sbp, -1.0);
         re += f.re;
         im += f.im;
     }
     if (nm > 0) {
-        const cplx f = cfrac({-e0, omega[iw]}, nm, am, bm, 1.0);
+        const cplx f = cfrac({-e0, omega[iw]}, nm, sam, sbm, 1.0);
         re += f.re;
         im += f.im;
     }
@@ -1348,7 +1363,9 @@ int launch_cf(hipStream_t s, int nw, const double *omega, double e0, int np, con
               const double *bp, int nm, const double *am, const double *bm, double *g,
               int accumulate)
 {
-    hipLaunchKernelGGL(k_cf, dim3((nw + 127) / 128), dim3(128), 0, s, nw, omega, e0, np, ap, bp,
+    if (np < 0 || nm < 0 || np > 1000 || nm > 1000) return -1;  // chains are at most 1000 levels
+    const size_t lds = sizeof(double) * 2 * ((size_t)np + nm);
+    hipLaunchKernelGGL(k_cf, dim3((nw + 127) / 128), dim3(128), lds, s, nw, omega, e0, np, ap, bp,
                        nm, am, bm, g, accumulate);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
