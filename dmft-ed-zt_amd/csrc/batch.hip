@@ -37,8 +37,9 @@ __device__ __forceinline__ T *slot_of(const BInst &I, int k)
 // H.v tile: columns [c0, c0 + ncol) of chain i, every row.  The tile's
 // columns are contiguous in memory and staged in LDS; up hops gather from
 // the staged column, down hops read the neighbour column at the same row
-// from global memory.  Hop records are the 32-bit compressed tables
-// (target rank | value index << 16, KC slots per row, self-padded).
+// from global memory (coalesced over the rows).  Hop records are the 32-bit
+// compressed tables (target rank | value index << 16, KC slots per row,
+// self-padded).
 template <typename T, typename ACC, int KS>
 __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, const int *__restrict__ hv_off, int ninst)
 {
@@ -92,32 +93,41 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, 
     double sx, bsp;
     if (!lanczos_scales<kBlock>(la, sx, bsp, t == 0)) return;
 
+    // thread -> (row, column group): columns of at most kBlock rows are
+    // covered by G = kBlock / nup groups of nup threads (group g takes columns
+    // g, g + G, ...); longer columns by one group striding over the rows.
+    // A thread loads its row's hop record, energy and mask once for all the
+    // columns it handles.
     const ACC U = (ACC)op.U;
-    const float inv = 1.0f / (float)nup;
+    const int span = nup <= kBlock ? nup : kBlock;
+    const int G = kBlock / span;
+    const int g = threadIdx.x / span;
     double dot = 0.0;
-    for (int e = threadIdx.x; e < nel; e += kBlock) {
-        int c = (int)((float)e * inv);
-        if (c * nup > e) --c;
-        else if ((c + 1) * nup <= e) ++c;
-        const int iup = e - c * nup;
-        const ACC xv = (ACC)xs[e];
-        ACC acc = (energy_of<ACC>(op.up, iup) + s_ed[c] + (gload(op.up.mask + iup) && s_md[c] ? U : ACC(0))) * xv;
-        const Row<uint32_t, KS> ur = gload(reinterpret_cast<const Row<uint32_t, KS> *>(op.up.hrec + (long long)iup * KS));
-        const T *xc = xs + c * nup;
-        const uint32_t *dr = s_drec + c * KS;
-        T gn[KS];
+    if (g < G) {
+        for (int iup = threadIdx.x - g * span; iup < nup; iup += span) {
+            const Row<uint32_t, KS> ur = gload(reinterpret_cast<const Row<uint32_t, KS> *>(op.up.hrec + (long long)iup * KS));
+            const ACC eu = energy_of<ACC>(op.up, iup);
+            const bool mu = gload(op.up.mask + iup) != 0;
+            for (int c = g; c < ncol; c += G) {
+                const T *xc = xs + c * nup;
+                const uint32_t *dr = s_drec + c * KS;
+                T gn[KS];
 #pragma unroll
-        for (int k = 0; k < KS; ++k) gn[k] = gload(x + (long long)(dr[k] & 0xFFFFu) * nup + iup);
+                for (int k = 0; k < KS; ++k) gn[k] = gload(x + (long long)(dr[k] & 0xFFFFu) * nup + iup);
+                const ACC xv = (ACC)xc[iup];
+                ACC acc = (eu + s_ed[c] + (mu && s_md[c] ? U : ACC(0))) * xv;
 #pragma unroll
-        for (int k = 0; k < KS; ++k) acc += s_vu[ur.v[k] >> 16] * (ACC)xc[ur.v[k] & 0xFFFFu];
+                for (int k = 0; k < KS; ++k) acc += s_vu[ur.v[k] >> 16] * (ACC)xc[ur.v[k] & 0xFFFFu];
 #pragma unroll
-        for (int k = 0; k < KS; ++k) acc += s_vd[dr[k] >> 16] * (ACC)gn[k];
-        const long long g = (long long)c0 * nup + e;
-        double r = sx * (double)acc;
-        if (pv) r -= bsp * (double)gload(pv + g);
-        const T rt = (T)r;
-        gstore(y + g, rt);
-        dot += (double)xv * (double)rt;
+                for (int k = 0; k < KS; ++k) acc += s_vd[dr[k] >> 16] * (ACC)gn[k];
+                const long long gi = (long long)(c0 + c) * nup + iup;
+                double r = sx * (double)acc;
+                if (pv) r -= bsp * (double)gload(pv + gi);
+                const T rt = (T)r;
+                gstore(y + gi, rt);
+                dot += (double)xv * (double)rt;
+            }
+        }
     }
     double mine[1] = {dot};
     block_partial_at<1>(mine, I.hvpart, t);
@@ -176,14 +186,30 @@ __global__ __launch_bounds__(kBlock) void k_baxpy(const BInst *__restrict__ inst
     const double coef = al * sj;
     T *r = uniform(slot_of<T>(I, j + 1));
     const T *u = uniform(slot_of<T>(I, j));
-    const long long n = (long long)I.op.nup * I.op.ndn;
+    // 16-byte vectors (slots are 256-byte aligned); workgroup 0 also takes
+    // the n % V tail
+    constexpr int V = 16 / sizeof(T);
+    using Vec = Row<T, V>;
+    const long long n = (long long)I.op.nup * I.op.ndn, nv = n / V;
     const long long stride = (long long)I.axn * kBlock;
     double v[1] = {0.0};
-    for (long long e = (long long)t * kBlock + threadIdx.x; e < n; e += stride) {
-        const T w = (T)((double)gload(r + e) - coef * (double)gload(u + e));
-        gstore(r + e, w);
-        v[0] += (double)w * (double)w;
+    for (long long q = (long long)t * kBlock + threadIdx.x; q < nv; q += stride) {
+        const Vec rv = gload(reinterpret_cast<const Vec *>(r) + q);
+        const Vec uv = gload(reinterpret_cast<const Vec *>(u) + q);
+        Vec w;
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            w.v[i] = (T)((double)rv.v[i] - coef * (double)uv.v[i]);
+            v[0] += (double)w.v[i] * (double)w.v[i];
+        }
+        gstore(reinterpret_cast<Vec *>(r) + q, w);
     }
+    if (t == 0)
+        for (long long e = nv * V + threadIdx.x; e < n; e += kBlock) {
+            const T w = (T)((double)gload(r + e) - coef * (double)gload(u + e));
+            gstore(r + e, w);
+            v[0] += (double)w * (double)w;
+        }
     block_partial_at<1>(v, I.axpart, t);
 }
 

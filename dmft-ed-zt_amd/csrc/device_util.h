@@ -128,6 +128,16 @@ __device__ __forceinline__ T gload(const T *p)
 {
     if constexpr (__is_scalar(T)) {
         return *(const __attribute__((address_space(1))) T *)p;
+    } else if constexpr (sizeof(T) % 16 == 0 && alignof(T) >= 16) {
+        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+        constexpr int n = (int)(sizeof(T) / 16);
+        const __attribute__((address_space(1))) u4 *q = (const __attribute__((address_space(1))) u4 *)p;
+        u4 w[n];
+#pragma unroll
+        for (int i = 0; i < n; ++i) w[i] = q[i];
+        T r;
+        __builtin_memcpy(&r, w, sizeof(T));
+        return r;
     } else {
         static_assert(sizeof(T) % 8 == 0 && alignof(T) >= 4, "gload: 8-byte multiple aggregates");
         using u2 = typename U32x2<(alignof(T) < 8 ? alignof(T) : 8)>::type;
@@ -155,7 +165,18 @@ __device__ __forceinline__ T *uniform(T *p)
 template <typename T>
 __device__ __forceinline__ void gstore(T *p, const T &v)
 {
-    *(__attribute__((address_space(1))) T *)p = v;
+    if constexpr (__is_scalar(T)) {
+        *(__attribute__((address_space(1))) T *)p = v;
+    } else {
+        static_assert(sizeof(T) % 16 == 0 && alignof(T) >= 16, "gstore: 16-byte multiple aggregates");
+        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+        constexpr int n = (int)(sizeof(T) / 16);
+        u4 w[n];
+        __builtin_memcpy(w, &v, sizeof(T));
+        __attribute__((address_space(1))) u4 *q = (__attribute__((address_space(1))) u4 *)p;
+#pragma unroll
+        for (int i = 0; i < n; ++i) q[i] = w[i];
+    }
 }
 
 __host__ __device__ constexpr size_t align16(size_t n) { return (n + 15) & ~size_t(15); }
