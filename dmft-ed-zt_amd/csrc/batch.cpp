@@ -161,8 +161,8 @@ struct edg_batch {
     int flip = 0;
     // chains in launch classes {short, long columns} x {8, 16 hop slots}
     const BInst *d_inst = nullptr;
-    const int *d_axoff = nullptr;
-    const int *d_hvoff[kClasses] = {};
+    const int *d_axoff = nullptr, *d_axown = nullptr;
+    const int *d_hvoff[kClasses] = {}, *d_hvown[kClasses] = {};
     int cls_start[kClasses] = {}, cls_n[kClasses] = {};
     int cls_tiles[kClasses] = {}, cls_lds[kClasses] = {};
     int nax = 0;
@@ -759,15 +759,28 @@ int rebuild(edg_batch *b)
         }
     }
     for (int i = 0; i < n; ++i) ax[i + 1] = ax[i] + order[i]->bi.axn;
+    // chain of every workgroup (a table lookup instead of a search per workgroup)
+    std::vector<int> own[kClasses], axown(ax[n]);
+    for (int c = 0; c < kClasses; ++c) {
+        own[c].resize(hv[c].back());
+        for (size_t k = 0; k + 1 < hv[c].size(); ++k)
+            for (int w = hv[c][k]; w < hv[c][k + 1]; ++w) own[c][w] = (int)k;
+    }
+    for (int i = 0; i < n; ++i)
+        for (int w = ax[i]; w < ax[i + 1]; ++w) axown[w] = i;
     auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
-    size_t off[kClasses];
+    size_t off[kClasses], off_own[kClasses];
     size_t total = al(sizeof(BInst) * std::max(n, 1));
     for (int c = 0; c < kClasses; ++c) {
         off[c] = total;
         total += al(hv[c].size() * 4);
+        off_own[c] = total;
+        total += al(own[c].size() * 4);
     }
     const size_t off_ax = total;
-    total += (size_t)(n + 1) * 4;
+    total += al((size_t)(n + 1) * 4);
+    const size_t off_axown = total;
+    total += axown.size() * 4;
     const int f = b->flip;
     if (b->hdesc_used[f]) BCHK(b, hipEventSynchronize(b->hdesc_ev[f]));
     if (b->hdesc_cap[f] < total) {
@@ -782,16 +795,22 @@ int rebuild(edg_batch *b)
     }
     char *h = b->hdesc[f];
     for (int i = 0; i < n; ++i) std::memcpy(h + sizeof(BInst) * i, &order[i]->bi, sizeof(BInst));
-    for (int c = 0; c < kClasses; ++c) std::memcpy(h + off[c], hv[c].data(), 4 * hv[c].size());
+    for (int c = 0; c < kClasses; ++c) {
+        std::memcpy(h + off[c], hv[c].data(), 4 * hv[c].size());
+        std::memcpy(h + off_own[c], own[c].data(), 4 * own[c].size());
+    }
     std::memcpy(h + off_ax, ax.data(), 4 * (size_t)(n + 1));
+    std::memcpy(h + off_axown, axown.data(), 4 * axown.size());
     BCHK(b, hipMemcpyAsync(b->ddesc, h, total, hipMemcpyHostToDevice, b->stream));
     BCHK(b, hipEventRecord(b->hdesc_ev[f], b->stream));
     b->hdesc_used[f] = true;
     b->flip ^= 1;
     b->d_inst = (const BInst *)b->ddesc;
     b->d_axoff = (const int *)(b->ddesc + off_ax);
+    b->d_axown = (const int *)(b->ddesc + off_axown);
     for (int c = 0; c < kClasses; ++c) {
         b->d_hvoff[c] = (const int *)(b->ddesc + off[c]);
+        b->d_hvown[c] = (const int *)(b->ddesc + off_own[c]);
         b->cls_start[c] = start[c];
         b->cls_n[c] = start[c + 1] - start[c];
         b->cls_tiles[c] = hv[c].back();
@@ -826,10 +845,10 @@ int enqueue_step(edg_batch *b)
     for (int c = 0; c < kClasses; ++c) {
         if (!b->cls_n[c]) continue;
         if (launch_batch_hv(b->stream, b->fp64, class_cols(c), (c & 1) ? 16 : 8, b->d_inst + b->cls_start[c],
-                            b->d_hvoff[c], b->cls_n[c], b->cls_tiles[c], b->cls_lds[c]))
+                            b->d_hvoff[c], b->d_hvown[c], b->cls_tiles[c], b->cls_lds[c]))
             return fail(b, EDG_ERR_HIP, "batched H.v launch failed");
     }
-    if (launch_batch_axpy(b->stream, b->fp64, b->d_inst, b->d_axoff, n, b->nax))
+    if (launch_batch_axpy(b->stream, b->fp64, b->d_inst, b->d_axoff, b->d_axown, b->nax))
         return fail(b, EDG_ERR_HIP, "batched axpy launch failed");
     if (launch_batch_advance(b->stream, b->d_inst, n)) return fail(b, EDG_ERR_HIP, "advance launch failed");
     return 0;

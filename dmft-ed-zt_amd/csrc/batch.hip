@@ -16,17 +16,6 @@
 
 namespace edg {
 
-// Index of the chain owning workgroup b: largest i with off[i] <= b.
-__device__ __forceinline__ int owner(const int *off, int ninst, int b)
-{
-    int lo = 0, hi = ninst;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (off[mid] <= b) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
 
 template <typename T>
 __device__ __forceinline__ T *slot_of(const BInst &I, int k)
@@ -41,7 +30,7 @@ __device__ __forceinline__ T *slot_of(const BInst &I, int k)
 // compressed tables (target rank | value index << 16, KC slots per row,
 // self-padded).
 template <typename T, typename ACC, int KS>
-__global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, const int *__restrict__ hv_off, int ninst)
+__global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, const int *__restrict__ hv_off, const int *__restrict__ own)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     T *xs = reinterpret_cast<T *>(smem);
@@ -50,7 +39,7 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, 
     __shared__ uint8_t s_md[kBatchMaxCols];
     __shared__ ACC s_vu[2 * (kMaxHops + 1)], s_vd[2 * (kMaxHops + 1)];
 
-    const int i = owner(hv_off, ninst, blockIdx.x);
+    const int i = own[blockIdx.x];  // chain of this workgroup
     const BInst &I = inst[i];
     if (I.flags[0]) return;
     const int t = blockIdx.x - hv_off[i];
@@ -141,9 +130,9 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, 
 // contiguous ranges, as in the single-chain kernel; tiles past the chain's
 // hvn are padding and exit at once.
 template <typename T, int C, int SLOTS>
-__global__ __launch_bounds__(kHvBlock) void k_bhvw(const BInst *__restrict__ inst, const int *__restrict__ hv_off, int ninst)
+__global__ __launch_bounds__(kHvBlock) void k_bhvw(const BInst *__restrict__ inst, const int *__restrict__ hv_off, const int *__restrict__ own)
 {
-    const int i = owner(hv_off, ninst, blockIdx.x);
+    const int i = own[blockIdx.x];  // chain of this workgroup
     const BInst &I = inst[i];
     const int t = blockIdx.x - hv_off[i];
     if (I.flags[0] || t >= I.hvn) return;
@@ -173,9 +162,9 @@ __global__ __launch_bounds__(kHvBlock) void k_bhvw(const BInst *__restrict__ ins
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_baxpy(const BInst *__restrict__ inst, const int *__restrict__ ax_off, int ninst)
+__global__ __launch_bounds__(kBlock) void k_baxpy(const BInst *__restrict__ inst, const int *__restrict__ ax_off, const int *__restrict__ own)
 {
-    const int i = owner(ax_off, ninst, blockIdx.x);
+    const int i = own[blockIdx.x];  // chain of this workgroup
     const BInst &I = inst[i];
     if (I.flags[0]) return;
     const int t = blockIdx.x - ax_off[i];
@@ -235,35 +224,35 @@ static int set_lds_ks(size_t max_lds)
 int batch_set_lds(size_t max_lds) { return set_lds_ks<8>(max_lds) || set_lds_ks<16>(max_lds) ? -1 : 0; }
 
 template <int C>
-static void (*wide_kernel(bool fp64, int ks))(const BInst *, const int *, int)
+static void (*wide_kernel(bool fp64, int ks))(const BInst *, const int *, const int *)
 {
     if (ks == 8) return fp64 ? k_bhvw<double, C, 8> : k_bhvw<float, C, 8>;
     return fp64 ? k_bhvw<double, C, 16> : k_bhvw<float, C, 16>;
 }
 
 int launch_batch_hv(hipStream_t s, bool fp64, int wide_cols, int ks, const BInst *inst, const int *hv_off,
-                    int ninst, int ntiles, int lds_elems)
+                    const int *own, int ntiles, int lds_elems)
 {
     if (ntiles <= 0) return 0;
     const size_t lds = (size_t)lds_elems * (fp64 ? 8 : 4);
     const bool wide = wide_cols > 0;
-    void (*f)(const BInst *, const int *, int);
+    void (*f)(const BInst *, const int *, const int *);
     if (wide) {
         f = wide_cols == 1 ? wide_kernel<1>(fp64, ks) : wide_kernel<kBatchWideCols>(fp64, ks);
     } else {
         if (ks == 8) f = fp64 ? k_bhv<double, double, 8> : k_bhv<float, float, 8>;
         else f = fp64 ? k_bhv<double, double, 16> : k_bhv<float, float, 16>;
     }
-    hipLaunchKernelGGL(f, dim3(ntiles), dim3(wide ? kHvBlock : kBlock), lds, s, inst, hv_off, ninst);
+    hipLaunchKernelGGL(f, dim3(ntiles), dim3(wide ? kHvBlock : kBlock), lds, s, inst, hv_off, own);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_batch_axpy(hipStream_t s, bool fp64, const BInst *inst, const int *ax_off, int ninst,
+int launch_batch_axpy(hipStream_t s, bool fp64, const BInst *inst, const int *ax_off, const int *own,
                       int nblocks)
 {
     if (nblocks <= 0) return 0;
-    if (fp64) hipLaunchKernelGGL(k_baxpy<double>, dim3(nblocks), dim3(kBlock), 0, s, inst, ax_off, ninst);
-    else hipLaunchKernelGGL(k_baxpy<float>, dim3(nblocks), dim3(kBlock), 0, s, inst, ax_off, ninst);
+    if (fp64) hipLaunchKernelGGL(k_baxpy<double>, dim3(nblocks), dim3(kBlock), 0, s, inst, ax_off, own);
+    else hipLaunchKernelGGL(k_baxpy<float>, dim3(nblocks), dim3(kBlock), 0, s, inst, ax_off, own);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -155,14 +155,15 @@ int launch_cf(hipStream_t s, int nw, const double *omega, double e0, int np, con
 // ---- batched Lanczos steps (batch.hip) ----
 // hv_off / ax_off: prefix sums (ninst + 1 entries) of the chains' H.v tiles
 // and axpy workgroups; the launches cover hv_off[ninst] / ax_off[ninst]
-// workgroups.  lds_elems: largest staged elements per tile over the chains.
+// workgroups, and own[w] is the chain of workgroup w.  lds_elems: largest
+// staged elements per tile over the chains.
 // wide_cols: 0 for short-column chains (BInst::cb columns per 256-thread
 // tile); 1 or kBatchWideCols for long-column chains (that many columns per
 // 512-thread tile, tile offsets of every chain a multiple of 8).  ks: hop
 // slots per row of every table of the launched chains (8 or 16).
 int launch_batch_hv(hipStream_t s, bool fp64, int wide_cols, int ks, const BInst *inst, const int *hv_off,
-                    int ninst, int ntiles, int lds_elems);
-int launch_batch_axpy(hipStream_t s, bool fp64, const BInst *inst, const int *ax_off, int ninst,
+                    const int *own, int ntiles, int lds_elems);
+int launch_batch_axpy(hipStream_t s, bool fp64, const BInst *inst, const int *ax_off, const int *own,
                       int nblocks);
 int launch_batch_advance(hipStream_t s, const BInst *inst, int ninst);
 // Allow max_lds bytes of dynamic LDS for the batched H.v (call once, outside
