@@ -42,6 +42,9 @@ constexpr long long kPadB = 64;       // element alignment of vectors
 constexpr int kPartialCap = 1 << 17;  // doubles of the shared partial-sum scratch
 constexpr size_t kLdsMax = 152 * 1024;
 constexpr int kClasses = 6;            // H.v launch classes, chain_class()
+constexpr int kChunk = 8;              // batched steps launched per chunk
+constexpr int kAhead = 2;              // chunks the host may run ahead of the device
+constexpr int kChunkEvents = 4;        // > kAhead
 
 long long rup(long long n, long long m) { return (n + m - 1) / m * m; }
 
@@ -70,6 +73,7 @@ struct Chain {
     BInst bi{};
     int launched = 0;  // steps launched
     bool exists = false;
+    bool pending = false;  // started on the side stream, not yet in the batched steps
     bool active = false;
     int ispin = 0, pm = 0;
     long long D = 0;
@@ -114,6 +118,10 @@ struct Job {
     double *host = nullptr;     // pinned readback / upload area
     size_t host_n = 0;
     hipEvent_t ev = nullptr;
+    // recorded on the side stream after the stage work that started chains;
+    // the chains join the batched steps once the host sees it complete
+    hipEvent_t start_ev = nullptr;
+    bool start_pending = false;
     double e0 = 0.0;
     int gs_nstep = 0;
     std::vector<double> G;
@@ -126,14 +134,16 @@ struct edg_batch {
     int device = 0;
     // stream: the batched Lanczos steps (and the reads of finished chains);
     // side: per-problem stage work (tables, start vectors, combine, RQ,
-    // apply_c, continued fractions), which then overlaps the steps.  The
-    // steps wait on side_ev, recorded after stage work that started chains;
-    // the host moves a problem to its next stage only after an event shows
-    // the previous stage's work (on either stream) is done.
+    // apply_c, continued fractions), which overlaps the steps.  The streams
+    // never wait on each other: the host moves a problem to its next stage,
+    // and lets its new chains join the steps, only after an event shows the
+    // previous stage's work is done.  The host stays at most kAhead chunks
+    // of kChunk steps ahead of the device, so chains that become ready join
+    // within a few steps.
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;
-    hipEvent_t side_ev = nullptr;
-    bool side_pending = false;
+    hipEvent_t chunk_ev[4] = {};
+    int chunk_head = 0, chunk_n = 0;
     bool fp64 = false;
     size_t es = 4;
     int nspin = 1;
@@ -426,10 +436,7 @@ int start_chain(edg_batch *b, Job *J, Chain &C, double *norm_out, int slot)
                      : launch_norm<float>(b->side, (const float *)I.base, C.D, b->partial, b->ticket,
                                           norm_out, I.scale, nullptr);
     if (rc) return fail(b, EDG_ERR_HIP, "norm launch failed");
-    C.active = true;
-    b->active.push_back(&C);
-    b->dirty = true;
-    b->side_pending = true;  // the chain's first step waits for this stage work
+    C.pending = true;  // joins the steps when the host sees J->start_ev complete
     return 0;
 }
 
@@ -449,6 +456,7 @@ int admit(edg_batch *b, Job *J)
     if (!J->host) return fail(b, EDG_ERR_NOMEM, "pinned host allocation failed");
     J->host_n = b->pinned_n;
     BCHK(b, hipEventCreateWithFlags(&J->ev, hipEventDisableTiming));
+    BCHK(b, hipEventCreateWithFlags(&J->start_ev, hipEventDisableTiming));
     const int nbath = p.nbath, ns = nsite_of(p);
     J->dparams = (double *)J->ar.take(8 * (2 + 4 * (size_t)nbath));
     if (J->ar.overflow) return fail(b, EDG_ERR_NOMEM, "problem arena overflow");
@@ -486,6 +494,8 @@ int admit(edg_batch *b, Job *J)
                             : launch_start<float>(b->side, (float *)J->gsc.bi.base, D, p.seed);
     if (rc0) return fail(b, EDG_ERR_HIP, "start launch failed");
     if (int rc = start_chain(b, J, J->gsc, &J->misc[0], 0)) return rc;
+    BCHK(b, hipEventRecord(J->start_ev, b->side));
+    J->start_pending = true;
     J->st = ST_GS;
     b->running.push_back(J);
     return 0;
@@ -586,6 +596,9 @@ int after_gs(edg_batch *b, Job *J)
         if (arc) return fail(b, EDG_ERR_HIP, "apply_c launch failed");
         if (int r = start_chain(b, J, C, &J->misc[8 + k], 1 + k)) return r;
     }
+    // also covers the RQ sums a ground-state-only problem reads next
+    BCHK(b, hipEventRecord(J->start_ev, b->side));
+    J->start_pending = true;
     J->st = ST_CHAINS;
     return 0;
 }
@@ -593,7 +606,7 @@ int after_gs(edg_batch *b, Job *J)
 bool chains_running(const Job *J)
 {
     for (int k = 0; k < 4; ++k)
-        if (J->ch[k].exists && J->ch[k].active) return true;
+        if (J->ch[k].exists && (J->ch[k].active || J->ch[k].pending)) return true;
     return false;
 }
 
@@ -679,6 +692,8 @@ int finish(edg_batch *b, Job *J)
     J->host = nullptr;
     (void)hipEventDestroy(J->ev);
     J->ev = nullptr;
+    (void)hipEventDestroy(J->start_ev);
+    J->start_ev = nullptr;
     J->st = ST_DONE;
     b->st_done += 1;
     return 0;
@@ -689,6 +704,21 @@ int progress(edg_batch *b)
 {
     for (size_t i = 0; i < b->running.size();) {
         Job *J = b->running[i];
+        if (J->start_pending) {
+            if (hipEventQuery(J->start_ev) != hipSuccess) {
+                ++i;
+                continue;
+            }
+            J->start_pending = false;
+            Chain *cs[5] = {&J->gsc, &J->ch[0], &J->ch[1], &J->ch[2], &J->ch[3]};
+            for (Chain *C : cs)
+                if (C->pending) {
+                    C->pending = false;
+                    C->active = true;
+                    b->active.push_back(C);
+                    b->dirty = true;
+                }
+        }
         if (J->st == ST_GS && !J->gsc.active) {
             read_chain(b, J, J->gsc, 0);
             BCHK(b, hipEventRecord(J->ev, b->stream));
@@ -696,10 +726,7 @@ int progress(edg_batch *b)
         } else if (J->st == ST_GS_READ && hipEventQuery(J->ev) == hipSuccess) {
             if (int rc = after_gs(b, J)) return rc;
         } else if (J->st == ST_CHAINS && !chains_running(J)) {
-            // misc holds the RQ sums and start norms written on the side stream
-            // (a ground-state-only problem started no chain that waited for them)
-            BCHK(b, hipEventRecord(b->side_ev, b->side));
-            BCHK(b, hipStreamWaitEvent(b->stream, b->side_ev, 0));
+            // misc (RQ sums, start norms) was written before start_ev
             BCHK(b, hipMemcpyAsync(J->host, J->misc, 16 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
             const int mg = green_len(J->p);
             for (int k = 0; k < 4; ++k)
@@ -871,11 +898,6 @@ int run_steps(edg_batch *b, int k)
         if (e != hipSuccess) return fail(b, EDG_ERR_HIP, "graph instantiation failed");
         b->graph_valid = true;
     }
-    if (b->side_pending) {
-        BCHK(b, hipEventRecord(b->side_ev, b->side));
-        BCHK(b, hipStreamWaitEvent(b->stream, b->side_ev, 0));
-        b->side_pending = false;
-    }
     for (int s = 0; s < k; ++s) {
         if (b->use_graphs) {
             BCHK(b, hipGraphLaunch(b->gexec, b->stream));
@@ -925,7 +947,7 @@ int edg_batch_create(int device, int precision, int nspin, int nw, const double 
     };
     chk(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     chk(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
-    chk(hipEventCreateWithFlags(&b->side_ev, hipEventDisableTiming));
+    for (auto &e : b->chunk_ev) chk(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     chk(hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, device));
     chk(hipMalloc((void **)&b->omega, sizeof(double) * nw));
     chk(hipMalloc((void **)&b->partial, sizeof(double) * kPartialCap));
@@ -962,6 +984,7 @@ int edg_batch_destroy(edg_batch *b)
         if (J->ar.base && b->heap) b->heap->free(J->ar.base);
         if (J->host) (void)hipHostFree(J->host);
         if (J->ev) (void)hipEventDestroy(J->ev);
+        if (J->start_ev) (void)hipEventDestroy(J->start_ev);
         delete J;
     };
     for (Job *J : b->queue) drop(J);
@@ -983,7 +1006,8 @@ int edg_batch_destroy(edg_batch *b)
     if (b->omega) (void)hipFree(b->omega);
     if (b->partial) (void)hipFree(b->partial);
     if (b->ticket) (void)hipFree(b->ticket);
-    if (b->side_ev) (void)hipEventDestroy(b->side_ev);
+    for (auto &e : b->chunk_ev)
+        if (e) (void)hipEventDestroy(e);
     if (b->side) (void)hipStreamDestroy(b->side);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -1070,17 +1094,35 @@ int edg_batch_step(edg_batch *b, int max_steps, int *pending)
         b->queue.pop_front();
     }
     if (!b->active.empty() && max_steps > 0) {
-        int k = max_steps;
-        for (const Chain *C : b->active) k = std::min(k, C->bi.maxn - C->launched);
-        if (int rc = run_steps(b, k)) return rc;
-        if (int rc = progress(b)) return rc;
+        int left = max_steps;
+        while (left > 0 && !b->active.empty()) {
+            int k = std::min(left, kChunk);
+            for (const Chain *C : b->active) k = std::min(k, C->bi.maxn - C->launched);
+            if (int rc = run_steps(b, k)) return rc;
+            left -= k;
+            // bounded run-ahead: wait for the chunk kAhead chunks back
+            BCHK(b, hipEventRecord(b->chunk_ev[b->chunk_head], b->stream));
+            b->chunk_head = (b->chunk_head + 1) % kChunkEvents;
+            if (++b->chunk_n > kAhead) {
+                const int old = (b->chunk_head + kChunkEvents - b->chunk_n) % kChunkEvents;
+                BCHK(b, hipEventSynchronize(b->chunk_ev[old]));
+                --b->chunk_n;
+            }
+            if (int rc = progress(b)) return rc;
+            if (b->dirty) break;  // chains joined or left: let the caller see progress
+        }
     } else if (!b->running.empty()) {
-        // nothing to step: wait for the oldest outstanding copy, then move on
-        for (Job *J : b->running)
+        // nothing to step: wait for the oldest outstanding event, then move on
+        for (Job *J : b->running) {
+            if (J->start_pending) {
+                BCHK(b, hipEventSynchronize(J->start_ev));
+                break;
+            }
             if (J->st == ST_GS_READ || J->st == ST_CH_READ || J->st == ST_CF_READ) {
                 BCHK(b, hipEventSynchronize(J->ev));
                 break;
             }
+        }
         if (int rc = progress(b)) return rc;
     }
     if (pending) *pending = (int)(b->queue.size() + b->running.size());
