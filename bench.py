@@ -114,15 +114,15 @@ def profile_hv(edgpu, em, ek, vmk, U, mu):
                       "copy_bytes_written": nbytes}))
 
 
-def bench_batched(edgpu, W, S):
+def bench_batched(edgpu, W, S, graphs=True):
     """512 independent single-orbital problems (Ns in {8,10,12}) through the
     batched solver: whole-batch solves/s, after a warm-up batch of 32."""
     from edgpu import autotune
     om = W.matsubara(2048, 100.0)
     probs = W.batched_problems(edgpu, n=512)
-    sched, timings = autotune.tune_batch(edgpu, probs, om)
-    S.run_batched(edgpu, probs[:32], om, schedule=sched)
-    res, wall, st = S.run_batched(edgpu, probs, om, schedule=sched)
+    sched, timings = autotune.tune_batch(edgpu, probs, om, graphs=graphs)
+    S.run_batched(edgpu, probs[:32], om, schedule=sched, graphs=graphs)
+    res, wall, st = S.run_batched(edgpu, probs, om, schedule=sched, graphs=graphs)
     ns = [p.nbath + 1 for p in probs]
     return dict(workload="512 single-orbital problems, Ns uniform in {8,10,12}, U~U[0,6], "
                          "n~U[0.8,1.2] (N=round(n Ns), N_up=ceil(N/2)), eps~U[-1.5,1.5], "
@@ -135,15 +135,15 @@ def bench_batched(edgpu, W, S):
                 e0_finite=bool(np.all(np.isfinite([r["e0"] for r in res]))))
 
 
-def bench_served(edgpu, W, S, seconds):
+def bench_served(edgpu, W, S, seconds, graphs=True):
     """Poisson arrivals at 200 req/s for `seconds` (Ns 10/12/14 with weights
     0.5/0.35/0.15, U~U[0,8]) replayed against the wall clock; latency =
     result time - arrival time, solves/s over the whole run (arrivals + drain)."""
     from edgpu import autotune
     om = W.matsubara(2048, 100.0)
     req = W.served_requests(edgpu, rate=200.0, duration=seconds)
-    sched, _ = autotune.tune_batch(edgpu, [p for _, p in req], om, sample=8)
-    r = S.serve(edgpu, req, om, drain_limit=300.0, log=log, schedule=sched)
+    sched, _ = autotune.tune_batch(edgpu, [p for _, p in req], om, sample=8, graphs=graphs)
+    r = S.serve(edgpu, req, om, drain_limit=300.0, log=log, schedule=sched, graphs=graphs)
     return dict(workload=f"Poisson 200 req/s for {seconds:g} s, Ns in {{10,12,14}} (0.5/0.35/0.15), "
                          "U~U[0,8], half filling, 200 GS steps + 2 chains x 150 steps, FP32 vectors; "
                          "admission by device bytes (stored Krylov basis + rings)",
@@ -195,6 +195,9 @@ def main():
                     help="seconds of Poisson arrivals for the served-stream figures (0: skip)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-kanamori", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="batched and served sections launch each step's kernels instead of "
+                         "replaying a captured HIP graph (for kernel-trace profiles)")
     ap.add_argument("--profile-hv", action="store_true",
                     help="only run H.v launches (FP32, Ns=14) and one device copy of the same "
                          "byte count, for hardware-counter profiles")
@@ -262,10 +265,10 @@ def main():
     from edgpu import serve as S
     if not args.no_batched:
         log("batched problems")
-        extra["batched"] = bench_batched(edgpu, W, S)
+        extra["batched"] = bench_batched(edgpu, W, S, graphs=not args.no_graphs)
     if args.serve_seconds > 0:
         log("served stream")
-        extra["served"] = bench_served(edgpu, W, S, args.serve_seconds)
+        extra["served"] = bench_served(edgpu, W, S, args.serve_seconds, graphs=not args.no_graphs)
     if not args.no_kanamori:
         log("two-orbital Kanamori")
         extra["kanamori"] = bench_kanamori(edgpu, W)

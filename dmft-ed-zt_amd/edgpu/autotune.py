@@ -46,13 +46,12 @@ def _classes(problems):
 
 
 def tune_batch(edgpu, problems, omega, precision="fp32", sample=8, tiles=BATCH_TILES, steps=(80, 60),
-               log=None):
+               log=None, graphs=True):
     """Per size class (number of sites) of `problems`, time the candidate
     tile shapes on up to `sample` of its problems (with `steps` =
     (ground-state, chain) Lanczos lengths) and return the schedule
     [(nsite_max, tile_elems, wide), ...] plus the timings
     {nsite: {candidate: seconds}}."""
-    from .serve import run_batched
     classes = _classes(problems)
     timings, chosen = {}, {}
     for nsite, probs in classes.items():
@@ -68,7 +67,7 @@ def tune_batch(edgpu, problems, omega, precision="fp32", sample=8, tiles=BATCH_T
             sched = [(64, tile, wide)]
             best_t = None
             for _ in range(2):
-                t = _timed(edgpu, sub, omega, precision, sched, run_batched)
+                t = _timed(edgpu, sub, omega, precision, sched, graphs)
                 best_t = t if best_t is None else min(best_t, t)
             res[(tile, wide)] = best_t
             if log:
@@ -80,8 +79,8 @@ def tune_batch(edgpu, problems, omega, precision="fp32", sample=8, tiles=BATCH_T
     return sched, timings
 
 
-def _timed(edgpu, problems, omega, precision, sched, run_batched):
-    b = edgpu.Batch(omega, precision=precision)
+def _timed(edgpu, problems, omega, precision, sched, graphs):
+    b = edgpu.Batch(omega, precision=precision, graphs=graphs)
     b.set_schedule(sched)
     t = time.perf_counter()
     for p in problems:
