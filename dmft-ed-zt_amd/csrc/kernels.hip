@@ -1009,13 +1009,30 @@ __global__ __launch_bounds__(kBlock) void k_axpy_norm(T *r, const T *u, long lon
     const double al = sj * sum_partials<kBlock>(hv_part, hv_n);
     if (blockIdx.x == 0 && threadIdx.x == 0) alpha[step] = al;
     const double coef = al * sj;
+    // 16-byte vectors (vectors are 256-byte aligned); workgroup 0 also takes
+    // the n % V tail.  The update itself is FP64.
+    constexpr int V = 16 / sizeof(T);
+    using Vec = Row<T, V>;
+    const long long nv = n / V, stride = (long long)gridDim.x * kBlock;
+    Vec *rv = reinterpret_cast<Vec *>(r);
+    const Vec *uv = reinterpret_cast<const Vec *>(u);
     double v[1] = {0.0};
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
-         i += (long long)gridDim.x * kBlock) {
-        const T t = (T)((double)r[i] - coef * (double)u[i]);
-        r[i] = t;
-        v[0] += (double)t * (double)t;
+    for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < nv; q += stride) {
+        const Vec a = rv[q], b = uv[q];
+        Vec w;
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            w.v[i] = (T)((double)a.v[i] - coef * (double)b.v[i]);
+            v[0] += (double)w.v[i] * (double)w.v[i];
+        }
+        rv[q] = w;
     }
+    if (blockIdx.x == 0)
+        for (long long i = nv * V + threadIdx.x; i < n; i += kBlock) {
+            const T t = (T)((double)r[i] - coef * (double)u[i]);
+            r[i] = t;
+            v[0] += (double)t * (double)t;
+        }
     block_partial<1>(v, partial);
 }
 
@@ -1024,6 +1041,7 @@ int launch_axpy_norm(hipStream_t s, T *r, const T *u, long long n, const double 
                      int hv_n, const double *scale, double *alpha, int step, double *partial,
                      int *stop)
 {
+    if (((uintptr_t)r | (uintptr_t)u) & 15) return -1;  // vector kernels need 16-byte alignment
     int grid = grid_for(n);
     if (grid > 1024) grid = 1024;
     hipLaunchKernelGGL(k_axpy_norm<T>, dim3(grid), dim3(kBlock), 0, s, r, u, n, hv_part, hv_n,
