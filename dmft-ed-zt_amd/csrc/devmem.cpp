@@ -13,7 +13,9 @@ namespace edg {
 namespace {
 
 constexpr size_t kAlign = size_t(2) << 20;   // region granularity (2 MiB)
-constexpr size_t kMinChunk = size_t(4) << 30;  // smallest chunk taken from the driver
+// smallest chunk taken from the driver: chunks left by small problems must
+// still hold the large ones (an Ns=14 problem with its Krylov basis is ~10 GB)
+constexpr size_t kMinChunk = size_t(16) << 30;
 constexpr int kMaxDevices = 64;
 
 size_t round_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
@@ -134,11 +136,25 @@ int DevHeap::alloc(size_t bytes, void **p)
         }
     }
     // 3. a new chunk from the driver: kMinChunk or the request, else exactly
-    //    the request after the idle cache has been returned
+    //    the request after returning idle cached chunks (largest first, one at
+    //    a time: unmapping costs about 4 ms per GB)
     DevChunk *c = new_chunk(std::max(n, kMinChunk));
     if (!c && n < kMinChunk) c = new_chunk(n);
-    if (!c) {
-        devmem_release_cache(device_);
+    while (!c) {
+        DevChunk *drop = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(g_mu);
+            auto &cache = g_cache[device_];
+            auto big = std::max_element(cache.begin(), cache.end(),
+                                        [](const DevChunk *a, const DevChunk *b) { return a->size < b->size; });
+            if (big != cache.end()) {
+                drop = *big;
+                cache.erase(big);
+            }
+        }
+        if (!drop) break;
+        (void)hipFree(drop->base);
+        delete drop;
         c = new_chunk(n);
     }
     if (!c) return 1;
