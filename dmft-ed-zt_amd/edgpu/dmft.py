@@ -15,10 +15,15 @@ dmft_green.f90), for callers that drive the self-consistency from Python:
   quasiparticle_weight       Z = 1 / (1 - Im Sigma(i w_1) / w_1) (post_process.f90:40-56)
   ground_state_sweep         the sector sweep of ground_state.f90:30-73 with
                              every candidate sector in one batched solve
+  fit_bath                   the bath fit of fit_h_imp_params
+                             (ed_projection.f90:23-550), conjugate gradients
+  run_dmft                   the self-consistency loop of dmft.f90:60-137
+                             around any impurity solver
 
-These run on the host (the hybridization, local Green's function and file
-I/O are O(nw * nbath) or O(nw * 10000) work per DMFT iteration) except the
-sweep, which runs on the device through edgpu.Batch.
+These run on the host (the hybridization, local Green's function, bath fit
+and file I/O are O(nw * nbath) or O(nw * 10000) work per DMFT iteration)
+except the sweep, which runs on the device through edgpu.Batch, and the
+impurity solve that run_dmft is given.
 """
 import numpy as np
 
@@ -174,3 +179,235 @@ def write_sigma(omega, Sigma, directory="."):
                     f.write(f"{omega[iw]:12.8f}{Sigma[iw, m, s].real:12.8f}{Sigma[iw, m, s].imag:12.8f}\n")
             paths.append(path)
     return paths
+
+
+# --------------------------------------------------------------- bath fit
+#
+# fit_h_imp_params (ed_projection.f90:23-115): new (em, ek, vmk) of one spin
+# minimize
+#   f(x) = 1/(norb nw) sum_{m, n} |em_m - em_m^old + D_m(iw_n; ek, vmk) - D_m^old(iw_n)
+#                                  + 1/G_loc^old - 1/G_cl^old|^2 / w_n
+# (func :182-213, weight 1/w_n :215-219) over x = (em(:), ek(:), vmk(:,:)) in
+# the order of emk_to_x (:117-135), by Polak-Ribiere conjugate gradients
+# with the analytic gradient (dfunc :221-280) and a bracketed Brent line
+# search (frprmn :313-357, linmin :359-378, mnbrak / brent :380-550): the
+# published Numerical Recipes method, written here for numpy.
+
+class _FitProblem:
+    def __init__(self, em, ek, vmk, D_cl, G_cl, G_loc, omega, ispin):
+        self.norb, self.nbath = vmk.shape[0], vmk.shape[1]
+        self.s = ispin
+        self.w = np.asarray(omega, dtype=np.float64)
+        self.iw = 1j * self.w
+        self.em_old = np.array(em[:, ispin], dtype=np.float64)
+        # the part of diff that does not depend on x
+        self.c = (-self.em_old[None, :] - D_cl[:, :, ispin] + 1.0 / G_loc[:, :, ispin]
+                  - 1.0 / G_cl[:, :, ispin])
+        self.nw = self.w.size
+
+    def split(self, x):
+        no, nb = self.norb, self.nbath
+        em = x[:no]
+        ek = x[no:no + nb]
+        v = x[no + nb:].reshape(nb, no).T  # vmk(iorb, ibath): iorb fastest in x
+        return em, ek, v
+
+    def diff(self, x):
+        em, ek, v = self.split(x)
+        den = self.iw[:, None] - ek[None, :]                       # (nw, nbath)
+        D = (v[None, :, :] ** 2 / den[:, None, :]).sum(axis=2)     # (nw, norb)
+        return em[None, :] + D + self.c, den
+
+    def f(self, x):
+        d, _ = self.diff(x)
+        return float(((np.abs(d) ** 2) / self.w[:, None]).sum() / (self.norb * self.nw))
+
+    def df(self, x):
+        em, ek, v = self.split(x)
+        d, den = self.diff(x)
+        wt = 1.0 / self.w[:, None]
+        g_em = (2.0 * wt * d.real).sum(axis=0)
+        # d D_m / d ek_b = v_mb^2 / (iw - ek_b)^2 ; d D_m / d v_mb = 2 v_mb / (iw - ek_b)
+        # d|d|^2 = 2 Re(d conj(dd))
+        inv = 1.0 / den                                            # (nw, nbath)
+        g_ek = np.zeros(self.nbath)
+        g_v = np.zeros((self.norb, self.nbath))
+        for m in range(self.norb):
+            dm = d[:, m][:, None]
+            g_ek += (2.0 * wt * (dm * np.conj(v[m][None, :] ** 2 * inv ** 2)).real).sum(axis=0)
+            g_v[m] = (2.0 * wt * (dm * np.conj(2.0 * v[m][None, :] * inv)).real).sum(axis=0)
+        g = np.concatenate([g_em, g_ek, g_v.T.reshape(-1)])
+        return g / (self.norb * self.nw)
+
+
+def _mnbrak(f1, ax, bx, gold=1.618034, glimit=100.0, tiny=1e-20):
+    fa, fb = f1(ax), f1(bx)
+    if fb > fa:
+        ax, bx, fa, fb = bx, ax, fb, fa
+    cx = bx + gold * (bx - ax)
+    fc = f1(cx)
+    while fb >= fc:
+        r = (bx - ax) * (fb - fc)
+        q = (bx - cx) * (fb - fa)
+        u = bx - ((bx - cx) * q - (bx - ax) * r) / (2.0 * np.copysign(max(abs(q - r), tiny), q - r))
+        ulim = bx + glimit * (cx - bx)
+        if (bx - u) * (u - cx) > 0.0:
+            fu = f1(u)
+            if fu < fc:
+                return bx, u, cx
+            if fu > fb:
+                return ax, bx, u
+            u = cx + gold * (cx - bx)
+            fu = f1(u)
+        elif (cx - u) * (u - ulim) > 0.0:
+            fu = f1(u)
+            if fu < fc:
+                bx, cx, u = cx, u, u + gold * (u - cx)
+                fb, fc, fu = fc, fu, f1(u)
+        elif (u - ulim) * (ulim - cx) >= 0.0:
+            u = ulim
+            fu = f1(u)
+        else:
+            u = cx + gold * (cx - bx)
+            fu = f1(u)
+        ax, bx, cx = bx, cx, u
+        fa, fb, fc = fb, fc, fu
+    return ax, bx, cx
+
+
+def _brent(f1, ax, bx, cx, tol=1e-8, itmax=10000, zeps=1e-10):
+    cgold = 0.3819660
+    a, b = min(ax, cx), max(ax, cx)
+    x = w = v = bx
+    fx = fw = fv = f1(x)
+    d = e = 0.0
+    for _ in range(itmax):
+        xm = 0.5 * (a + b)
+        tol1 = tol * abs(x) + zeps
+        tol2 = 2.0 * tol1
+        if abs(x - xm) <= tol2 - 0.5 * (b - a):
+            return x, fx
+        golden = True
+        if abs(e) > tol1:
+            r = (x - w) * (fx - fv)
+            q = (x - v) * (fx - fw)
+            p = (x - v) * q - (x - w) * r
+            q = 2.0 * (q - r)
+            if q > 0.0:
+                p = -p
+            q = abs(q)
+            etemp, e = e, d
+            if not (abs(p) >= abs(0.5 * q * etemp) or p <= q * (a - x) or p >= q * (b - x)):
+                d = p / q
+                u = x + d
+                if u - a < tol2 or b - u < tol2:
+                    d = np.copysign(tol1, xm - x)
+                golden = False
+        if golden:
+            e = (a - x) if x >= xm else (b - x)
+            d = cgold * e
+        u = x + d if abs(d) >= tol1 else x + np.copysign(tol1, d)
+        fu = f1(u)
+        if fu <= fx:
+            if u >= x:
+                a = x
+            else:
+                b = x
+            v, fv, w, fw, x, fx = w, fw, x, fx, u, fu
+        else:
+            if u < x:
+                a = u
+            else:
+                b = u
+            if fu <= fw or w == x:
+                v, fv, w, fw = w, fw, u, fu
+            elif fu <= fv or v == x or v == w:
+                v, fv = u, fu
+    raise RuntimeError("brent: maximum iterations exceeded")
+
+
+def conjugate_gradient(f, df, p, ftol=1e-6, itmax=20000, eps=1e-10):
+    """Polak-Ribiere minimization of f from p (frprmn): returns (p, fret,
+    iterations).  Stops when 2|f_new - f_old| <= ftol (|f_new| + |f_old| + eps)
+    after a line minimization, or when the gradient vanishes."""
+    p = np.array(p, dtype=np.float64)
+    fp = f(p)
+    xi = df(p)
+    g = -xi
+    h = g.copy()
+    xi = h.copy()
+    for it in range(1, itmax + 1):
+        f1 = lambda t: f(p + t * xi)
+        ax, bx, cx = _mnbrak(f1, 0.0, 1.0)
+        xmin, fret = _brent(f1, ax, bx, cx)
+        p = p + xmin * xi
+        if 2.0 * abs(fret - fp) <= ftol * (abs(fret) + abs(fp) + eps):
+            return p, fret, it
+        fp = fret
+        xi = df(p)
+        gg = float(g @ g)
+        if gg == 0.0:
+            return p, fret, it
+        gam = float((xi + g) @ xi) / gg
+        g = -xi
+        h = g + gam * h
+        xi = h.copy()
+    raise RuntimeError("conjugate_gradient: maximum iterations exceeded")
+
+
+def fit_bath(em, ek, vmk, D_cl, G_cl, G_loc, omega, nspin=1, tol=1e-6):
+    """New (em, ek, vmk) from the previous iteration's D_cl, G_cl and G_loc
+    (fit_h_imp_params, ed_projection.f90:23-115).  Arrays in the reference
+    layouts em(norb,2), ek(nbath,2), vmk(norb,nbath,2), D_cl/G_cl/G_loc
+    (nw,norb,nspin).  Returns (em, ek, vmk, [(iterations, f) per spin]); for
+    nspin = 1 spin 2 copies spin 1 (:95-100)."""
+    em = np.array(em, dtype=np.float64)
+    ek = np.array(ek, dtype=np.float64)
+    vmk = np.array(vmk, dtype=np.float64)
+    D_cl, G_cl, G_loc = (np.asarray(a, dtype=np.complex128) for a in (D_cl, G_cl, G_loc))
+    info = []
+    for s in range(nspin):
+        prob = _FitProblem(em, ek, vmk, D_cl, G_cl, G_loc, omega, s)
+        x0 = np.concatenate([em[:, s], ek[:, s], vmk[:, :, s].T.reshape(-1)])
+        x, fret, it = conjugate_gradient(prob.f, prob.df, x0, ftol=tol)
+        e, k, v = prob.split(x)
+        em[:, s], ek[:, s], vmk[:, :, s] = e, k, v
+        info.append((it, fret))
+    if nspin == 1:
+        em[:, 1], ek[:, 1], vmk[:, :, 1] = em[:, 0], ek[:, 0], vmk[:, :, 0]
+    return em, ek, vmk, info
+
+
+def scf_difference(G_cl, G_loc, nbath):
+    """test_convergence (dmft.f90:111-137): sum |G_cl - G_loc|^2 / (norb nbath nw)."""
+    nw, norb, nspin = np.shape(G_cl)
+    return float((np.abs(np.asarray(G_cl) - np.asarray(G_loc)) ** 2).sum() / (norb * nbath * nw))
+
+
+def run_dmft(solve, em, ek, vmk, omega, nspin=1, nloop=100, scf_tol=1e-6, log=None):
+    """The self-consistency loop of dmft.f90:60-104 on the Bethe lattice
+    (half bandwidth 1): per iteration, from the second on, fit the bath to
+    the previous iteration, then D_cl, the impurity solve, G_loc, and the
+    convergence test.  solve(em, ek, vmk) -> G_cl(nw, norb, nspin) is the
+    impurity solver (ground state + cluster Green's function).  Returns a
+    dict with the final em/ek/vmk, D_cl, G_cl, G_loc and the history of
+    the convergence measure."""
+    em, ek, vmk = (np.array(a, dtype=np.float64) for a in (em, ek, vmk))
+    hist = []
+    D_cl = G_cl = G_loc = None
+    converged = False
+    for it in range(1, nloop + 1):
+        if it > 1:
+            em, ek, vmk, _ = fit_bath(em, ek, vmk, D_cl, G_cl, G_loc, omega, nspin=nspin)
+        D_cl = cluster_hybridization_ftn(ek, vmk, omega, nspin=nspin)
+        G_cl = solve(em, ek, vmk)
+        G_loc = local_green_ftn(em, D_cl, G_cl)
+        d = scf_difference(G_cl, G_loc, ek.shape[0])
+        hist.append(d)
+        if log:
+            log(f"DMFT iteration {it}: |G_cl - G_loc|^2 = {d:.3e}")
+        if d < scf_tol:
+            converged = True
+            break
+    return dict(em=em, ek=ek, vmk=vmk, D_cl=D_cl, G_cl=G_cl, G_loc=G_loc, history=hist,
+                converged=converged, iterations=len(hist))
