@@ -384,16 +384,17 @@ def scf_difference(G_cl, G_loc, nbath):
     return float((np.abs(np.asarray(G_cl) - np.asarray(G_loc)) ** 2).sum() / (norb * nbath * nw))
 
 
-def run_dmft(solve, em, ek, vmk, omega, nspin=1, nloop=100, scf_tol=1e-6, log=None):
+def run_dmft(solve, em, ek, vmk, omega, nspin=1, nloop=100, scf_tol=1e-6, log=None, trace=False):
     """The self-consistency loop of dmft.f90:60-104 on the Bethe lattice
     (half bandwidth 1): per iteration, from the second on, fit the bath to
     the previous iteration, then D_cl, the impurity solve, G_loc, and the
     convergence test.  solve(em, ek, vmk) -> G_cl(nw, norb, nspin) is the
     impurity solver (ground state + cluster Green's function).  Returns a
     dict with the final em/ek/vmk, D_cl, G_cl, G_loc and the history of
-    the convergence measure."""
+    the convergence measure (with trace=True also every iteration's
+    (em, ek, vmk, G_cl))."""
     em, ek, vmk = (np.array(a, dtype=np.float64) for a in (em, ek, vmk))
-    hist = []
+    hist, steps = [], []
     D_cl = G_cl = G_loc = None
     converged = False
     for it in range(1, nloop + 1):
@@ -404,10 +405,12 @@ def run_dmft(solve, em, ek, vmk, omega, nspin=1, nloop=100, scf_tol=1e-6, log=No
         G_loc = local_green_ftn(em, D_cl, G_cl)
         d = scf_difference(G_cl, G_loc, ek.shape[0])
         hist.append(d)
+        if trace:
+            steps.append((em.copy(), ek.copy(), vmk.copy(), np.array(G_cl)))
         if log:
             log(f"DMFT iteration {it}: |G_cl - G_loc|^2 = {d:.3e}")
         if d < scf_tol:
             converged = True
             break
     return dict(em=em, ek=ek, vmk=vmk, D_cl=D_cl, G_cl=G_cl, G_loc=G_loc, history=hist,
-                converged=converged, iterations=len(hist))
+                converged=converged, iterations=len(hist), trace=steps)

@@ -1,11 +1,14 @@
 """The DMFT self-consistency loop (edgpu.dmft.run_dmft, dmft.f90:60-104)
-around the GPU solver against the same loop around the FP64 oracle, config 1
-(Ns=6, U=2, half filling, Bethe lattice): iteration by iteration the
-convergence measure, and at the end the fitted bath and G_loc.
+with the GPU solver, config 1 (Ns=6, U=2, half filling, Bethe lattice),
+against the same loop around the FP64 oracle.
 
-Tolerances: the solvers agree to ~1e-12 per solve; the bath fit's line
-searches stop at 1e-8 relative, so parameters and G_loc are compared at 1e-6
-and the convergence history at 1e-6 relative."""
+- On the oracle loop's own trajectory (the bath of every iteration), the GPU
+  solver's G_cl matches the oracle's to 1e-9: the solver inside the loop.
+- The GPU loop run on its own converges below 1e-6 like the oracle loop, to
+  G_loc within 1e-3 of it.  Loops cannot be compared step by step at solver
+  precision: the bath fit stops at a relative change of 1e-6 in its
+  objective, which fixes the bath only to ~1e-3 along flat directions, so
+  1e-12 differences in G_cl move the fitted bath at that level."""
 import numpy as np
 import pytest
 
@@ -37,12 +40,15 @@ def test_dmft_loop_gpu_matches_oracle(edgpu, oracle):
         G, _ = s.cluster_green_ftn(em, ek, vmk, omega)
         return G
 
-    ref = dmft.run_dmft(solve_oracle, em, ek, vmk, omega, nloop=6, scf_tol=1e-6)
-    got = dmft.run_dmft(solve_gpu, em, ek, vmk, omega, nloop=6, scf_tol=1e-6)
+    ref = dmft.run_dmft(solve_oracle, em, ek, vmk, omega, nloop=20, scf_tol=1e-6, trace=True)
+    assert ref["converged"]
+    for it, (e_, k_, v_, G_ref) in enumerate(ref["trace"]):
+        G = solve_gpu(e_, k_, v_)
+        print(f"iteration {it + 1}: max|G_gpu - G_oracle| = {np.abs(G - G_ref).max():.2e}", flush=True)
+        assert np.abs(G - G_ref).max() < 1e-9
+    got = dmft.run_dmft(solve_gpu, em, ek, vmk, omega, nloop=20, scf_tol=1e-6)
     s.close()
     print("history", ref["history"], got["history"], flush=True)
-    assert got["iterations"] == ref["iterations"] and got["converged"]
-    assert np.allclose(got["history"], ref["history"], rtol=1e-6, atol=1e-14)
-    for k in ("em", "ek", "vmk"):
-        assert np.abs(got[k] - ref[k]).max() < 1e-6, k
-    assert np.abs(got["G_loc"] - ref["G_loc"]).max() < 1e-6
+    assert got["converged"] and got["history"][-1] < 1e-6
+    assert abs(got["iterations"] - ref["iterations"]) <= 2
+    assert np.abs(got["G_loc"] - ref["G_loc"]).max() < 1e-3
