@@ -25,7 +25,11 @@ The same JSON line also carries:
             against the wall clock for --serve-seconds: solves/s, p50/p99
             latency (result time - arrival time);
   kanamori  the two-orbital Kanamori Ns=14 problem, FP32 vs FP64 vectors:
-            solve times, |dE0|, max|dG|, H.v time.
+            solve times, |dE0|, max|dG|, H.v time, and the oracle's ms per
+            Lanczos step on the host cores;
+  dmft_config1  the Ns=6 problem through the DMFT loop (bath fit on the
+            host) with the FP64 GPU solver and with the oracle: iterations,
+            time per solve and per iteration.
 
 Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1]
 """
@@ -89,6 +93,74 @@ def cpu_baseline(em, ek, vmk, U, mu, nsteps, threads):
                 sample=f"{nsteps} FP64 Lanczos steps of the oracle at Ns=14 sector (7,7) "
                        f"(D=11778624), {dt * 1e3:.1f} ms/step, scaled to {hv_per_solve} "
                        f"steps per solve", ms_per_lanczos_step=dt * 1e3)
+
+
+def cpu_kanamori_step_ms(W, nsteps, threads):
+    """Oracle FP64 Lanczos steps of the two-orbital Kanamori problem (Ns=14,
+    sector (7,7)): ms per step on the host cores."""
+    from oracle import oracle as O
+    O.build()
+    O.set_threads(threads)
+    em, ek, vmk, mu, J, Up = W.two_orbital_kanamori(6, 3.0, seed=1)
+    m = O.Model(2, 12, U=3.0, J=J, Up=Up, Jp=J, mu=mu)
+    b = O.Basis(14, 7, 7)
+    v0 = O.start_vector(1, b.ntot)
+    O.lanczos_iteration(m, em, ek, vmk, b, v0, 2)
+    t = time.perf_counter()
+    O.lanczos_iteration(m, em, ek, vmk, b, v0, nsteps)
+    return (time.perf_counter() - t) / nsteps * 1e3
+
+
+def bench_dmft_config1(edgpu, W, cpu, threads):
+    """Config 1: the Ns=6 single-orbital Bethe impurity (U=2, half filling,
+    2048 Matsubara points, beta=100) through the DMFT loop (edgpu.dmft.run_dmft)
+    with the FP64 GPU solver; with cpu, the same loop with the oracle solver
+    on the host cores.  Times per impurity solve and per DMFT iteration."""
+    from edgpu import dmft
+    om = W.matsubara(2048, 100.0)
+    em, ek, vmk, mu = W.single_orbital(5, 2.0, 1)
+    s = edgpu.Solver(norb=1, nbath=5, U=2.0, mu=mu, maxnstep=80, precision="fp64", seed=1)
+    solves = []
+
+    def gpu_solve(em, ek, vmk):
+        t = time.perf_counter()
+        s.ground_state(em, ek, vmk, [(3, 3)])
+        G, _ = s.cluster_green_ftn(em, ek, vmk, om)
+        solves.append(time.perf_counter() - t)
+        return G
+
+    gpu_solve(em, ek, vmk)  # warm-up
+    solves.clear()
+    t = time.perf_counter()
+    r = dmft.run_dmft(gpu_solve, em, ek, vmk, om, nloop=20, scf_tol=1e-6)
+    wall = time.perf_counter() - t
+    s.close()
+    out = dict(workload="config 1: single-orbital Bethe impurity Ns=6, sector (3,3), D=400, U=2, "
+                        "half filling, 2048 Matsubara points, beta=100; DMFT loop (bath fit on the "
+                        "host) to |G_cl - G_loc|^2 < 1e-6",
+               iterations=r["iterations"], converged=r["converged"], final_scf=r["history"][-1],
+               gpu_solve_ms=float(np.median(solves)) * 1e3, loop_s=wall,
+               loop_ms_per_iteration=wall / r["iterations"] * 1e3)
+    if cpu:
+        from oracle import oracle as O
+        O.build()
+        O.set_threads(threads)
+        m = O.Model(1, 5, U=2.0, mu=mu)
+        cs = []
+
+        def cpu_solve(em, ek, vmk):
+            t = time.perf_counter()
+            gs = O.ground_state(m, em, ek, vmk, [(3, 3)], 80, seed=1)
+            G, _ = O.cluster_green_ftn(m, em, ek, vmk, gs, 80, om)
+            cs.append(time.perf_counter() - t)
+            return G
+
+        cpu_solve(em, ek, vmk)
+        cs.clear()
+        rc = dmft.run_dmft(cpu_solve, em, ek, vmk, om, nloop=20, scf_tol=1e-6)
+        out.update(cpu_solve_ms=float(np.median(cs)) * 1e3, cpu_iterations=rc["iterations"],
+                   cpu_cores=threads, cpu_kind="port")
+    return out
 
 
 def profile_hv(edgpu, em, ek, vmk, U, mu):
@@ -269,14 +341,20 @@ def main():
     if args.serve_seconds > 0:
         log("served stream")
         extra["served"] = bench_served(edgpu, W, S, args.serve_seconds, graphs=not args.no_graphs)
+    threads = min(os.cpu_count() or 1, 16)
     if not args.no_kanamori:
         log("two-orbital Kanamori")
         extra["kanamori"] = bench_kanamori(edgpu, W)
+        if not args.no_cpu_baseline:
+            ms = cpu_kanamori_step_ms(W, 3, threads)
+            extra["kanamori"].update(cpu_ms_per_lanczos_step=ms, cpu_cores=threads,
+                                     cpu_solves_per_s=1e3 / (ms * (2 * 200 + 1 + 8 * 150)))
+    log("config 1 DMFT loop")
+    extra["dmft_config1"] = bench_dmft_config1(edgpu, W, not args.no_cpu_baseline, threads)
 
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle)")
-        threads = min(os.cpu_count() or 1, 16)
         cpu = cpu_baseline(em, ek, vmk, U, mu, args.cpu_steps, threads)
 
     lat_ms = np.array(lat) * 1e3
