@@ -194,16 +194,23 @@ def write_sigma(omega, Sigma, directory="."):
 # published Numerical Recipes method, written here for numpy.
 
 class _FitProblem:
+    """Objective and gradient of one spin's fit in real arithmetic:
+    1/(i w - e) = (-e - i w) q with q = 1/(w^2 + e^2), so D, diff and the
+    derivatives are a few real matrix products over (nw, nbath) arrays."""
+
     def __init__(self, em, ek, vmk, D_cl, G_cl, G_loc, omega, ispin):
         self.norb, self.nbath = vmk.shape[0], vmk.shape[1]
         self.s = ispin
         self.w = np.asarray(omega, dtype=np.float64)
-        self.iw = 1j * self.w
+        self.w2 = self.w * self.w
+        self.rw = 1.0 / self.w                      # weight 1/w_n (:215-219)
         self.em_old = np.array(em[:, ispin], dtype=np.float64)
         # the part of diff that does not depend on x
-        self.c = (-self.em_old[None, :] - D_cl[:, :, ispin] + 1.0 / G_loc[:, :, ispin]
-                  - 1.0 / G_cl[:, :, ispin])
+        c = (-self.em_old[None, :] - D_cl[:, :, ispin] + 1.0 / G_loc[:, :, ispin]
+             - 1.0 / G_cl[:, :, ispin])
+        self.cr, self.ci = np.ascontiguousarray(c.real), np.ascontiguousarray(c.imag)
         self.nw = self.w.size
+        self.scale = 1.0 / (self.norb * self.nw)
 
     def split(self, x):
         no, nb = self.norb, self.nbath
@@ -212,32 +219,31 @@ class _FitProblem:
         v = x[no + nb:].reshape(nb, no).T  # vmk(iorb, ibath): iorb fastest in x
         return em, ek, v
 
-    def diff(self, x):
+    def _diff(self, x):
         em, ek, v = self.split(x)
-        den = self.iw[:, None] - ek[None, :]                       # (nw, nbath)
-        D = (v[None, :, :] ** 2 / den[:, None, :]).sum(axis=2)     # (nw, norb)
-        return em[None, :] + D + self.c, den
+        q = 1.0 / (self.w2[:, None] + (ek * ek)[None, :])          # (nw, nbath)
+        v2 = (v * v).T                                             # (nbath, norb)
+        dr = em[None, :] - (q * ek[None, :]) @ v2 + self.cr        # Re diff (nw, norb)
+        di = -self.w[:, None] * (q @ v2) + self.ci                 # Im diff
+        return em, ek, v, q, dr, di
 
     def f(self, x):
-        d, _ = self.diff(x)
-        return float(((np.abs(d) ** 2) / self.w[:, None]).sum() / (self.norb * self.nw))
+        _, _, _, _, dr, di = self._diff(x)
+        return float((self.rw @ (dr * dr + di * di)).sum() * self.scale)
 
     def df(self, x):
-        em, ek, v = self.split(x)
-        d, den = self.diff(x)
-        wt = 1.0 / self.w[:, None]
-        g_em = (2.0 * wt * d.real).sum(axis=0)
-        # d D_m / d ek_b = v_mb^2 / (iw - ek_b)^2 ; d D_m / d v_mb = 2 v_mb / (iw - ek_b)
-        # d|d|^2 = 2 Re(d conj(dd))
-        inv = 1.0 / den                                            # (nw, nbath)
-        g_ek = np.zeros(self.nbath)
-        g_v = np.zeros((self.norb, self.nbath))
-        for m in range(self.norb):
-            dm = d[:, m][:, None]
-            g_ek += (2.0 * wt * (dm * np.conj(v[m][None, :] ** 2 * inv ** 2)).real).sum(axis=0)
-            g_v[m] = (2.0 * wt * (dm * np.conj(2.0 * v[m][None, :] * inv)).real).sum(axis=0)
-        g = np.concatenate([g_em, g_ek, g_v.T.reshape(-1)])
-        return g / (self.norb * self.nw)
+        em, ek, v, q, dr, di = self._diff(x)
+        drw, diw = dr * self.rw[:, None], di * self.rw[:, None]    # d / w_n
+        g_em = 2.0 * drw.sum(axis=0)
+        # dD_m/de_b = v^2 (e^2 - w^2 + 2 i e w) q^2 ; dD_m/dv_mb = 2 v (-e - i w) q
+        q2 = q * q
+        re_e = (ek * ek)[None, :] * q2 - self.w2[:, None] * q2    # (nw, nbath)
+        im_e = 2.0 * self.w[:, None] * ek[None, :] * q2
+        A = drw.T @ re_e + diw.T @ im_e                            # (norb, nbath)
+        g_ek = 2.0 * (A * (v * v)).sum(axis=0)
+        B = drw.T @ (q * ek[None, :]) + diw.T @ (q * self.w[:, None])
+        g_v = -4.0 * v * B
+        return np.concatenate([g_em, g_ek, g_v.T.reshape(-1)]) * self.scale
 
 
 def _mnbrak(f1, ax, bx, gold=1.618034, glimit=100.0, tiny=1e-20):
