@@ -752,15 +752,14 @@ int chain_class(const BInst &I)
 {
     return (I.wide ? (I.cb > 1 ? 2 : 4) : 0) + (I.op.up.KC == 16 ? 1 : 0);
 }
-bool class_wide(int c) { return c >= 2; }
 int class_cols(int c) { return c >= 4 ? 1 : (c >= 2 ? kBatchWideCols : 0); }
 
 // Upload the descriptor table of the active chains grouped by launch class,
 // with the tile prefix sums of each class and the axpy prefix sums over all
-// chains.  In the long-column classes every chain's tiles start at a
-// multiple of 8 (kXcds): workgroup b runs on XCD b % 8, so tile t of a chain
-// runs on XCD t % 8 and the kernel can deal the chain's column groups to the
-// XCDs in contiguous ranges.  The padding tiles exit at once.
+// chains.  Every chain's tiles start at a multiple of 8 (kXcds): workgroup b
+// runs on XCD b % 8, so tile t of a chain runs on XCD t % 8 and the kernels
+// deal the chain's column groups to the XCDs in contiguous ranges.  The
+// padding tiles exit at once.
 int rebuild(edg_batch *b)
 {
     std::vector<Chain *> order;
@@ -780,7 +779,7 @@ int rebuild(edg_batch *b)
         hv[c].assign(start[c + 1] - start[c] + 1, 0);
         for (int i = start[c]; i < start[c + 1]; ++i) {
             const BInst &I = order[i]->bi;
-            const int tiles = class_wide(c) ? (I.hvn + 7) / 8 * 8 : I.hvn;
+            const int tiles = (I.hvn + 7) / 8 * 8;  // every class: XCD-aligned tile offsets
             hv[c][i - start[c] + 1] = hv[c][i - start[c]] + tiles;
             lds[c] = std::max(lds[c], I.cb * I.op.nup);
         }

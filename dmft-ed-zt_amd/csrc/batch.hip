@@ -41,11 +41,13 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, 
 
     const int i = own[blockIdx.x];  // chain of this workgroup
     const BInst &I = inst[i];
-    if (I.flags[0]) return;
     const int t = blockIdx.x - hv_off[i];
+    if (I.flags[0] || t >= I.hvn) return;  // stopped chain, or XCD padding tile
     const HvOp op = I.op;  // private copy: stores to y cannot alias it
     const int nup = op.nup, ndn = op.ndn, cb = I.cb;
-    const int c0 = t * cb;
+    // tile t runs on XCD t % 8: contiguous column ranges per XCD keep the
+    // down-hop neighbour columns in that XCD's L2
+    const int c0 = xcd_group(t, I.hvn) * cb;
     const int ncol = ndn - c0 < cb ? ndn - c0 : cb;
     constexpr int kcd = KS;
     const int j = uniform(*I.step);
