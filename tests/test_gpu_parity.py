@@ -73,7 +73,8 @@ def test_spin_tables_bit_exact(edgpu, oracle, norb, nbath, ne):
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 2e-6)])
 @pytest.mark.parametrize("norb,nbath,nu,nd", [(1, 5, 3, 3), (1, 5, 2, 4), (1, 9, 5, 5), (2, 4, 3, 3),
-                                              (2, 6, 4, 3), (2, 8, 5, 5), (3, 3, 3, 3), (1, 0, 1, 0)])
+                                              (2, 6, 4, 3), (2, 8, 5, 5), (3, 3, 3, 3), (1, 0, 1, 0),
+                                              (4, 2, 3, 2), (5, 2, 4, 3), (1, 5, 0, 3), (1, 5, 6, 2)])
 def test_hx_parity(edgpu, oracle, precision, tol, norb, nbath, nu, nd):
     print(f"hx {precision} norb={norb} nbath={nbath} sector=({nu},{nd})", flush=True)
     rng = np.random.default_rng(100 + nbath + nu)
@@ -268,6 +269,32 @@ def test_kanamori_two_orbital_vs_oracle(edgpu, oracle):
         occ, docc = s.observables()
         o2, d2 = oracle.observables(m, ogs.basis, ogs.vec)
         assert np.abs(occ - o2).max() < etol * 100 and np.abs(docc - d2).max() < etol * 100
+
+
+@pytest.mark.parametrize("norb,nbath,sec", [(3, 3, (3, 3)), (4, 2, (3, 2))])
+def test_multi_orbital_solve_vs_oracle(edgpu, oracle, norb, nbath, sec):
+    """Three and four orbitals (general Kanamori H.v, apply_c on every
+    orbital, G per orbital): E0, G_cl of every orbital and both spins,
+    occupancies and double occupancy against the oracle, FP64 vectors.
+    G to 1e-7 and occupancies to 1e-8 (as for two orbitals): 120-step
+    chains in sectors of 225-400 states run past convergence, where Lanczos
+    amplifies summation-order rounding."""
+    print(f"solve norb={norb} nbath={nbath} sector={sec}", flush=True)
+    rng = np.random.default_rng(40 + norb)
+    em, ek, vmk = rand_params(rng, norb, nbath)
+    s, m = make(edgpu, oracle, norb, nbath, U=2.5, J=0.4, mu=1.2, maxnstep=120, nspin=2)
+    gs = s.ground_state(em, ek, vmk, [sec])
+    ogs = oracle.ground_state(m, em, ek, vmk, [sec], maxnstep=120)
+    assert abs(gs.val - ogs.val) < 1e-10
+    om = oracle.matsubara(128, 50.0)
+    G, _ = s.cluster_green_ftn(em, ek, vmk, om)
+    G2, _ = oracle.cluster_green_ftn(m, em, ek, vmk, ogs, 120, om, nspin=2)
+    assert G.shape == (128, norb, 2)
+    assert np.abs(G - G2).max() < 1e-7
+    occ, docc = s.observables()
+    o2, d2 = oracle.observables(m, ogs.basis, ogs.vec)
+    assert np.abs(occ - o2).max() < 1e-8 and np.abs(docc - d2).max() < 1e-8
+    s.close()
 
 
 def test_nspin2_outputs_both_spins(edgpu, oracle):

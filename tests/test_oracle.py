@@ -78,7 +78,8 @@ def test_row_order_up_fastest(oracle):
 
 @pytest.mark.parametrize("norb,nbath,nu,nd,sym", [
     (1, 3, 2, 2, False), (1, 5, 3, 3, True), (1, 4, 1, 4, False), (2, 2, 2, 2, False),
-    (2, 3, 3, 2, False), (2, 2, 1, 3, True), (3, 1, 2, 2, False), (3, 2, 3, 2, False)])
+    (2, 3, 3, 2, False), (2, 2, 1, 3, True), (3, 1, 2, 2, False), (3, 2, 3, 2, False),
+    (4, 2, 3, 2, False), (5, 2, 4, 3, False), (1, 5, 0, 3, False), (1, 5, 6, 2, False)])
 def test_hx_matches_jordan_wigner(oracle, dense, norb, nbath, nu, nd, sym):
     rng = np.random.default_rng(10 * norb + nbath + nu)
     em, ek, vmk = rand_params(rng, norb, nbath, sym)
