@@ -260,9 +260,12 @@ int chain_hvn(const edg_batch *b, int nsite, long long nup, long long ndn)
     return (int)((ndn + sh.cb - 1) / sh.cb);
 }
 
+// Axpy workgroups of a chain: 32 elements per thread.  Each workgroup first
+// reduces the chain's H.v partials, and every H.v tile of the next step
+// reduces the chain's axpy partials, so fewer, fuller workgroups cut both.
 int chain_axn(long long D)
 {
-    long long g = (D + kBlock * 8 - 1) / (kBlock * 8);
+    long long g = (D + kBlock * 32 - 1) / (kBlock * 32);
     return (int)std::min<long long>(std::max<long long>(g, 1), 1024);
 }
 
