@@ -27,7 +27,7 @@ The same JSON line also carries:
   kanamori  the two-orbital Kanamori Ns=14 problem, FP32 vs FP64 vectors:
             solve times, |dE0|, max|dG|, H.v time, and the oracle's ms per
             Lanczos step on the host cores;
-  dmft_config1  the Ns=6 problem through the DMFT loop (bath fit on the
+  dmft_ns6      the Ns=6 problem through the DMFT loop (bath fit on the
             host) with the FP64 GPU solver and with the oracle: iterations,
             time per solve and per iteration.
 
@@ -111,8 +111,8 @@ def cpu_kanamori_step_ms(W, nsteps, threads):
     return (time.perf_counter() - t) / nsteps * 1e3
 
 
-def bench_dmft_config1(edgpu, W, cpu, threads):
-    """Config 1: the Ns=6 single-orbital Bethe impurity (U=2, half filling,
+def bench_dmft_ns6(edgpu, W, cpu, threads):
+    """The Ns=6 single-orbital Bethe impurity (U=2, half filling,
     2048 Matsubara points, beta=100) through the DMFT loop (edgpu.dmft.run_dmft)
     with the FP64 GPU solver; with cpu, the same loop with the oracle solver
     on the host cores.  Times per impurity solve and per DMFT iteration."""
@@ -135,7 +135,7 @@ def bench_dmft_config1(edgpu, W, cpu, threads):
     r = dmft.run_dmft(gpu_solve, em, ek, vmk, om, nloop=20, scf_tol=1e-6)
     wall = time.perf_counter() - t
     s.close()
-    out = dict(workload="config 1: single-orbital Bethe impurity Ns=6, sector (3,3), D=400, U=2, "
+    out = dict(workload="single-orbital Bethe impurity Ns=6, sector (3,3), D=400, U=2, "
                         "half filling, 2048 Matsubara points, beta=100; DMFT loop (bath fit on the "
                         "host) to |G_cl - G_loc|^2 < 1e-6",
                iterations=r["iterations"], converged=r["converged"], final_scf=r["history"][-1],
@@ -349,8 +349,8 @@ def main():
             ms = cpu_kanamori_step_ms(W, 3, threads)
             extra["kanamori"].update(cpu_ms_per_lanczos_step=ms, cpu_cores=threads,
                                      cpu_solves_per_s=1e3 / (ms * (2 * 200 + 1 + 8 * 150)))
-    log("config 1 DMFT loop")
-    extra["dmft_config1"] = bench_dmft_config1(edgpu, W, not args.no_cpu_baseline, threads)
+    log("Ns=6 DMFT loop")
+    extra["dmft_ns6"] = bench_dmft_ns6(edgpu, W, not args.no_cpu_baseline, threads)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -5,7 +5,7 @@ so these fixtures are computed here WITHOUT the C oracle and without the HIP
 library: explicit Jordan-Wigner fermion operators (oracle/dense.py), numpy
 eigh and the Lehmann sum.  They pin both the oracle and the GPU path.
 
-  config1_ns6.npz   single-orbital Bethe bath, Nbath=5 (Ns=6), U=2, half
+  bethe_ns6.npz     single-orbital Bethe bath, Nbath=5 (Ns=6), U=2, half
                     filling sector (3,3), D=400, seed 2024: E0, gs occupancy
                     and double occupancy, G(i w_n) on the 2048-point beta=100
                     grid from the Lehmann sum, and the sum-rule weights.
@@ -34,7 +34,7 @@ from oracle import dense as Dn  # noqa: E402
 from edgpu import workloads as W  # noqa: E402
 
 
-def config1():
+def bethe_ns6():
     em, ek, vmk, mu = W.single_orbital(5, 2.0, seed=2024)
     U, J = 2.0, 0.3
     Up, Jp = U - 2 * J, J
@@ -48,7 +48,7 @@ def config1():
     n_dn = float(np.sum(gs ** 2 * (dn & 1)))
     docc = float(np.sum(gs ** 2 * (up & 1) * (dn & 1)))
     G, E0, wp, wm = Dn.lehmann_green(1, 5, em, ek, vmk, U, Up, J, Jp, mu, 3, 3, 1, 1, om)
-    np.savez(os.path.join(HERE, "config1_ns6.npz"), em=em, ek=ek, vmk=vmk, mu=mu, U=U, J=J,
+    np.savez(os.path.join(HERE, "bethe_ns6.npz"), em=em, ek=ek, vmk=vmk, mu=mu, U=U, J=J,
              omega=om, E0=w[0], E1=w[1], n_up=n_up, n_dn=n_dn, docc=docc, G=G, w_plus=wp,
              w_minus=wm)
 
@@ -79,7 +79,7 @@ def tables():
 
 
 if __name__ == "__main__":
-    config1()
+    bethe_ns6()
     kanamori()
     tables()
     print("fixtures written to", HERE)

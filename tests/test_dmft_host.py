@@ -18,7 +18,7 @@ construction rather than by restating its formula.
   fit_bath                   analytic gradient against central differences;
                              recovers a bath whose hybridization is the exact
                              minimizer (G_loc built from it by Dyson's equation)
-  run_dmft                   config 1 (Ns=6, U=2, half filling) converges below
+  run_dmft                   the Ns=6 problem (U=2, half filling) converges below
                              the reference's default tolerance (1e-6) within 20
                              iterations to a particle-hole symmetric bath
                              (em = 0, Re Delta(i w) = 0); for U = 0 it reaches
@@ -212,8 +212,8 @@ def test_fit_recovers_known_bath():
 
 
 @pytest.mark.parametrize("U", [2.0, 0.0])
-def test_dmft_loop_config1(oracle, U):
-    """dmft.f90:60-104 around the FP64 oracle solver: config 1 (Ns=6, half
+def test_dmft_loop_ns6(oracle, U):
+    """dmft.f90:60-104 around the FP64 oracle solver: the single-orbital Ns=6 problem (half
     filling, Bethe lattice of half bandwidth 1) on a 512-point grid."""
     nw = 512
     omega = (2 * np.arange(nw) + 1) * np.pi / 100.0

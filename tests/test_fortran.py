@@ -1,5 +1,5 @@
 """The Fortran side of the boundary: the iso_c_binding module compiles with
-amdflang and the example caller (built by build()) solves config 1 on the
+amdflang and the example caller (built by build()) solves the Ns=6 problem on the
 GPU through libedgpu.so with the golden fixture's numbers."""
 import os
 import shutil
@@ -21,11 +21,11 @@ def test_interface_module_compiles(tmp_path):
 
 
 @pytest.mark.gpu
-def test_fortran_caller_solves_config1():
+def test_fortran_caller_solves_ns6():
     exe = os.path.join(FDIR, "example_solve")
     if not os.path.exists(exe):
         pytest.fail("example_solve not built: run __graft_entry__.build()")
-    g = np.load(os.path.join(ROOT, "tests", "golden", "config1_ns6.npz"))
+    g = np.load(os.path.join(ROOT, "tests", "golden", "bethe_ns6.npz"))
     inp = " ".join(f"{v:.17e}" for v in g["ek"][:, 0]) + "\n" + \
           " ".join(f"{v:.17e}" for v in g["vmk"][0, :, 0]) + "\n"
     out = subprocess.run([exe], input=inp, capture_output=True, text=True, timeout=120)

@@ -1,5 +1,5 @@
 """The DMFT self-consistency loop (edgpu.dmft.run_dmft, dmft.f90:60-104)
-with the GPU solver, config 1 (Ns=6, U=2, half filling, Bethe lattice),
+with the GPU solver, the single-orbital Ns=6 problem (U=2, half filling, Bethe lattice),
 against the same loop around the FP64 oracle.
 
 - On the oracle loop's own trajectory (the bath of every iteration), the GPU

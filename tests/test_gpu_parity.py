@@ -183,8 +183,8 @@ def test_lanczos_breakdown_exact_eigenvector(edgpu, oracle, dense):
 # Occupancies are first order in the eigenvector error; with FP32-stored
 # Krylov vectors that error is ~ eps32 * |H| / gap, hence 1e-5 for FP32.
 @pytest.mark.parametrize("precision,etol,otol", [("fp64", 1e-10, 1e-10), ("fp32", 1e-6, 1e-5)])
-def test_ground_state_config1_golden(edgpu, oracle, precision, etol, otol):
-    g = np.load(os.path.join(HERE, "golden", "config1_ns6.npz"))
+def test_ground_state_ns6_golden(edgpu, oracle, precision, etol, otol):
+    g = np.load(os.path.join(HERE, "golden", "bethe_ns6.npz"))
     s = edgpu.Solver(norb=1, nbath=5, U=float(g["U"]), J=float(g["J"]), mu=float(g["mu"]),
                      maxnstep=200, precision=precision, seed=9)
     gs = s.ground_state(g["em"], g["ek"], g["vmk"], [(3, 3)])
@@ -332,7 +332,7 @@ def test_error_paths(edgpu):
     assert np.isfinite(gs.val)
 
 
-# ----------------------------------------------- Ns=14 (config 2 shape)
+# ----------------------------------------------- Ns=14, sector (7,7)
 
 def test_ns14_properties_fp32_vs_fp64(edgpu, oracle):
     """Config-2 shape (Ns=14, sector (7,7), D=11,778,624): hx against the

@@ -261,8 +261,8 @@ def test_green_spin_asymmetric_average(oracle, dense):
     assert np.abs(G[:, 0, 0] - 0.5 * (g1 + g2)).max() < 1e-10
 
 
-def test_golden_config1(oracle):
-    g = np.load(os.path.join(HERE, "golden", "config1_ns6.npz"))
+def test_golden_bethe_ns6(oracle):
+    g = np.load(os.path.join(HERE, "golden", "bethe_ns6.npz"))
     m = oracle.Model(1, 5, U=float(g["U"]), J=float(g["J"]), mu=float(g["mu"]))
     gs = oracle.ground_state(m, g["em"], g["ek"], g["vmk"], [(3, 3)], maxnstep=200, seed=9)
     assert abs(gs.val - float(g["E0"])) < 1e-11
