@@ -427,36 +427,48 @@ __global__ __launch_bounds__(kHvBlock) void k_hv1(HvOp op, HvArgs a)
 #include "hv1_body.inc"
 }
 
-// Two-orbital (Kanamori) H.v, one sector column per 512-thread workgroup.
-// Besides the column idn itself, the workgroup stages in LDS the target
-// columns of the column's down-spin orbital moves: for each ordered orbital
-// pair (i, j) the spin-flip term moves the down electron i -> j and the
-// pair-hop term j -> i, and a down pattern allows at most one of the two,
-// so each pair contributes at most one staged column.  The spin-flip and
+// Two-orbital (Kanamori) H.v over the sector columns [idn0, idn0 + C), one
+// 512-thread workgroup per column group (XCD-aware group order).  For each
+// ordered orbital pair (i, j) the spin-flip term moves the down electron
+// i -> j and the pair-hop term j -> i; with two orbitals a down pattern
+// whose impurity part is (1,0) or (0,1) allows exactly one move per pair and
+// both pairs lead to the same target column, (1,1) and (0,0) allow none.  So
+// each column has at most one orbital-move target column, staged in LDS next
+// to the column itself (zeros when there is none), and the spin-flip and
 // pair-hop elements +-Jp * s_up * s_dn (impurity_hamiltonian.f90:176-230)
-// then gather from LDS at the up-move target row.  Up hops use the 32-bit
-// records (KS slots stored, KH processed) gathered from the staged column;
-// down hops are coalesced loads of neighbour columns at wave-uniform
-// offsets.  The diagonal adds U n_m,up n_m,dn and U' for unlike orbitals of
-// opposite spin to the one-spin energies (which carry U' - J for like spins).
+// gather from it at the up-move target row.  Columns are row-interleaved in
+// LDS, xs[i*C + c], so a gather reads all C columns at once, and the per-row
+// loads (up-hop records, up moves, mask, energy) serve all C columns.  Up
+// hops use the 32-bit records (KS slots stored, KH processed); down hops are
+// coalesced loads of neighbour columns at wave-uniform offsets.  The
+// diagonal adds U n_m,up n_m,dn and U' for unlike orbitals of opposite spin
+// to the one-spin energies (which carry U' - J for like spins).
 constexpr int kNorb2 = 2;
 constexpr int kPairs2 = kNorb2 * (kNorb2 - 1);
 
-template <typename T, typename ACC, int MODE, int KH>
+// Dynamic LDS of k_hv2: the C columns and their C target columns.
+__host__ __device__ constexpr size_t hv2_lds(int nup, int C, size_t esize)
+{
+    return 2 * align16((size_t)nup * C * esize);
+}
+
+template <typename T, typename ACC, int C, int MODE, int KH>
 __global__ __launch_bounds__(kHvBlock) void k_hv2(HvOp op, HvArgs a)
 {
     constexpr int KS = KH <= 8 ? 8 : 16;
+    constexpr int kStage = C == 1 ? 4 : 2;  // rows per thread per staging pass
     if (a.stop && *a.stop) return;
     using Hop = typename HopOf<ACC>::type;
     extern __shared__ __align__(64) unsigned char smem[];
     const int nup = op.nup, ndn = op.ndn;
-    const size_t csz = align16((size_t)nup * sizeof(T));
-    T *xs = reinterpret_cast<T *>(smem);                    // the column itself
+    T *xs = reinterpret_cast<T *>(smem);                                        // the columns
+    T *xt = reinterpret_cast<T *>(smem + align16((size_t)nup * C * sizeof(T)));  // their targets
     __shared__ ACC s_v[2 * (kMaxHops + 1)];
-    __shared__ int s_kcol[kPairs2];
-    __shared__ ACC s_kval[kPairs2];
+    __shared__ int s_kcol[C];
+    __shared__ Row<ACC, C> s_kval[kPairs2];  // +-Jp of pair p for column c, 0: no move
+    __shared__ Row<ACC, C> s_dval[KH];       // down-hop elements, slot-major
 
-    const int idn = xcd_group(blockIdx.x, ndn);
+    const int idn0 = xcd_group(blockIdx.x, (ndn + C - 1) / C) * C;
     const T *x = static_cast<const T *>(a.x);
     const unsigned char *xb = static_cast<const unsigned char *>(a.x);
     for (int e = threadIdx.x; e <= kMaxHops; e += kHvBlock) {
@@ -464,47 +476,73 @@ __global__ __launch_bounds__(kHvBlock) void k_hv2(HvOp op, HvArgs a)
         s_v[e] = v;
         s_v[kMaxHops + 1 + e] = -v;
     }
-    // down orbital moves of this column: pair p = (i, j), up move j -> i;
-    // spin flip: down i -> j; pair hop: down j -> i (at most one is valid)
-    if (threadIdx.x < kPairs2) {
-        const int p = threadIdx.x;
-        const int i = p / (kNorb2 - 1), jj = p % (kNorb2 - 1), j = jj >= i ? jj + 1 : jj;
-        const uint32_t tsf = op.dn.orb[(long long)(j * kNorb2 + i) * ndn + idn];
-        const uint32_t tph = op.dn.orb[(long long)(i * kNorb2 + j) * ndn + idn];
-        const uint32_t t = tsf != kInvalid ? tsf : tph;
-        s_kcol[p] = t != kInvalid ? (int)(t & 0x7FFFFFFFu) : -1;
-        s_kval[p] = t != kInvalid ? ((t >> 31) ? -(ACC)op.Jp : (ACC)op.Jp) : ACC(0);
-    }
-    // down hops of this column (wave-uniform)
-    uint32_t coff[KH];
-    ACC dval[KH];
-    const Hop *hr = HopOf<ACC>::tab(op.dn) + (long long)idn * op.dn.K;
+    // down orbital moves: pair p = (i, j), up move j -> i; spin flip: down
+    // i -> j; pair hop: down j -> i (at most one is valid).  Columns past
+    // ndn (last group) repeat column ndn-1 and are not stored.
+    if (threadIdx.x < C) {
+        const int c = threadIdx.x;
+        const int idn = idn0 + c < ndn ? idn0 + c : ndn - 1;
+        int col = -1;
 #pragma unroll
-    for (int k = 0; k < KH; ++k) {
-        const Hop h = hr[k];
-        coff[k] = h.col * (uint32_t)nup * (uint32_t)sizeof(T);
-        dval[k] = (ACC)h.val;
+        for (int p = 0; p < kPairs2; ++p) {
+            const int i = p / (kNorb2 - 1), jj = p % (kNorb2 - 1), j = jj >= i ? jj + 1 : jj;
+            const uint32_t tsf = op.dn.orb[(long long)(j * kNorb2 + i) * ndn + idn];
+            const uint32_t tph = op.dn.orb[(long long)(i * kNorb2 + j) * ndn + idn];
+            const uint32_t t = tsf != kInvalid ? tsf : tph;
+            s_kval[p].v[c] = t != kInvalid ? ((t >> 31) ? -(ACC)op.Jp : (ACC)op.Jp) : ACC(0);
+            if (t != kInvalid) col = (int)(t & 0x7FFFFFFFu);
+        }
+        s_kcol[c] = col;
     }
-    const ACC ed = energy_of<ACC>(op.dn, idn);
-    const int md = op.dn.mask[idn];
+    // down hops of the C columns (wave-uniform)
+    uint32_t coff[C][KH];
+    ACC ed[C];
+    int md[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        // wave-uniform (kept out of the divergent block above, whose equal
+        // expression the compiler would otherwise share and move to VGPRs)
+        const int idn = uniform(idn0 + c < ndn ? idn0 + c : ndn - 1);
+        const Hop *hr = HopOf<ACC>::tab(op.dn) + (long long)idn * op.dn.K;
+#pragma unroll
+        for (int k = 0; k < KH; ++k) {
+            const Hop h = gload(hr + k);
+            coff[c][k] = h.col * (uint32_t)nup * (uint32_t)sizeof(T);
+            if (threadIdx.x == 0) s_dval[k].v[c] = (ACC)h.val;
+        }
+        ed[c] = energy_of<ACC>(op.dn, idn);
+        md[c] = gload(op.dn.mask + idn);
+    }
     __syncthreads();  // s_kcol
-    // stage the column and the (up to kPairs2) orbital-move target columns
+    // stage the columns and their target columns, row-interleaved
+    for (int i0 = threadIdx.x; i0 < nup; i0 += kStage * kHvBlock) {
+        T v[C][kStage], w[C][kStage];
 #pragma unroll
-    for (int q = 0; q <= kPairs2; ++q) {
-        const int col = q == 0 ? idn : s_kcol[q - 1];
-        if (col < 0) continue;  // uniform
-        T *dst = reinterpret_cast<T *>(smem + (size_t)q * csz);
-        const T *src = x + (long long)col * nup;
-        for (int i0 = threadIdx.x; i0 < nup; i0 += 8 * kHvBlock) {
-            T v[8];
+        for (int c = 0; c < C; ++c) {
+            const int idn = idn0 + c < ndn ? idn0 + c : ndn - 1;
+            const T *xc = x + (long long)idn * nup;
+            const int tc = s_kcol[c];
 #pragma unroll
-            for (int r = 0; r < 8; ++r) {
-                const int e = i0 + r * kHvBlock;
-                v[r] = src[e < nup ? e : nup - 1];
+            for (int r = 0; r < kStage; ++r) {
+                const int i = i0 + r * kHvBlock;
+                const int ic = i < nup ? i : nup - 1;
+                v[c][r] = xc[ic];
+                w[c][r] = tc >= 0 ? x[(long long)tc * nup + ic] : T(0);
             }
+        }
 #pragma unroll
-            for (int r = 0; r < 8; ++r)
-                if (i0 + r * kHvBlock < nup) dst[i0 + r * kHvBlock] = v[r];
+        for (int r = 0; r < kStage; ++r) {
+            const int i = i0 + r * kHvBlock;
+            if (i < nup) {
+                Row<T, C> rv, rw;
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    rv.v[c] = v[c][r];
+                    rw.v[c] = w[c][r];
+                }
+                *reinterpret_cast<Row<T, C> *>(xs + i * C) = rv;
+                *reinterpret_cast<Row<T, C> *>(xt + i * C) = rw;
+            }
         }
     }
     __syncthreads();
@@ -516,12 +554,18 @@ __global__ __launch_bounds__(kHvBlock) void k_hv2(HvOp op, HvArgs a)
     }
     T *y = static_cast<T *>(a.y);
     const T *pv = static_cast<const T *>(a.p);
+    const bool has_p = MODE == HV_LANCZOS && pv != nullptr;
     const ACC U = (ACC)op.U, Up = (ACC)op.Up;
     for (int iup = threadIdx.x; iup < nup; iup += kHvBlock) {
+        // the LDS tables are re-read each row: hoisted out of the loop, the
+        // C * KH down-hop elements alone would hold 4 * C * KH VGPRs (FP64)
+        __asm__ volatile("" ::: "memory");
         const uint32_t ib = (uint32_t)iup * (uint32_t)sizeof(T);
-        T g[KH];
+        T g[C][KH];
 #pragma unroll
-        for (int k = 0; k < KH; ++k) g[k] = *reinterpret_cast<const T *>(xb + (coff[k] + ib));
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int k = 0; k < KH; ++k) g[c][k] = *reinterpret_cast<const T *>(xb + (coff[c][k] + ib));
         const Row<uint32_t, KS> hrow = *reinterpret_cast<const Row<uint32_t, KS> *>(op.up.hrec + (long long)iup * KS);
         uint32_t tu[kPairs2];
 #pragma unroll
@@ -529,44 +573,73 @@ __global__ __launch_bounds__(kHvBlock) void k_hv2(HvOp op, HvArgs a)
             const int i = p / (kNorb2 - 1), jj = p % (kNorb2 - 1), j = jj >= i ? jj + 1 : jj;
             tu[p] = op.up.orb[(long long)(i * kNorb2 + j) * nup + iup];  // up move j -> i
         }
-        const int mu = op.up.mask[iup];
-        const int both = __popc(mu & md);
-        const ACC diag = energy_of<ACC>(op.up, iup) + ed + U * (ACC)both +
-                         Up * (ACC)(__popc(mu) * __popc(md) - both);
-        const ACC xv = (ACC)xs[iup];
-        ACC acc = diag * xv;
+        const int mu = gload(op.up.mask + iup);
+        const ACC eu = energy_of<ACC>(op.up, iup);
+        T old[C];
+        if constexpr (MODE == HV_ACCUM || MODE == HV_LANCZOS) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int idn = idn0 + c < ndn ? idn0 + c : ndn - 1;
+                const long long e = (long long)idn * nup + iup;
+                if constexpr (MODE == HV_ACCUM) old[c] = gload(y + e);
+                else old[c] = has_p ? gload(pv + e) : T(0);
+            }
+        }
+        const Row<T, C> xr = *reinterpret_cast<const Row<T, C> *>(xs + iup * C);
+        ACC acc[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int both = __popc(mu & md[c]);
+            const ACC diag = eu + ed[c] + U * (ACC)both + Up * (ACC)(__popc(mu) * __popc(md[c]) - both);
+            acc[c] = diag * (ACC)xr.v[c];
+        }
 #pragma unroll
         for (int k = 0; k < KH; ++k) {
             const uint32_t w = hrow.v[k];
-            acc += s_v[w >> 16] * (ACC)xs[w & 0xFFFFu];
+            const ACC val = s_v[w >> 16];
+            const Row<T, C> src = *reinterpret_cast<const Row<T, C> *>(xs + (w & 0xFFFFu) * C);
+#pragma unroll
+            for (int c = 0; c < C; ++c) acc[c] += val * (ACC)src.v[c];
         }
 #pragma unroll
-        for (int k = 0; k < KH; ++k) acc += dval[k] * (ACC)g[k];
+        for (int k = 0; k < KH; ++k) {
+            const Row<ACC, C> dv = s_dval[k];
+#pragma unroll
+            for (int c = 0; c < C; ++c) acc[c] += dv.v[c] * (ACC)g[c][k];
+        }
 #pragma unroll
         for (int p = 0; p < kPairs2; ++p) {
-            if (s_kcol[p] < 0) continue;  // uniform
             const bool ok = tu[p] != kInvalid;
             const uint32_t ru = ok ? (tu[p] & 0x7FFFFFFFu) : 0u;
-            const ACC su = ok ? ((tu[p] >> 31) ? -s_kval[p] : s_kval[p]) : ACC(0);
-            const T *xk = reinterpret_cast<const T *>(smem + (size_t)(p + 1) * csz);
-            acc += su * (ACC)xk[ru];
+            const bool neg = (tu[p] >> 31) != 0;
+            const Row<ACC, C> kv = s_kval[p];
+            const Row<T, C> src = *reinterpret_cast<const Row<T, C> *>(xt + ru * C);
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const ACC su = ok ? (neg ? -kv.v[c] : kv.v[c]) : ACC(0);
+                acc[c] += su * (ACC)src.v[c];
+            }
         }
-        const long long e = (long long)idn * nup + iup;
-        if constexpr (MODE == HV_PLAIN) {
-            y[e] = (T)acc;
-        } else if constexpr (MODE == HV_ACCUM) {
-            y[e] = (T)((ACC)y[e] + acc);
-        } else if constexpr (MODE == HV_LANCZOS) {
-            double r = sx * (double)acc;
-            if (pv) r -= bsp * (double)pv[e];
-            const T rt = (T)r;
-            y[e] = rt;
-            sums[0] += (double)xv * (double)rt;
-        } else {
-            const double hv = (double)acc;
-            sums[0] += (double)xv * hv;
-            sums[1] += hv * hv;
-            sums[2] += (double)xv * (double)xv;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            if (idn0 + c >= ndn) break;
+            const long long e = (long long)(idn0 + c) * nup + iup;
+            if constexpr (MODE == HV_PLAIN) {
+                y[e] = (T)acc[c];
+            } else if constexpr (MODE == HV_ACCUM) {
+                y[e] = (T)((ACC)old[c] + acc[c]);
+            } else if constexpr (MODE == HV_LANCZOS) {
+                double r = sx * (double)acc[c];
+                if (has_p) r -= bsp * (double)old[c];
+                const T rt = (T)r;
+                y[e] = rt;
+                sums[0] += (double)xr.v[c] * (double)rt;
+            } else {
+                const double xv = (double)xr.v[c], hv = (double)acc[c];
+                sums[0] += xv * hv;
+                sums[1] += hv * hv;
+                sums[2] += xv * xv;
+            }
         }
     }
     if constexpr (MODE == HV_LANCZOS) {
@@ -577,21 +650,22 @@ __global__ __launch_bounds__(kHvBlock) void k_hv2(HvOp op, HvArgs a)
     }
 }
 
-template <typename T, typename ACC, int KH>
-static int launch_hv2_k(hipStream_t s, const HvOp &op, int mode, const HvArgs &a)
+template <typename T, typename ACC, int C, int KH>
+static int launch_hv2_c(hipStream_t s, const HvOp &op, int mode, const HvArgs &a)
 {
-    const size_t lds = (size_t)(kPairs2 + 1) * align16((size_t)op.nup * sizeof(T));
+    const size_t lds = hv2_lds(op.nup, C, sizeof(T));
+    const int grid = (op.ndn + C - 1) / C;
     void (*f)(HvOp, HvArgs);
-    if (mode == HV_PLAIN) f = k_hv2<T, ACC, HV_PLAIN, KH>;
-    else if (mode == HV_ACCUM) f = k_hv2<T, ACC, HV_ACCUM, KH>;
-    else if (mode == HV_LANCZOS) f = k_hv2<T, ACC, HV_LANCZOS, KH>;
-    else f = k_hv2<T, double, HV_RQ, KH>;
+    if (mode == HV_PLAIN) f = k_hv2<T, ACC, C, HV_PLAIN, KH>;
+    else if (mode == HV_ACCUM) f = k_hv2<T, ACC, C, HV_ACCUM, KH>;
+    else if (mode == HV_LANCZOS) f = k_hv2<T, ACC, C, HV_LANCZOS, KH>;
+    else f = k_hv2<T, double, C, HV_RQ, KH>;
     if (lds > 64 * 1024 &&
         hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return -1;
-    hipLaunchKernelGGL(f, dim3(op.ndn), dim3(kHvBlock), lds, s, op, a);
+    hipLaunchKernelGGL(f, dim3(grid), dim3(kHvBlock), lds, s, op, a);
     if (hipGetLastError() != hipSuccess) return -1;
-    return finish_hv(s, mode, op.ndn, a);
+    return finish_hv(s, mode, grid, a);
 }
 
 // Persistent single-orbital H.v with the up-spin hop table resident in LDS.
@@ -889,18 +963,31 @@ static int launch_hv_k(hipStream_t s, const HvOp &op, int mode, int cols, const 
     }
 }
 
+// Two columns per workgroup when the tuning asks for more than one and their
+// LDS fits.  Measured on MI355X at Ns=14, sector (7,7), FP32 vectors: the
+// plain H.v takes 76 us with two columns against 98 us with one, while the
+// Lanczos solve ties (158 ms): the LDS of two columns leaves room for two
+// workgroups per CU, that of one column for four.  FP64: one column is
+// faster (254 against 259 ms per solve), so contexts default to one.
+template <typename T, typename ACC, int KH>
+static int launch_hv2_k(hipStream_t s, const HvOp &op, int mode, int cols, const HvArgs &a)
+{
+    if (cols >= 2 && hv2_lds(op.nup, 2, sizeof(T)) <= kLdsBudget) return launch_hv2_c<T, ACC, 2, KH>(s, op, mode, a);
+    return launch_hv2_c<T, ACC, 1, KH>(s, op, mode, a);
+}
+
 template <typename T, typename ACC>
 static int launch_hv_a(hipStream_t s, const HvOp &op, int mode, int cols, const HvArgs &a)
 {
     const bool kan = op.norb >= 2;
     const int km = op.up.kmax > op.dn.kmax ? op.up.kmax : op.dn.kmax;
-    // two orbitals: the specialized kernel (tables with <= 16 slots, LDS for 3 columns)
+    // two orbitals: the specialized kernel (tables with <= 16 slots, LDS for a column and its target)
     if (op.norb == kNorb2 && op.up.KC >= 8 && op.up.KC == op.dn.KC && km <= 16 && op.up.hrec &&
-        (size_t)(kPairs2 + 1) * align16((size_t)op.nup * sizeof(T)) <= kLdsBudget && op.sched < 16) {
-        if (km <= 8 && op.up.KC == 8) return launch_hv2_k<T, ACC, 8>(s, op, mode, a);
+        hv2_lds(op.nup, 1, sizeof(T)) <= kLdsBudget && op.sched < 16) {
+        if (km <= 8 && op.up.KC == 8) return launch_hv2_k<T, ACC, 8>(s, op, mode, cols, a);
         if (op.up.KC == 16) {
-            if (km <= 12) return launch_hv2_k<T, ACC, 12>(s, op, mode, a);
-            return launch_hv2_k<T, ACC, 16>(s, op, mode, a);
+            if (km <= 12) return launch_hv2_k<T, ACC, 12>(s, op, mode, cols, a);
+            return launch_hv2_k<T, ACC, 16>(s, op, mode, cols, a);
         }
     }
     if (kan) return launch_hv_k<T, ACC, true, 0>(s, op, mode, cols, a);

@@ -74,7 +74,8 @@ def test_spin_tables_bit_exact(edgpu, oracle, norb, nbath, ne):
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 2e-6)])
 @pytest.mark.parametrize("norb,nbath,nu,nd", [(1, 5, 3, 3), (1, 5, 2, 4), (1, 9, 5, 5), (2, 4, 3, 3),
                                               (2, 6, 4, 3), (2, 8, 5, 5), (3, 3, 3, 3), (1, 0, 1, 0),
-                                              (4, 2, 3, 2), (5, 2, 4, 3), (1, 5, 0, 3), (1, 5, 6, 2)])
+                                              (4, 2, 3, 2), (5, 2, 4, 3), (1, 5, 0, 3), (1, 5, 6, 2),
+                                              (2, 5, 3, 2), (2, 7, 4, 1)])
 def test_hx_parity(edgpu, oracle, precision, tol, norb, nbath, nu, nd):
     print(f"hx {precision} norb={norb} nbath={nbath} sector=({nu},{nd})", flush=True)
     rng = np.random.default_rng(100 + nbath + nu)
