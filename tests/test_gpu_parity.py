@@ -336,7 +336,7 @@ def test_error_paths(edgpu):
 # ----------------------------------------------- Ns=14, sector (7,7)
 
 def test_ns14_properties_fp32_vs_fp64(edgpu, oracle):
-    """Config-2 shape (Ns=14, sector (7,7), D=11,778,624): hx against the
+    """Single orbital at full size (Ns=14, sector (7,7), D=11,778,624): hx against the
     oracle on one vector; symmetry <y,Hx> = <x,Hy>; FP32-vector solve vs
     FP64 solve: |dE0| <= 1e-6 and max|dG| <= 1e-5; sum rule; n_up = n_dn."""
     from edgpu import workloads as W
@@ -386,6 +386,42 @@ def test_ns14_properties_fp32_vs_fp64(edgpu, oracle):
     G2, _ = s2.cluster_green_ftn(em, ek, vmk, om)
     assert np.abs(G2[:, 0, 0] - G2[:, 0, 1]).max() < 1e-5
     assert np.abs(G2[:, 0, 0] - G64[:, 0, 0]).max() < 1e-5
+
+
+def test_kanamori_ns14_columns_vs_oracle(edgpu, oracle):
+    """Two-orbital Kanamori at full size (6 bath sites per orbital, Ns=14,
+    sector (7,7), D=11,778,624): the H.v with one and with two sector
+    columns per workgroup against the oracle's hx (FP64 vectors, one
+    vector), and the FP32-vector solve with either tiling against the FP64
+    solve: |dE0| <= 1e-6, max|dG| <= 1e-5."""
+    from edgpu import workloads as W
+    oracle.set_threads(16)
+    em, ek, vmk, mu, J, Up = W.two_orbital_kanamori(6, 3.0, seed=1)
+    kw = dict(norb=2, nbath=12, U=3.0, J=J, Up=Up, Jp=J, mu=mu, maxnstep=200, maxnstep_green=150,
+              both_spins=True, seed=1)
+    m = oracle.Model(2, 12, U=3.0, J=J, mu=mu, Up=Up, Jp=J)
+    b = oracle.Basis(14, 7, 7)
+    x = np.random.default_rng(3).normal(size=b.ntot)
+    print("kanamori ns14 oracle hx", flush=True)
+    ref = oracle.hx(m, em, ek, vmk, b, x)
+    s64 = edgpu.Solver(precision="fp64", **kw)
+    for cols in (1, 2):
+        print(f"kanamori ns14 hx cols={cols}", flush=True)
+        s64.set_tuning(cols, False)
+        y = s64.hx(em, ek, vmk, 7, 7, x)
+        assert np.abs(y - ref).max() < 1e-13 * np.abs(ref).max(), cols
+    om = W.matsubara(2048, 100.0)
+    s64.set_tuning(1, False)
+    e64 = s64.ground_state(em, ek, vmk, [(7, 7)]).val
+    G64, _ = s64.cluster_green_ftn(em, ek, vmk, om)
+    for cols in (1, 2):
+        print(f"kanamori ns14 fp32 solve cols={cols}", flush=True)
+        s = edgpu.Solver(precision="fp32", **kw)
+        s.set_tuning(cols, True)  # FP64 row sums, the two-orbital default
+        e32 = s.ground_state(em, ek, vmk, [(7, 7)]).val
+        G32, _ = s.cluster_green_ftn(em, ek, vmk, om)
+        assert abs(e32 - e64) <= 1e-6, cols
+        assert np.abs(G32 - G64).max() <= 1e-5, cols
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
