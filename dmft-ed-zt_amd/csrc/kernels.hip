@@ -1085,26 +1085,33 @@ template int launch_norm<double>(hipStream_t, const double *, long long, double 
 // Lanczos step j, second half: alpha_j = s_j * sum(hv_part) (every
 // workgroup reduces the H.v partials in the same order; workgroup 0 records
 // alpha_j), r <- r - alpha_j s_j u_j, partial sums of |r|^2 for beta_{j+1}.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_axpy_norm(T *r, const T *u, long long n,
+// Few large workgroups: every workgroup of the next H.v sums this kernel's
+// partials before its row loop, and every workgroup here sums the H.v's, so
+// the partial counts are kept small (1024 x 256 threads as many as 4096 x
+// 64 would be, a quarter of the partials of 256-thread workgroups).
+constexpr int kAxpyBlock = 1024;
+constexpr int kAxpyGrid = 256;
+
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void k_axpy_norm(T *r, const T *u, long long n,
                                                       const double *hv_part, int hv_n,
                                                       const double *scale, double *alpha,
                                                       int step, double *partial, const int *stop)
 {
     if (*stop) return;
     const double sj = scale[step];
-    const double al = sj * sum_partials<kBlock>(hv_part, hv_n);
+    const double al = sj * sum_partials<BS>(hv_part, hv_n);
     if (blockIdx.x == 0 && threadIdx.x == 0) alpha[step] = al;
     const double coef = al * sj;
     // 16-byte vectors (vectors are 256-byte aligned); workgroup 0 also takes
     // the n % V tail.  The update itself is FP64.
     constexpr int V = 16 / sizeof(T);
     using Vec = Row<T, V>;
-    const long long nv = n / V, stride = (long long)gridDim.x * kBlock;
+    const long long nv = n / V, stride = (long long)gridDim.x * BS;
     Vec *rv = reinterpret_cast<Vec *>(r);
     const Vec *uv = reinterpret_cast<const Vec *>(u);
     double v[1] = {0.0};
-    for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < nv; q += stride) {
+    for (long long q = (long long)blockIdx.x * BS + threadIdx.x; q < nv; q += stride) {
         const Vec a = rv[q], b = uv[q];
         Vec w;
 #pragma unroll
@@ -1115,12 +1122,12 @@ __global__ __launch_bounds__(kBlock) void k_axpy_norm(T *r, const T *u, long lon
         rv[q] = w;
     }
     if (blockIdx.x == 0)
-        for (long long i = nv * V + threadIdx.x; i < n; i += kBlock) {
+        for (long long i = nv * V + threadIdx.x; i < n; i += BS) {
             const T t = (T)((double)r[i] - coef * (double)u[i]);
             r[i] = t;
             v[0] += (double)t * (double)t;
         }
-    block_partial<1>(v, partial);
+    block_partial<1, BS>(v, partial);
 }
 
 template <typename T>
@@ -1129,10 +1136,10 @@ int launch_axpy_norm(hipStream_t s, T *r, const T *u, long long n, const double 
                      int *stop)
 {
     if (((uintptr_t)r | (uintptr_t)u) & 15) return -1;  // vector kernels need 16-byte alignment
-    int grid = grid_for(n);
-    if (grid > 1024) grid = 1024;
-    hipLaunchKernelGGL(k_axpy_norm<T>, dim3(grid), dim3(kBlock), 0, s, r, u, n, hv_part, hv_n,
-                       scale, alpha, step, partial, stop);
+    long long g = (n + (long long)kAxpyBlock * 16 - 1) / ((long long)kAxpyBlock * 16);
+    const int grid = g < 1 ? 1 : (g > kAxpyGrid ? kAxpyGrid : (int)g);
+    hipLaunchKernelGGL((k_axpy_norm<T, kAxpyBlock>), dim3(grid), dim3(kAxpyBlock), 0, s, r, u, n, hv_part,
+                       hv_n, scale, alpha, step, partial, stop);
     return hipGetLastError() == hipSuccess ? grid : -1;
 }
 template int launch_axpy_norm<float>(hipStream_t, float *, const float *, long long,
