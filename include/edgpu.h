@@ -76,8 +76,12 @@ int edg_device_count(int *n);
  * workgroups (one per compute unit, single-orbital hop table resident in
  * LDS): 16 = 1 column and 512 threads, 32 = 1 column and 1024 threads,
  * 64 = 2 columns and 1024 threads; and FP64 accumulation of the row sums (1) or accumulation in
- * the storage precision (0).  No counterpart in the reference (its hx is one
- * scalar loop); set by the autotuner or a caller that has measured. */
+ * the storage precision (0).  The two-orbital kernel takes one column for 1
+ * and two for 2, 4 or 8 (when their LDS fits); 16, 32 and 64 send two-orbital
+ * models to the general kernel.  Defaults: 2 columns for one orbital, 1 for
+ * more; FP64 row sums for FP32 vectors with two or more orbitals.  No
+ * counterpart in the reference (its hx is one scalar loop); set by the
+ * autotuner or a caller that has measured. */
 int edg_set_tuning(edg_ctx *ctx, int cols_per_block, int fp64_accumulate);
 
 /* Per-spin basis and hop tables of one (spin, N) sector half, built on the
