@@ -9,7 +9,8 @@ single-orbital Bethe-bath impurity at Ns=14 (Nbath=13), half-filled sector
   spins) of 150 steps each and G(i w_n) on 2048 Matsubara points (beta=100).
 Vectors are FP32 with FP64 dot products, alpha/beta, tridiagonal solve and
 continued fraction.  The same solve in FP64 vectors is timed for the
-FP32-vs-FP64 speed-up and the errors |dE0|, max|dG|.
+FP32-vs-FP64 speed-up and the errors |dE0|, max|dG| (fp32_errors_by_U: the
+same errors at U = 1, 2 and 4).
 
 value = ED solves/s over the timed steps (inputs resident on the device).
 roofline = the H.v kernel: algorithmic bytes per launch (x read once + y
@@ -27,7 +28,7 @@ The same JSON line also carries:
   kanamori  the two-orbital Kanamori Ns=14 problem, FP32 vs FP64 vectors:
             solve times, |dE0|, max|dG|, H.v time, and the oracle's ms per
             Lanczos step on the host cores;
-  dmft_ns6      the Ns=6 problem through the DMFT loop (bath fit on the
+  dmft_ns6  the Ns=6 problem through the DMFT loop (bath fit on the
             host) with the FP64 GPU solver and with the oracle: iterations,
             time per solve and per iteration.
 
@@ -332,6 +333,22 @@ def main():
                    dE0=float(abs(e32 - e64)), max_dG=float(np.abs(G32 - G64).max()),
                    hv_fp64_gbs=hv64_bytes / (hv64_ms * 1e-3) / 1e9, hv_fp64_us=hv64_ms * 1e3)
         s64.close()
+        # the same comparison at the other interaction strengths of the Ns=14
+        # workload (one solve per precision each)
+        errs = {f"{U:g}": dict(dE0=out["dE0"], max_dG=out["max_dG"])}
+        for U2 in (1.0, 4.0):
+            if U2 == U:
+                continue
+            log(f"FP32 vs FP64 at U={U2:g}")
+            em2, ek2, vmk2, mu2 = W.single_orbital(13, U2, seed=1)
+            r = {}
+            for prec in ("fp32", "fp64"):
+                s2 = make_solver(edgpu, prec, U2, mu2)
+                r[prec] = solve(s2, em2, ek2, vmk2, om)
+                s2.close()
+            errs[f"{U2:g}"] = dict(dE0=float(abs(r["fp32"][0] - r["fp64"][0])),
+                                   max_dG=float(np.abs(r["fp32"][1] - r["fp64"][1]).max()))
+        out["fp32_errors_by_U"] = errs
 
     extra = {}
     from edgpu import serve as S

@@ -388,6 +388,27 @@ def test_ns14_properties_fp32_vs_fp64(edgpu, oracle):
     assert np.abs(G2[:, 0, 0] - G64[:, 0, 0]).max() < 1e-5
 
 
+@pytest.mark.parametrize("U", [1.0, 4.0])
+def test_ns14_fp32_vs_fp64_weak_and_strong_coupling(edgpu, U):
+    """Single orbital at full size (Ns=14, sector (7,7)) at U = 1 and U = 4,
+    the other interaction strengths of the full-size workload: FP32-vector
+    solve against the FP64 solve, |dE0| <= 1e-6 and max|dG| <= 1e-5."""
+    from edgpu import workloads as W
+    em, ek, vmk, mu = W.single_orbital(13, U, seed=1)
+    om = W.matsubara(2048, 100.0)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        print(f"ns14 U={U} {prec} solve", flush=True)
+        s = edgpu.Solver(norb=1, nbath=13, U=U, mu=mu, maxnstep=200, maxnstep_green=150,
+                         precision=prec, both_spins=True, seed=1)
+        e0 = s.ground_state(em, ek, vmk, [(7, 7)]).val
+        G, _ = s.cluster_green_ftn(em, ek, vmk, om)
+        res[prec] = (e0, G)
+        s.close()
+    assert abs(res["fp32"][0] - res["fp64"][0]) <= 1e-6
+    assert np.abs(res["fp32"][1] - res["fp64"][1]).max() <= 1e-5
+
+
 def test_kanamori_ns14_columns_vs_oracle(edgpu, oracle):
     """Two-orbital Kanamori at full size (6 bath sites per orbital, Ns=14,
     sector (7,7), D=11,778,624): the H.v with one and with two sector
