@@ -254,11 +254,17 @@ def test_set_gs_then_green_matches_oracle_fp32(edgpu, oracle):
     assert np.abs(G - G2).max() < 1e-5
 
 
-def test_kanamori_two_orbital_vs_oracle(edgpu, oracle):
+@pytest.mark.parametrize("cols", [1, 2])
+def test_kanamori_two_orbital_vs_oracle(edgpu, oracle, cols):
+    """Two orbitals, Ns=6: ground state, G and occupancies against the oracle
+    with one and two sector columns per workgroup (the c+ down chain's
+    sector (3,4) has 15 columns, so the last two-column group is partial)."""
     from edgpu import workloads as W
     em, ek, vmk, mu, J, Up = W.two_orbital_kanamori(2, 3.0, seed=4)
     for precision, etol, gtol in (("fp64", 1e-10, 1e-9), ("fp32", 1e-6, 1e-5)):
-        s = edgpu.Solver(norb=2, nbath=4, U=3.0, J=J, mu=mu, maxnstep=150, precision=precision)
+        s = edgpu.Solver(norb=2, nbath=4, U=3.0, J=J, mu=mu, maxnstep=150, precision=precision,
+                         both_spins=True)  # spin-down chains too (spin-symmetric bath)
+        s.set_tuning(cols, precision == "fp32")
         m = oracle.Model(2, 4, U=3.0, J=J, mu=mu)
         gs = s.ground_state(em, ek, vmk, [(3, 3)])
         ogs = oracle.ground_state(m, em, ek, vmk, [(3, 3)], maxnstep=150)
