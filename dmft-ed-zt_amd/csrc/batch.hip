@@ -65,7 +65,20 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, 
     }
     const int nel = ncol * nup;
     const T *xt = x + (long long)c0 * nup;
-    for (int e = threadIdx.x; e < nel; e += kBlock) xs[e] = gload(xt + e);
+    // the tile's columns are contiguous: eight loads in flight per thread
+    // before the LDS stores (one at a time measured 3% fewer solves/s on a
+    // batch of 512 problems with Ns in {8, 10, 12})
+    for (int e0 = threadIdx.x; e0 < nel; e0 += 8 * kBlock) {
+        T v[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int e = e0 + r * kBlock;
+            v[r] = gload(xt + (e < nel ? e : nel - 1));
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+            if (e0 + r * kBlock < nel) xs[e0 + r * kBlock] = v[r];
+    }
     for (int e = threadIdx.x; e < ncol * kcd; e += kBlock) s_drec[e] = gload(op.dn.hrec + (long long)c0 * kcd + e);
     for (int c = threadIdx.x; c < ncol; c += kBlock) {
         s_ed[c] = energy_of<ACC>(op.dn, c0 + c);
