@@ -86,6 +86,17 @@ struct edg_ctx {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     long long n_hv = 0, n_steps = 0;
     bool have_gs_ev = false, have_gf_ev = false;
+
+    // A second stream with its own partial sums, scalars and ring: the c+
+    // and c chains of one (orbital, spin) run concurrently, the second
+    // filling the first one's kernel tails and host round trips.
+    hipStream_t s2 = nullptr;
+    double *partial2 = nullptr;
+    unsigned *ticket2 = nullptr;
+    Scalars sc2{};
+    void *scal_mem2 = nullptr;
+    DevBuf ring2, mapbuf2;
+    hipEvent_t ev2[2] = {nullptr, nullptr};  // [0] main stream ready, [1] second stream done
 };
 
 namespace {
@@ -94,6 +105,21 @@ int fail(edg_ctx *c, int code, const std::string &msg)
 {
     if (c) c->err = msg;
     return code;
+}
+
+// The stream and per-run device buffers of one Lanczos run in flight.
+struct Lane {
+    hipStream_t stream;
+    double *partial;
+    unsigned *ticket;
+    const Scalars *sc;
+    DevBuf *ring, *mapbuf;
+};
+
+Lane lane_of(edg_ctx *c, int k)
+{
+    if (k == 0) return Lane{c->stream, c->partial, c->ticket, &c->sc, &c->ring, &c->mapbuf};
+    return Lane{c->s2, c->partial2, c->ticket2, &c->sc2, &c->ring2, &c->mapbuf2};
 }
 
 #define HIPCHK(c, call)                                                                 \
@@ -121,8 +147,9 @@ bool dev_malloc(void **p, size_t bytes, int device)
 int ensure(edg_ctx *c, DevBuf &b, size_t bytes)
 {
     if (b.bytes >= bytes) return 0;
-    if (b.p) {
+    if (b.p) {  // either stream may still use it
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->s2) HIPCHK(c, hipStreamSynchronize(c->s2));
         HIPCHK(c, hipFree(b.p));
         b.p = nullptr;
         b.bytes = 0;
@@ -264,29 +291,30 @@ HvArgs base_args(edg_ctx *c)
 
 // Run up to maxn Lanczos steps (lanczos_solver.f90:92-173) on vectors held
 // either as a stored basis (slot j at base + j*stride, maxn+1 slots) or a
-// ring of 3 slots.  Slot 0 holds the unnormalized start vector.  On return
-// the host arrays a, b (maxn entries, zero-padded) and scales s_j (if not
-// null) are filled; *vinit_norm receives |v_init|.
+// ring of 3 slots.  Slot 0 holds the unnormalized start vector.  The steps
+// are queued on the lane's stream without waiting; lanczos_finish waits for
+// them and fills the host arrays a, b (maxn entries, zero-padded) and the
+// scales s_j (if not null); *vinit_norm receives |v_init|.
 template <typename T>
-int run_lanczos(edg_ctx *c, const HvOp &op, T *base, long long stride, bool stored, int maxn,
-                int *nstep, double *a, double *b, double *scales, double *vinit_norm)
+int lanczos_enqueue(edg_ctx *c, const Lane &L, const HvOp &op, T *base, long long stride, bool stored,
+                    int maxn)
 {
     const long long D = (long long)op.nup * op.ndn;
-    const Scalars &S = c->sc;
+    const Scalars &S = *L.sc;
     auto slot = [&](int j) -> T * { return base + (stored ? (long long)j : (long long)(j % 3)) * stride; };
-    HIPCHK(c, hipMemsetAsync(S.alpha, 0, sizeof(double) * maxn, c->stream));
-    HIPCHK(c, hipMemsetAsync(S.beta, 0, sizeof(double) * (maxn + 1), c->stream));
-    HIPCHK(c, hipMemsetAsync(S.scale, 0, sizeof(double) * (maxn + 1), c->stream));
+    HIPCHK(c, hipMemsetAsync(S.alpha, 0, sizeof(double) * maxn, L.stream));
+    HIPCHK(c, hipMemsetAsync(S.beta, 0, sizeof(double) * (maxn + 1), L.stream));
+    HIPCHK(c, hipMemsetAsync(S.scale, 0, sizeof(double) * (maxn + 1), L.stream));
     int fl[2] = {0, maxn};
-    HIPCHK(c, hipMemcpyAsync(S.flags, fl, sizeof(fl), hipMemcpyHostToDevice, c->stream));
-    if (launch_norm<T>(c->stream, slot(0), D, c->partial, c->ticket, &S.misc[0], &S.scale[0], nullptr))
+    HIPCHK(c, hipMemcpyAsync(S.flags, fl, sizeof(fl), hipMemcpyHostToDevice, L.stream));
+    if (launch_norm<T>(L.stream, slot(0), D, L.partial, L.ticket, &S.misc[0], &S.scale[0], nullptr))
         return fail(c, EDG_ERR_HIP, "norm launch failed");
     // H.v partials and axpy partials live in separate halves of the buffer:
     // step j's H.v reads step j-1's axpy partials while writing its own
-    double *hv_part = c->partial, *ax_part = c->partial + kPartialHalf;
+    double *hv_part = L.partial, *ax_part = L.partial + kPartialHalf;
     int ax_n = 0;
     for (int j = 0; j < maxn; ++j) {
-        HvArgs ha = base_args(c);
+        HvArgs ha{};
         ha.x = slot(j);
         ha.y = slot(j + 1);
         ha.p = j > 0 ? slot(j - 1) : nullptr;
@@ -298,28 +326,35 @@ int run_lanczos(edg_ctx *c, const HvOp &op, T *base, long long stride, bool stor
         ha.beta = S.beta;
         ha.stop = S.flags;
         ha.nstep = S.flags + 1;
-        const int hv_n = launch_hv<T>(c->stream, op, HV_LANCZOS, c->cols, c->acc64, ha);
+        const int hv_n = launch_hv<T>(L.stream, op, HV_LANCZOS, c->cols, c->acc64, ha);
         if (hv_n < 0) return fail(c, EDG_ERR_HIP, "H.v launch failed");
         ++c->n_hv;
         ++c->n_steps;
         if (j == maxn - 1) {
-            if (launch_alpha(c->stream, hv_part, hv_n, &S.scale[j], &S.alpha[j], S.flags))
+            if (launch_alpha(L.stream, hv_part, hv_n, &S.scale[j], &S.alpha[j], S.flags))
                 return fail(c, EDG_ERR_HIP, "alpha launch failed");
             break;
         }
-        ax_n = launch_axpy_norm<T>(c->stream, slot(j + 1), slot(j), D, hv_part, hv_n, S.scale, S.alpha,
+        ax_n = launch_axpy_norm<T>(L.stream, slot(j + 1), slot(j), D, hv_part, hv_n, S.scale, S.alpha,
                                    j, ax_part, S.flags);
         if (ax_n < 0) return fail(c, EDG_ERR_HIP, "axpy launch failed");
     }
+    return 0;
+}
+
+int lanczos_finish(edg_ctx *c, const Lane &L, int maxn, int *nstep, double *a, double *b, double *scales,
+                   double *vinit_norm)
+{
+    const Scalars &S = *L.sc;
     std::vector<double> ha(maxn), hb(maxn + 1), hs(maxn + 1);
     int hf[2];
     double nv;
-    HIPCHK(c, hipMemcpyAsync(ha.data(), S.alpha, sizeof(double) * maxn, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(hb.data(), S.beta, sizeof(double) * (maxn + 1), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(hs.data(), S.scale, sizeof(double) * (maxn + 1), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(hf, S.flags, sizeof(hf), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(&nv, &S.misc[0], sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpyAsync(ha.data(), S.alpha, sizeof(double) * maxn, hipMemcpyDeviceToHost, L.stream));
+    HIPCHK(c, hipMemcpyAsync(hb.data(), S.beta, sizeof(double) * (maxn + 1), hipMemcpyDeviceToHost, L.stream));
+    HIPCHK(c, hipMemcpyAsync(hs.data(), S.scale, sizeof(double) * (maxn + 1), hipMemcpyDeviceToHost, L.stream));
+    HIPCHK(c, hipMemcpyAsync(hf, S.flags, sizeof(hf), hipMemcpyDeviceToHost, L.stream));
+    HIPCHK(c, hipMemcpyAsync(&nv, &S.misc[0], sizeof(double), hipMemcpyDeviceToHost, L.stream));
+    HIPCHK(c, hipStreamSynchronize(L.stream));
     const int ns = hf[1];
     *nstep = ns;
     for (int k = 0; k < maxn; ++k) {
@@ -332,6 +367,14 @@ int run_lanczos(edg_ctx *c, const HvOp &op, T *base, long long stride, bool stor
         for (int k = 0; k <= maxn; ++k) scales[k] = hs[k];
     if (vinit_norm) *vinit_norm = nv;
     return 0;
+}
+
+template <typename T>
+int run_lanczos(edg_ctx *c, const Lane &L, const HvOp &op, T *base, long long stride, bool stored, int maxn,
+                int *nstep, double *a, double *b, double *scales, double *vinit_norm)
+{
+    if (int rc = lanczos_enqueue<T>(c, L, op, base, stride, stored, maxn)) return rc;
+    return lanczos_finish(c, L, maxn, nstep, a, b, scales, vinit_norm);
 }
 
 template <typename T>
@@ -381,7 +424,8 @@ int diag_sector(edg_ctx *c, int nu, int nd, unsigned long long seed, T *gs_out, 
     if (launch_start<T>(c->stream, base, D, seed)) return fail(c, EDG_ERR_HIP, "start launch failed");
     int nstep;
     std::vector<double> a(maxn), b(maxn), s(maxn + 1);
-    if (int rc = run_lanczos<T>(c, op, base, Dp, true, maxn, &nstep, a.data(), b.data(), s.data(), nullptr))
+    if (int rc = run_lanczos<T>(c, lane_of(c, 0), op, base, Dp, true, maxn, &nstep, a.data(), b.data(), s.data(),
+                                nullptr))
         return rc;
     std::vector<double> z(nstep);
     double lam;
@@ -457,34 +501,58 @@ int ground_state_t(edg_ctx *c, int nsector, const int *sectors, int *isector, do
 
 // One green_diag chain (dmft_green.f90:202-257, one of G+ / G-): v_init =
 // c(+)|gs> in the neighbouring sector, Lanczos, and the coefficient arrays
-// with b(1) = |v_init| (:219,:249).
+// with b(1) = |v_init| (:219,:249).  chain_enqueue queues it on a lane;
+// chain_finish waits for it and fills the coefficients.
+struct ChainRun {
+    bool active = false;  // false: the sector does not exist (no weight)
+    int maxn = 0;
+};
+
+// The sector of a chain; false when it does not exist.
+bool chain_sector(const edg_ctx *c, int ispin, int pm, int *nu, int *nd)
+{
+    const int ns_site = c->P.norb + c->P.nbath;
+    *nu = c->gs_nu + (ispin == 1 ? (pm == 1 ? 1 : -1) : 0);
+    *nd = c->gs_nd + (ispin == 2 ? (pm == 1 ? 1 : -1) : 0);
+    return *nu >= 0 && *nd >= 0 && *nu <= ns_site && *nd <= ns_site;
+}
+
 template <typename T>
-int chain(edg_ctx *c, int iorb, int ispin, int pm, double *aa, double *bb, int *nstep)
+int chain_enqueue(edg_ctx *c, const Lane &L, int iorb, int ispin, int pm, ChainRun *r)
 {
     const int ns_site = c->P.norb + c->P.nbath, maxn = c->P.maxnstep_green;
-    const int nu = c->gs_nu + (ispin == 1 ? (pm == 1 ? 1 : -1) : 0);
-    const int nd = c->gs_nd + (ispin == 2 ? (pm == 1 ? 1 : -1) : 0);
-    for (int k = 0; k < maxn; ++k) aa[k] = bb[k] = 0.0;
-    *nstep = 0;
-    if (nu < 0 || nd < 0 || nu > ns_site || nd > ns_site) return 0;  // no such sector: no weight
+    r->active = false;
+    r->maxn = maxn;
+    int nu, nd;
+    if (!chain_sector(c, ispin, pm, &nu, &nd)) return 0;
     HvOp op, op0;
     if (int rc = make_op(c, nu, nd, &op)) return rc;
     if (int rc = make_op(c, c->gs_nu, c->gs_nd, &op0)) return rc;
     const long long D = (long long)op.nup * op.ndn, Dp = round_up(D, kPad);
     if (int rc = admit(c, 3 * Dp * sizeof(T))) return rc;
-    if (int rc = ensure(c, c->ring, 3 * Dp * sizeof(T))) return rc;
-    if (int rc = ensure(c, c->mapbuf, sizeof(uint32_t) * 8192)) return rc;
-    T *base = (T *)c->ring.p;
+    if (int rc = ensure(c, *L.ring, 3 * Dp * sizeof(T))) return rc;
+    if (int rc = ensure(c, *L.mapbuf, sizeof(uint32_t) * 8192)) return rc;
+    T *base = (T *)L.ring->p;
     const SpinTab &tout = ispin == 1 ? op.up : op.dn;
     const int ne_in = ispin == 1 ? c->gs_nu : c->gs_nd;
-    if (launch_cmap(c->stream, tout.pattern, tout.n, ns_site, ne_in, iorb - 1, pm, (uint32_t *)c->mapbuf.p))
+    if (launch_cmap(L.stream, tout.pattern, tout.n, ns_site, ne_in, iorb - 1, pm, (uint32_t *)L.mapbuf->p))
         return fail(c, EDG_ERR_HIP, "cmap launch failed");
     const int godd = ispin == 2 ? (c->gs_nu & 1) : 0;
-    if (launch_apply_c<T>(c->stream, (const T *)c->gs[c->gs_slot].p, base, op.nup, op.ndn, op0.nup, ispin,
-                          (const uint32_t *)c->mapbuf.p, godd))
+    if (launch_apply_c<T>(L.stream, (const T *)c->gs[c->gs_slot].p, base, op.nup, op.ndn, op0.nup, ispin,
+                          (const uint32_t *)L.mapbuf->p, godd))
         return fail(c, EDG_ERR_HIP, "apply_c launch failed");
+    if (int rc = lanczos_enqueue<T>(c, L, op, base, Dp, false, maxn)) return rc;
+    r->active = true;
+    return 0;
+}
+
+int chain_finish(edg_ctx *c, const Lane &L, const ChainRun &r, double *aa, double *bb, int *nstep)
+{
+    for (int k = 0; k < r.maxn; ++k) aa[k] = bb[k] = 0.0;
+    *nstep = 0;
+    if (!r.active) return 0;
     double nv;
-    if (int rc = run_lanczos<T>(c, op, base, Dp, false, maxn, nstep, aa, bb, nullptr, &nv)) return rc;
+    if (int rc = lanczos_finish(c, L, r.maxn, nstep, aa, bb, nullptr, &nv)) return rc;
     bb[0] = nv;
     return 0;
 }
@@ -500,6 +568,20 @@ int cluster_green_t(edg_ctx *c, int nw, const double *omega, double *G_cl, doubl
     if (int rc = ensure(c, c->coefbuf, sizeof(double) * 4 * (maxn + 8))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->omegabuf.p, omega, sizeof(double) * nw, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    // the tables of the ground-state and every chain sector first: building
+    // one waits for the main stream, which would serialize the two streams'
+    // chains
+    {
+        HvOp op;
+        if (int rc = make_op(c, c->gs_nu, c->gs_nd, &op)) return rc;
+    }
+    for (int ispin = 1; ispin <= 2; ++ispin)
+        for (int pm = 1; pm <= 2; ++pm) {
+            int nu, nd;
+            HvOp op;
+            if (chain_sector(c, ispin, pm, &nu, &nd))
+                if (int rc = make_op(c, nu, nd, &op)) return rc;
+        }
     std::vector<double> g(2 * nw * 2);
     std::vector<double> cf(4 * maxn);
     for (int iorb = 1; iorb <= norb; ++iorb) {
@@ -513,9 +595,19 @@ int cluster_green_t(edg_ctx *c, int nw, const double *omega, double *G_cl, doubl
                 keep[1] = keep[0];  // the reference writes spin-1 coefficients again (:102-109)
                 continue;
             }
+            // c+ chain on the main stream, c chain on the second one, after
+            // everything queued so far (ground state, tables)
+            const Lane L0 = lane_of(c, 0), L1 = lane_of(c, 1);
+            HIPCHK(c, hipEventRecord(c->ev2[0], c->stream));
+            HIPCHK(c, hipStreamWaitEvent(c->s2, c->ev2[0], 0));
+            ChainRun rp, rm;
+            if (int rc = chain_enqueue<T>(c, L0, iorb, ispin, 1, &rp)) return rc;
+            if (int rc = chain_enqueue<T>(c, L1, iorb, ispin, 2, &rm)) return rc;
             int np, nm;
-            if (int rc = chain<T>(c, iorb, ispin, 1, co[ispin - 1][0], co[ispin - 1][1], &np)) return rc;
-            if (int rc = chain<T>(c, iorb, ispin, 2, co[ispin - 1][2], co[ispin - 1][3], &nm)) return rc;
+            if (int rc = chain_finish(c, L0, rp, co[ispin - 1][0], co[ispin - 1][1], &np)) return rc;
+            if (int rc = chain_finish(c, L1, rm, co[ispin - 1][2], co[ispin - 1][3], &nm)) return rc;
+            HIPCHK(c, hipEventRecord(c->ev2[1], c->s2));
+            HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev2[1], 0));
             HIPCHK(c, hipMemcpyAsync(c->coefbuf.p, keep[ispin - 1].data(), sizeof(double) * 4 * maxn,
                                      hipMemcpyHostToDevice, c->stream));
             const double *dc = (const double *)c->coefbuf.p;
@@ -588,7 +680,8 @@ int lanczos_t(edg_ctx *c, int nu, int nd, const double *v_init, int *nstep, doub
     if (int rc = admit(c, 3 * Dp * sizeof(T))) return rc;
     if (int rc = ensure(c, c->ring, 3 * Dp * sizeof(T))) return rc;
     if (int rc = upload<T>(c, (T *)c->ring.p, v_init, D)) return rc;
-    return run_lanczos<T>(c, op, (T *)c->ring.p, Dp, false, c->P.maxnstep, nstep, a, b, nullptr, nullptr);
+    return run_lanczos<T>(c, lane_of(c, 0), op, (T *)c->ring.p, Dp, false, c->P.maxnstep, nstep, a, b, nullptr,
+                          nullptr);
 }
 
 template <typename T>
@@ -728,6 +821,20 @@ int edg_create(const edg_params *p, int device, edg_ctx **out)
     c->sc.flags = (int *)(s + 3 * mx + 32);
     for (auto &e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) return bad();
+    // the second stream of the Green's-function chains (default priority)
+    if (hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess) return bad();
+    if (!dev_malloc((void **)&c->partial2, sizeof(double) * 16 * 8192 + 64, device)) return bad();
+    if (!dev_malloc((void **)&c->ticket2, 256, device)) return bad();
+    if (hipMemset(c->ticket2, 0, 256) != hipSuccess) return bad();
+    if (!dev_malloc(&c->scal_mem2, sizeof(double) * (3 * mx + 40) + 64, device)) return bad();
+    double *s2 = (double *)c->scal_mem2;
+    c->sc2.alpha = s2;
+    c->sc2.beta = s2 + mx;
+    c->sc2.scale = s2 + 2 * mx;
+    c->sc2.misc = s2 + 3 * mx;
+    c->sc2.flags = (int *)(s2 + 3 * mx + 32);
+    for (auto &e : c->ev2)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return bad();
     *out = c;
     return EDG_OK;
 }
@@ -737,18 +844,25 @@ int edg_destroy(edg_ctx *c)
     if (!c) return EDG_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->s2) (void)hipStreamSynchronize(c->s2);
     for (auto &kv : c->tabs)
         for (void *p : kv.second.allocs) (void)hipFree(p);
     DevBuf *bufs[] = {&c->dparams, &c->basis, &c->ring, &c->gs[0], &c->gs[1], &c->stage, &c->tmp,
-                      &c->coefbuf, &c->gbuf, &c->omegabuf, &c->mapbuf};
+                      &c->coefbuf, &c->gbuf, &c->omegabuf, &c->mapbuf, &c->ring2, &c->mapbuf2};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (c->partial) (void)hipFree(c->partial);
     if (c->ticket) (void)hipFree(c->ticket);
     if (c->scal_mem) (void)hipFree(c->scal_mem);
+    if (c->partial2) (void)hipFree(c->partial2);
+    if (c->ticket2) (void)hipFree(c->ticket2);
+    if (c->scal_mem2) (void)hipFree(c->scal_mem2);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : c->ev2)
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->s2) (void)hipStreamDestroy(c->s2);
     delete c;
     return EDG_OK;
 }
