@@ -15,7 +15,9 @@
  *   complex arrays: interleaved (re, im) FP64, Fortran double complex layout.
  *
  * Threading: one context per host thread; calls on a context are serialized
- * on its HIP stream.  Every call returns an edg_status (0 = ok).
+ * on its HIP streams (a main stream, and a second one on which
+ * edg_cluster_green runs each c chain beside its c+ chain).  Every call
+ * returns an edg_status (0 = ok).
  */
 #ifndef EDGPU_H
 #define EDGPU_H
