@@ -278,6 +278,33 @@ def test_kanamori_two_orbital_vs_oracle(edgpu, oracle, cols):
         assert np.abs(occ - o2).max() < etol * 100 and np.abs(docc - d2).max() < etol * 100
 
 
+@pytest.mark.parametrize("sec", [(8, 4), (0, 4)])
+def test_green_with_missing_chain_sector(edgpu, oracle, sec):
+    """Ground state in a full (8, 4) or empty (0, 4) spin-up sector at Ns=8:
+    the c+ (resp. c) up chain has no sector, so one of the two concurrent
+    chains of that spin carries no work.  G(i w) and the coefficients against
+    the oracle with nspin = 2 (both spins reported).  30 steps stay well below
+    every sector dimension (56 and more): Lanczos runs past convergence would
+    magnify rounding-order differences further."""
+    rng = np.random.default_rng(41)
+    em, ek, vmk = rand_params(rng, 1, 7)
+    s, m = make(edgpu, oracle, 1, 7, U=2.0, mu=1.0, maxnstep=30, nspin=2)
+    gs = s.ground_state(em, ek, vmk, [sec])
+    ogs = oracle.ground_state(m, em, ek, vmk, [sec], maxnstep=30)
+    assert abs(gs.val - ogs.val) < 1e-10
+    om = oracle.matsubara(128, 50.0)
+    G, co = s.cluster_green_ftn(em, ek, vmk, om)
+    G2, co2 = oracle.cluster_green_ftn(m, em, ek, vmk, ogs, 30, om, nspin=2)
+    # these sectors have an excitation close to E0: near that pole G(i w_n)
+    # moves with the pole position as 1/w_n^2 (w_0 = pi/50), so rounding-order
+    # differences of 1e-12 in the poles show as 1e-9 at the lowest points only
+    d = np.abs(G - G2)
+    assert d.max() < 1e-8
+    assert d[4:].max() < 1e-12
+    missing = 0 if sec[0] == 8 else 2  # ap/bp (c+) or am/bm (c) of spin up
+    assert np.all(co[(1, 1)][missing] == 0) and np.all(co[(1, 1)][missing + 1] == 0)
+
+
 @pytest.mark.parametrize("norb,nbath,sec", [(3, 3, (3, 3)), (4, 2, (3, 2))])
 def test_multi_orbital_solve_vs_oracle(edgpu, oracle, norb, nbath, sec):
     """Three and four orbitals (general Kanamori H.v, apply_c on every
