@@ -45,6 +45,12 @@ constexpr int kClasses = 6;            // H.v launch classes, chain_class()
 constexpr int kChunk = 8;              // batched steps launched per chunk
 constexpr int kAhead = 2;              // chunks the host may run ahead of the device
 constexpr int kChunkEvents = 4;        // > kAhead
+// Problems whose state vector takes kLargeVec bytes or more are admitted at
+// most kMaxLarge at a time: several such vectors in flight compete for the
+// MALL (a batch of 8 Ns=14 problems, 47 MB FP32 vectors, measured 41.5 ms per
+// problem admitted one at a time against 44.8 all at once)
+constexpr size_t kLargeVec = size_t(32) << 20;
+constexpr int kMaxLarge = 1;
 
 long long rup(long long n, long long m) { return (n + m - 1) / m * m; }
 
@@ -104,6 +110,7 @@ struct Job {
     std::vector<double> ek, vmk;
     Stage st = ST_QUEUED;
     size_t bytes = 0;
+    bool large = false;  // state vector of kLargeVec bytes or more
     Arena ar;
     double *dparams = nullptr;  // em(1,2) | ek(nbath,2) | vmk(1,nbath,2)
     SpinTab tab[2][3];          // [spin][ne - N_spin + 1]
@@ -1068,6 +1075,10 @@ int edg_batch_submit(edg_batch *b, const edg_problem *p, long long *id)
     J->p.ek = nullptr;
     J->p.vmk = nullptr;
     J->bytes = problem_bytes(b, *p);
+    {
+        const int ns = nsite_of(*p);
+        J->large = (size_t)(binom_ll(ns, p->nup) * binom_ll(ns, p->ndn)) * b->es >= kLargeVec;
+    }
     if (J->bytes > b->budget) {
         delete J;
         return fail(b, EDG_ERR_ADMISSION, "problem needs " + std::to_string(problem_bytes(b, *p)) +
@@ -1084,8 +1095,18 @@ int edg_batch_step(edg_batch *b, int max_steps, int *pending)
     if (!b) return EDG_ERR_ARG;
     (void)hipSetDevice(b->device);
     if (int rc = progress(b)) return rc;
-    while (!b->queue.empty() && b->in_use + b->queue.front()->bytes <= b->budget) {
-        Job *J = b->queue.front();
+    // admission in submission order by device bytes; a large problem waits
+    // while kMaxLarge others run, and the problems behind it go ahead when
+    // they fit
+    int large_in_flight = 0;
+    for (const Job *R : b->running) large_in_flight += R->large ? 1 : 0;
+    for (auto it = b->queue.begin(); it != b->queue.end();) {
+        Job *J = *it;
+        if (J->large && large_in_flight >= kMaxLarge) {
+            ++it;
+            continue;
+        }
+        if (b->in_use + J->bytes > b->budget) break;
         const int rc = admit(b, J);
         if (rc == 1) {  // device full: wait for running problems to release memory
             if (b->running.empty())
@@ -1093,7 +1114,8 @@ int edg_batch_step(edg_batch *b, int max_steps, int *pending)
             break;
         }
         if (rc) return rc;
-        b->queue.pop_front();
+        large_in_flight += J->large ? 1 : 0;
+        it = b->queue.erase(it);
     }
     if (!b->active.empty() && max_steps > 0) {
         int left = max_steps;
