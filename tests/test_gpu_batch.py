@@ -320,3 +320,28 @@ def test_ground_state_sweep_matches_single_path(edgpu, oracle):
     m = oracle.Model(1, 5, U=2.0, mu=1.0)
     ogs = oracle.ground_state(m, em, ek, vmk, sectors, 60, seed=7)
     assert ogs.isector == isec and abs(ogs.val - e0) < 1e-9
+
+
+def test_large_problems_admitted_one_at_a_time(edgpu, oracle):
+    """Two Ns=14 problems (FP32 state vectors of 47 MB, above the 32 MiB
+    large-problem line) and two Ns=10 problems submitted in that order: the
+    first step admits the first large problem and both small ones, holding
+    the second large one; every result equals the same problem run alone."""
+    omega = oracle.matsubara(64, 50.0)
+    probs = [problem(edgpu, 13, 2.0, 71, maxnstep=30, maxnstep_green=20),
+             problem(edgpu, 13, 3.0, 72, maxnstep=30, maxnstep_green=20),
+             problem(edgpu, 9, 2.0, 73, maxnstep=30, maxnstep_green=20),
+             problem(edgpu, 9, 1.0, 74, maxnstep=30, maxnstep_green=20)]
+    b = edgpu.Batch(omega, precision="fp32")
+    ids = [b.submit(p) for p in probs]
+    pending = b.step(1)
+    assert b.stats()["in_flight"] == 3
+    while pending:
+        pending = b.step(16)
+    res = {r["id"]: r for r in b.poll()}
+    b.close()
+    for i, p in zip(ids, probs):
+        alone, _, _ = run_batch(edgpu, [p], omega, precision="fp32")
+        assert res[i]["e0"] == alone[0]["e0"]
+        assert np.array_equal(res[i]["G"], alone[0]["G"])
+
