@@ -17,7 +17,8 @@ roofline = the H.v kernel: algorithmic bytes per launch (x read once + y
 written once, 2*4*D) / average launch time from HIP events on the solver's
 stream.  cpu_baseline = the FP64 C restatement of the reference algorithm
 (oracle/, OpenMP) timed on the host for a bounded number of Lanczos steps
-at the same size and scaled to one solve.
+at the same size and scaled to one solve, on up to 16 host threads and
+(one_core) on one.
 
 The same JSON line also carries:
   batched   512 independent single-orbital problems (Ns in {8,10,12}, random
@@ -89,11 +90,21 @@ def cpu_baseline(em, ek, vmk, U, mu, nsteps, threads):
     t = time.perf_counter()
     O.lanczos_iteration(m, em, ek, vmk, b, v0, nsteps)
     dt = (time.perf_counter() - t) / nsteps
+    # the same on one core, fewer steps (the data is already paged in)
+    O.set_threads(1)
+    n1 = max(1, nsteps // 4)
+    t = time.perf_counter()
+    O.lanczos_iteration(m, em, ek, vmk, b, v0, n1)
+    dt1 = (time.perf_counter() - t) / n1
+    O.set_threads(threads)
     hv_per_solve = 2 * 200 + 1 + 4 * 150
     return dict(value=1.0 / (dt * hv_per_solve), unit="solves/s", cores=used, kind="port",
                 sample=f"{nsteps} FP64 Lanczos steps of the oracle at Ns=14 sector (7,7) "
                        f"(D=11778624), {dt * 1e3:.1f} ms/step, scaled to {hv_per_solve} "
-                       f"steps per solve", ms_per_lanczos_step=dt * 1e3)
+                       f"steps per solve", ms_per_lanczos_step=dt * 1e3,
+                one_core=dict(value=1.0 / (dt1 * hv_per_solve), unit="solves/s",
+                              ms_per_lanczos_step=dt1 * 1e3, steps=n1),
+                host_cpus=os.cpu_count())
 
 
 def cpu_kanamori_step_ms(W, nsteps, threads):
