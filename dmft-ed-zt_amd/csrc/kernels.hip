@@ -965,10 +965,10 @@ static int launch_hv_k(hipStream_t s, const HvOp &op, int mode, int cols, const 
 
 // Two columns per workgroup when the tuning asks for more than one and their
 // LDS fits.  Measured on MI355X at Ns=14, sector (7,7), FP32 vectors: the
-// plain H.v takes 76 us with two columns against 98 us with one, while the
-// Lanczos solve ties (158 ms): the LDS of two columns leaves room for two
-// workgroups per CU, that of one column for four.  FP64: one column is
-// faster (254 against 259 ms per solve), so contexts default to one.
+// plain H.v takes 79 us with two columns against 100 us with one, the solve
+// 135 against 141 ms (the LDS of two columns leaves room for two workgroups
+// per CU, that of one column for four).  FP64: one column is faster (262
+// against 280 ms per solve).  Contexts default accordingly.
 template <typename T, typename ACC, int KH>
 static int launch_hv2_k(hipStream_t s, const HvOp &op, int mode, int cols, const HvArgs &a)
 {

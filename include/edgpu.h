@@ -80,8 +80,9 @@ int edg_device_count(int *n);
  * 64 = 2 columns and 1024 threads; and FP64 accumulation of the row sums (1) or accumulation in
  * the storage precision (0).  The two-orbital kernel takes one column for 1
  * and two for 2, 4 or 8 (when their LDS fits); 16, 32 and 64 send two-orbital
- * models to the general kernel.  Defaults: 2 columns for one orbital, 1 for
- * more; FP64 row sums for FP32 vectors with two or more orbitals.  No
+ * models to the general kernel.  Defaults: 2 columns for one orbital and for
+ * two orbitals with FP32 vectors, else 1; FP64 row sums for FP32 vectors
+ * with two or more orbitals.  No
  * counterpart in the reference (its hx is one scalar loop); set by the
  * autotuner or a caller that has measured. */
 int edg_set_tuning(edg_ctx *ctx, int cols_per_block, int fp64_accumulate);
