@@ -185,8 +185,7 @@ struct edg_batch {
     int cls_tiles[kClasses] = {}, cls_lds[kClasses] = {};
     int nax = 0;
     bool use_graphs = true;
-    hipGraphExec_t gexec = nullptr;
-    bool graph_valid = false;
+    hipGraphExec_t gexec[kChunk + 1] = {};  // [k]: a chunk of k steps, captured on first use
     // replaced graphs may still have launches queued: destroyed once the
     // stream has passed the event recorded after their last launch
     std::vector<std::pair<hipGraphExec_t, hipEvent_t>> retired;
@@ -856,14 +855,14 @@ int rebuild(edg_batch *b)
     b->nax = ax[n];
     b->dirty = false;
     b->st_rebuild += 1;
-    if (b->gexec) {
+    for (hipGraphExec_t &ge : b->gexec) {
+        if (!ge) continue;
         hipEvent_t ev;
         BCHK(b, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         BCHK(b, hipEventRecord(ev, b->stream));
-        b->retired.emplace_back(b->gexec, ev);
-        b->gexec = nullptr;
+        b->retired.emplace_back(ge, ev);
+        ge = nullptr;
     }
-    b->graph_valid = false;
     for (size_t i = 0; i < b->retired.size();) {
         if (hipEventQuery(b->retired[i].second) == hipSuccess) {
             (void)hipGraphExecDestroy(b->retired[i].first);
@@ -876,45 +875,53 @@ int rebuild(edg_batch *b)
     return 0;
 }
 
-int enqueue_step(edg_batch *b)
+// Queue k batched steps (k <= kChunk): step sub of the chunk is one batched
+// H.v launch per class and the axpy, all reading the chains' step + sub;
+// one advance after the chunk adds k to every chain's step.
+int enqueue_chunk(edg_batch *b, int k)
 {
     const int n = (int)b->active.size();
-    for (int c = 0; c < kClasses; ++c) {
-        if (!b->cls_n[c]) continue;
-        if (launch_batch_hv(b->stream, b->fp64, class_cols(c), (c & 1) ? 16 : 8, b->d_inst + b->cls_start[c],
-                            b->d_hvoff[c], b->d_hvown[c], b->cls_tiles[c], b->cls_lds[c]))
-            return fail(b, EDG_ERR_HIP, "batched H.v launch failed");
+    for (int sub = 0; sub < k; ++sub) {
+        for (int c = 0; c < kClasses; ++c) {
+            if (!b->cls_n[c]) continue;
+            if (launch_batch_hv(b->stream, b->fp64, class_cols(c), (c & 1) ? 16 : 8, b->d_inst + b->cls_start[c],
+                                b->d_hvoff[c], b->d_hvown[c], b->cls_tiles[c], b->cls_lds[c], sub))
+                return fail(b, EDG_ERR_HIP, "batched H.v launch failed");
+        }
+        if (launch_batch_axpy(b->stream, b->fp64, b->d_inst, b->d_axoff, b->d_axown, b->nax, sub))
+            return fail(b, EDG_ERR_HIP, "batched axpy launch failed");
     }
-    if (launch_batch_axpy(b->stream, b->fp64, b->d_inst, b->d_axoff, b->d_axown, b->nax))
-        return fail(b, EDG_ERR_HIP, "batched axpy launch failed");
-    if (launch_batch_advance(b->stream, b->d_inst, n)) return fail(b, EDG_ERR_HIP, "advance launch failed");
+    if (launch_batch_advance(b->stream, b->d_inst, n, k)) return fail(b, EDG_ERR_HIP, "advance launch failed");
     return 0;
 }
 
-// Run k batched steps of the active chains (k <= every chain's remainder).
+// Run k batched steps of the active chains (1 <= k <= kChunk, k <= every
+// chain's remainder) as one chunk: a captured graph of k steps, captured on
+// the first chunk of that length after the set of chains last changed.
 int run_steps(edg_batch *b, int k)
 {
+    if (k < 1 || k > kChunk) return fail(b, EDG_ERR_ARG, "chunk length out of range");
     if (b->dirty)
         if (int rc = rebuild(b)) return rc;
-    if (b->use_graphs && !b->graph_valid) {
-        hipGraph_t g = nullptr;
-        BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_step(b);
-        hipError_t e = hipStreamEndCapture(b->stream, &g);
-        if (rc) return rc;
-        if (e != hipSuccess) return fail(b, EDG_ERR_HIP, "graph capture failed");
-        e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) return fail(b, EDG_ERR_HIP, "graph instantiation failed");
-        b->graph_valid = true;
-    }
-    for (int s = 0; s < k; ++s) {
-        if (b->use_graphs) {
-            BCHK(b, hipGraphLaunch(b->gexec, b->stream));
-            b->st_graph += 1;
-        } else if (int rc = enqueue_step(b)) {
-            return rc;
+    if (b->use_graphs) {
+        if (!b->gexec[k]) {
+            hipGraph_t g = nullptr;
+            BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue_chunk(b, k);
+            hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (rc) return rc;
+            if (e != hipSuccess) return fail(b, EDG_ERR_HIP, "graph capture failed");
+            e = hipGraphInstantiate(&b->gexec[k], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                b->gexec[k] = nullptr;
+                return fail(b, EDG_ERR_HIP, "graph instantiation failed");
+            }
         }
+        BCHK(b, hipGraphLaunch(b->gexec[k], b->stream));
+        b->st_graph += 1;
+    } else if (int rc = enqueue_chunk(b, k)) {
+        return rc;
     }
     b->st_steps += k;
     b->st_chain_steps += (double)k * b->active.size();
@@ -1001,7 +1008,8 @@ int edg_batch_destroy(edg_batch *b)
     for (Job *J : b->running) drop(J);
     for (Job *J : b->finished) drop(J);
     for (double *p : b->pinned_free) (void)hipHostFree(p);
-    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    for (hipGraphExec_t ge : b->gexec)
+        if (ge) (void)hipGraphExecDestroy(ge);
     for (auto &r : b->retired) {
         (void)hipGraphExecDestroy(r.first);
         (void)hipEventDestroy(r.second);

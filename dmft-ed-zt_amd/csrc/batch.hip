@@ -30,7 +30,8 @@ __device__ __forceinline__ T *slot_of(const BInst &I, int k)
 // compressed tables (target rank | value index << 16, KC slots per row,
 // self-padded).
 template <typename T, typename ACC, int KS>
-__global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, const int *__restrict__ hv_off, const int *__restrict__ own)
+__global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, const int *__restrict__ hv_off, const int *__restrict__ own,
+                       int sub)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     T *xs = reinterpret_cast<T *>(smem);
@@ -50,7 +51,7 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, 
     const int c0 = xcd_group(t, I.hvn) * cb;
     const int ncol = ndn - c0 < cb ? ndn - c0 : cb;
     constexpr int kcd = KS;
-    const int j = uniform(*I.step);
+    const int j = uniform(*I.step) + sub;
     const T *x = uniform(slot_of<T>(I, j));
     T *y = uniform(slot_of<T>(I, j + 1));
     const T *pv = j > 0 ? uniform(slot_of<T>(I, j - 1)) : nullptr;
@@ -145,13 +146,14 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, 
 // contiguous ranges, as in the single-chain kernel; tiles past the chain's
 // hvn are padding and exit at once.
 template <typename T, int C, int SLOTS>
-__global__ __launch_bounds__(kHvBlock) void k_bhvw(const BInst *__restrict__ inst, const int *__restrict__ hv_off, const int *__restrict__ own)
+__global__ __launch_bounds__(kHvBlock) void k_bhvw(const BInst *__restrict__ inst, const int *__restrict__ hv_off, const int *__restrict__ own,
+                       int sub)
 {
     const int i = own[blockIdx.x];  // chain of this workgroup
     const BInst &I = inst[i];
     const int t = blockIdx.x - hv_off[i];
     if (I.flags[0] || t >= I.hvn) return;
-    const int j = uniform(*I.step);
+    const int j = uniform(*I.step) + sub;
     HvArgs la{};
     la.x = uniform(slot_of<T>(I, j));
     la.y = uniform(slot_of<T>(I, j + 1));
@@ -177,13 +179,14 @@ __global__ __launch_bounds__(kHvBlock) void k_bhvw(const BInst *__restrict__ ins
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_baxpy(const BInst *__restrict__ inst, const int *__restrict__ ax_off, const int *__restrict__ own)
+__global__ __launch_bounds__(kBlock) void k_baxpy(const BInst *__restrict__ inst, const int *__restrict__ ax_off, const int *__restrict__ own,
+                        int sub)
 {
     const int i = own[blockIdx.x];  // chain of this workgroup
     const BInst &I = inst[i];
     if (I.flags[0]) return;
     const int t = blockIdx.x - ax_off[i];
-    const int j = uniform(*I.step);
+    const int j = uniform(*I.step) + sub;
     const double sj = I.scale[j];
     const double al = sj * sum_partials<kBlock>(I.hvpart, I.hvn);
     if (t == 0 && threadIdx.x == 0) I.alpha[j] = al;
@@ -217,10 +220,10 @@ __global__ __launch_bounds__(kBlock) void k_baxpy(const BInst *__restrict__ inst
     block_partial_at<1>(v, I.axpart, t);
 }
 
-__global__ void k_badvance(const BInst *inst, int ninst)
+__global__ void k_badvance(const BInst *inst, int ninst, int k)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ninst) *inst[i].step += 1;
+    if (i < ninst) *inst[i].step += k;
 }
 
 template <int KS>
@@ -239,42 +242,42 @@ static int set_lds_ks(size_t max_lds)
 int batch_set_lds(size_t max_lds) { return set_lds_ks<8>(max_lds) || set_lds_ks<16>(max_lds) ? -1 : 0; }
 
 template <int C>
-static void (*wide_kernel(bool fp64, int ks))(const BInst *, const int *, const int *)
+static void (*wide_kernel(bool fp64, int ks))(const BInst *, const int *, const int *, int)
 {
     if (ks == 8) return fp64 ? k_bhvw<double, C, 8> : k_bhvw<float, C, 8>;
     return fp64 ? k_bhvw<double, C, 16> : k_bhvw<float, C, 16>;
 }
 
 int launch_batch_hv(hipStream_t s, bool fp64, int wide_cols, int ks, const BInst *inst, const int *hv_off,
-                    const int *own, int ntiles, int lds_elems)
+                    const int *own, int ntiles, int lds_elems, int sub)
 {
     if (ntiles <= 0) return 0;
     const size_t lds = (size_t)lds_elems * (fp64 ? 8 : 4);
     const bool wide = wide_cols > 0;
-    void (*f)(const BInst *, const int *, const int *);
+    void (*f)(const BInst *, const int *, const int *, int);
     if (wide) {
         f = wide_cols == 1 ? wide_kernel<1>(fp64, ks) : wide_kernel<kBatchWideCols>(fp64, ks);
     } else {
         if (ks == 8) f = fp64 ? k_bhv<double, double, 8> : k_bhv<float, float, 8>;
         else f = fp64 ? k_bhv<double, double, 16> : k_bhv<float, float, 16>;
     }
-    hipLaunchKernelGGL(f, dim3(ntiles), dim3(wide ? kHvBlock : kBlock), lds, s, inst, hv_off, own);
+    hipLaunchKernelGGL(f, dim3(ntiles), dim3(wide ? kHvBlock : kBlock), lds, s, inst, hv_off, own, sub);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_batch_axpy(hipStream_t s, bool fp64, const BInst *inst, const int *ax_off, const int *own,
-                      int nblocks)
+                      int nblocks, int sub)
 {
     if (nblocks <= 0) return 0;
-    if (fp64) hipLaunchKernelGGL(k_baxpy<double>, dim3(nblocks), dim3(kBlock), 0, s, inst, ax_off, own);
-    else hipLaunchKernelGGL(k_baxpy<float>, dim3(nblocks), dim3(kBlock), 0, s, inst, ax_off, own);
+    if (fp64) hipLaunchKernelGGL(k_baxpy<double>, dim3(nblocks), dim3(kBlock), 0, s, inst, ax_off, own, sub);
+    else hipLaunchKernelGGL(k_baxpy<float>, dim3(nblocks), dim3(kBlock), 0, s, inst, ax_off, own, sub);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_batch_advance(hipStream_t s, const BInst *inst, int ninst)
+int launch_batch_advance(hipStream_t s, const BInst *inst, int ninst, int k)
 {
     if (ninst <= 0) return 0;
-    hipLaunchKernelGGL(k_badvance, dim3((ninst + 255) / 256), dim3(256), 0, s, inst, ninst);
+    hipLaunchKernelGGL(k_badvance, dim3((ninst + 255) / 256), dim3(256), 0, s, inst, ninst, k);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -99,7 +99,7 @@ struct BInst {
     int wide;          // 1: one long column per 512-thread tile; 0: cb columns per 256-thread tile
     int cb;            // sector columns per H.v tile
     int hvn, axn;      // H.v tiles / axpy workgroups of this chain
-    int *step;         // current step j
+    int *step;         // step of the current chunk's first step (a launch adds its offset)
     int *flags;        // [0] stop, [1] nstep
     double *alpha;     // [maxn]
     double *beta;      // [maxn + 1]
@@ -161,11 +161,13 @@ int launch_cf(hipStream_t s, int nw, const double *omega, double e0, int np, con
 // tile); 1 or kBatchWideCols for long-column chains (that many columns per
 // 512-thread tile, tile offsets of every chain a multiple of 8).  ks: hop
 // slots per row of every table of the launched chains (8 or 16).
+// The batched kernels run step *step + sub of every chain; the advance adds
+// k to every chain's step after a chunk of k steps (sub = 0 .. k-1).
 int launch_batch_hv(hipStream_t s, bool fp64, int wide_cols, int ks, const BInst *inst, const int *hv_off,
-                    const int *own, int ntiles, int lds_elems);
+                    const int *own, int ntiles, int lds_elems, int sub);
 int launch_batch_axpy(hipStream_t s, bool fp64, const BInst *inst, const int *ax_off, const int *own,
-                      int nblocks);
-int launch_batch_advance(hipStream_t s, const BInst *inst, int ninst);
+                      int nblocks, int sub);
+int launch_batch_advance(hipStream_t s, const BInst *inst, int ninst, int k);
 // Allow max_lds bytes of dynamic LDS for the batched H.v (call once, outside
 // any graph capture).
 int batch_set_lds(size_t max_lds);
