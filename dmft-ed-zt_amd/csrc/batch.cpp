@@ -14,7 +14,8 @@
 //   CF_READ  G(i w) on its way to the host; then DONE and memory released.
 // Every Lanczos chain in flight (ground state or c+/c, any problem, any
 // step) is advanced by the same launches per step (batch.hip: one batched
-// H.v per launch class, the axpy, the step advance); the stages between
+// H.v per launch class and the axpy; one step advance per chunk of up to 8
+// steps); the stages between
 // chains are short per-problem launches on a second stream.
 // Host-side waits use events polled without blocking, so new problems are
 // admitted while others run (continuous batching).
