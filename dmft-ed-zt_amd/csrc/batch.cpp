@@ -167,7 +167,6 @@ struct edg_batch {
     std::deque<Job *> finished;
     long long next_id = 1;
     double *partial = nullptr;
-    unsigned *ticket = nullptr;
     // chains in the batched launches
     std::vector<Chain *> active;
     bool dirty = true;
@@ -442,9 +441,9 @@ int start_chain(edg_batch *b, Job *J, Chain &C, double *norm_out, int slot)
     h[0] = 0;
     h[1] = I.maxn;
     BCHK(b, hipMemcpyAsync(I.flags, h, 2 * sizeof(int), hipMemcpyHostToDevice, b->side));
-    int rc = b->fp64 ? launch_norm<double>(b->side, (const double *)I.base, C.D, b->partial, b->ticket,
+    int rc = b->fp64 ? launch_norm<double>(b->side, (const double *)I.base, C.D, b->partial,
                                            norm_out, I.scale, nullptr)
-                     : launch_norm<float>(b->side, (const float *)I.base, C.D, b->partial, b->ticket,
+                     : launch_norm<float>(b->side, (const float *)I.base, C.D, b->partial,
                                           norm_out, I.scale, nullptr);
     if (rc) return fail(b, EDG_ERR_HIP, "norm launch failed");
     C.pending = true;  // joins the steps when the host sees J->start_ev complete
@@ -563,11 +562,11 @@ int after_gs(edg_batch *b, Job *J)
     int rc;
     if (b->fp64) {
         rc = launch_combine<double>(b->side, (double *)J->gs, (const double *)J->gsc.bi.base, J->gsc.bi.stride,
-                                    nst, J->coef, D, b->partial, b->ticket, &J->misc[4], &J->misc[5]);
+                                    nst, J->coef, D, b->partial, &J->misc[4], &J->misc[5]);
         rc |= launch_scale<double>(b->side, (double *)J->gs, D, &J->misc[5]);
     } else {
         rc = launch_combine<float>(b->side, (float *)J->gs, (const float *)J->gsc.bi.base, J->gsc.bi.stride,
-                                   nst, J->coef, D, b->partial, b->ticket, &J->misc[4], &J->misc[5]);
+                                   nst, J->coef, D, b->partial, &J->misc[4], &J->misc[5]);
         rc |= launch_scale<float>(b->side, (float *)J->gs, D, &J->misc[5]);
     }
     if (rc) return fail(b, EDG_ERR_HIP, "combine launch failed");
@@ -969,7 +968,6 @@ int edg_batch_create(int device, int precision, int nspin, int nw, const double 
     chk(hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, device));
     chk(hipMalloc((void **)&b->omega, sizeof(double) * nw));
     chk(hipMalloc((void **)&b->partial, sizeof(double) * kPartialCap));
-    chk(hipMalloc((void **)&b->ticket, 256));
     chk(hipEventCreateWithFlags(&b->hdesc_ev[0], hipEventDisableTiming));
     chk(hipEventCreateWithFlags(&b->hdesc_ev[1], hipEventDisableTiming));
     if (!rc) chk(hipMemcpy(b->omega, omega, sizeof(double) * nw, hipMemcpyHostToDevice));
@@ -1024,7 +1022,6 @@ int edg_batch_destroy(edg_batch *b)
     }
     if (b->omega) (void)hipFree(b->omega);
     if (b->partial) (void)hipFree(b->partial);
-    if (b->ticket) (void)hipFree(b->ticket);
     for (auto &e : b->chunk_ev)
         if (e) (void)hipEventDestroy(e);
     if (b->side) (void)hipStreamDestroy(b->side);

@@ -71,7 +71,6 @@ struct edg_ctx {
     DevBuf dparams;  // em | ek | vmk on device
 
     double *partial = nullptr;
-    unsigned *ticket = nullptr;
     Scalars sc{};
     void *scal_mem = nullptr;
 
@@ -92,7 +91,6 @@ struct edg_ctx {
     // filling the first one's kernel tails and host round trips.
     hipStream_t s2 = nullptr;
     double *partial2 = nullptr;
-    unsigned *ticket2 = nullptr;
     Scalars sc2{};
     void *scal_mem2 = nullptr;
     DevBuf ring2, mapbuf2;
@@ -111,15 +109,14 @@ int fail(edg_ctx *c, int code, const std::string &msg)
 struct Lane {
     hipStream_t stream;
     double *partial;
-    unsigned *ticket;
     const Scalars *sc;
     DevBuf *ring, *mapbuf;
 };
 
 Lane lane_of(edg_ctx *c, int k)
 {
-    if (k == 0) return Lane{c->stream, c->partial, c->ticket, &c->sc, &c->ring, &c->mapbuf};
-    return Lane{c->s2, c->partial2, c->ticket2, &c->sc2, &c->ring2, &c->mapbuf2};
+    if (k == 0) return Lane{c->stream, c->partial, &c->sc, &c->ring, &c->mapbuf};
+    return Lane{c->s2, c->partial2, &c->sc2, &c->ring2, &c->mapbuf2};
 }
 
 #define HIPCHK(c, call)                                                                 \
@@ -307,7 +304,7 @@ int lanczos_enqueue(edg_ctx *c, const Lane &L, const HvOp &op, T *base, long lon
     HIPCHK(c, hipMemsetAsync(S.scale, 0, sizeof(double) * (maxn + 1), L.stream));
     int fl[2] = {0, maxn};
     HIPCHK(c, hipMemcpyAsync(S.flags, fl, sizeof(fl), hipMemcpyHostToDevice, L.stream));
-    if (launch_norm<T>(L.stream, slot(0), D, L.partial, L.ticket, &S.misc[0], &S.scale[0], nullptr))
+    if (launch_norm<T>(L.stream, slot(0), D, L.partial, &S.misc[0], &S.scale[0], nullptr))
         return fail(c, EDG_ERR_HIP, "norm launch failed");
     // H.v partials and axpy partials live in separate halves of the buffer:
     // step j's H.v reads step j-1's axpy partials while writing its own
@@ -438,7 +435,7 @@ int diag_sector(edg_ctx *c, int nu, int nd, unsigned long long seed, T *gs_out, 
                              c->stream));
     double *misc = c->sc.misc;
     if (launch_combine<T>(c->stream, gs_out, base, Dp, nstep, (const double *)c->coefbuf.p, D,
-                          c->partial, c->ticket, &misc[4], &misc[5]))
+                          c->partial, &misc[4], &misc[5]))
         return fail(c, EDG_ERR_HIP, "combine launch failed");
     if (launch_scale<T>(c->stream, gs_out, D, &misc[5])) return fail(c, EDG_ERR_HIP, "scale launch failed");
     HvArgs ha = base_args(c);
@@ -811,8 +808,6 @@ int edg_create(const edg_params *p, int device, edg_ctx **out)
     if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return bad();
     if (!dev_malloc((void **)&c->partial, sizeof(double) * 16 * 8192 + 64, device)) return bad();
-    if (!dev_malloc((void **)&c->ticket, 256, device)) return bad();
-    if (hipMemset(c->ticket, 0, 256) != hipSuccess) return bad();
     const int mx = (p->maxnstep > p->maxnstep_green ? p->maxnstep : p->maxnstep_green) + 8;
     if (!dev_malloc(&c->scal_mem, sizeof(double) * (3 * mx + 40) + 64, device)) return bad();
     double *s = (double *)c->scal_mem;
@@ -826,8 +821,6 @@ int edg_create(const edg_params *p, int device, edg_ctx **out)
     // the second stream of the Green's-function chains (default priority)
     if (hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess) return bad();
     if (!dev_malloc((void **)&c->partial2, sizeof(double) * 16 * 8192 + 64, device)) return bad();
-    if (!dev_malloc((void **)&c->ticket2, 256, device)) return bad();
-    if (hipMemset(c->ticket2, 0, 256) != hipSuccess) return bad();
     if (!dev_malloc(&c->scal_mem2, sizeof(double) * (3 * mx + 40) + 64, device)) return bad();
     double *s2 = (double *)c->scal_mem2;
     c->sc2.alpha = s2;
@@ -854,10 +847,8 @@ int edg_destroy(edg_ctx *c)
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (c->partial) (void)hipFree(c->partial);
-    if (c->ticket) (void)hipFree(c->ticket);
     if (c->scal_mem) (void)hipFree(c->scal_mem);
     if (c->partial2) (void)hipFree(c->partial2);
-    if (c->ticket2) (void)hipFree(c->ticket2);
     if (c->scal_mem2) (void)hipFree(c->scal_mem2);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1022,9 +1013,9 @@ int edg_observables(edg_ctx *c, double *occ, double *docc)
     double *out = c->sc.misc + 6;
     int rc = c->P.precision == EDG_FP32
                  ? launch_observables<float>(c->stream, (const float *)c->gs[c->gs_slot].p, op, nullptr,
-                                             c->partial, c->ticket, out)
+                                             c->partial, out)
                  : launch_observables<double>(c->stream, (const double *)c->gs[c->gs_slot].p, op, nullptr,
-                                              c->partial, c->ticket, out);
+                                              c->partial, out);
     if (rc) return fail(c, EDG_ERR_HIP, "observables launch failed");
     const int norb = c->P.norb;
     std::vector<double> h(3 * norb);

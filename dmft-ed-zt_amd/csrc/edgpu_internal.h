@@ -120,7 +120,7 @@ int launch_hv(hipStream_t s, const HvOp &op, int mode, int cols, bool acc64, con
 int launch_alpha(hipStream_t s, const double *partial, int nblk, const double *scale_j,
                  double *alpha_j, int *stop);
 template <typename T>
-int launch_norm(hipStream_t s, const T *x, long long n, double *partial, unsigned *ticket,
+int launch_norm(hipStream_t s, const T *x, long long n, double *partial,
                 double *norm_out, double *scale_out, const int *stop);
 // r <- r - alpha_j s_j u_j with alpha_j = s_j * sum(hv_part) (recorded in
 // alpha[step]); writes partial sums of |r|^2.  Returns the workgroup count.
@@ -132,7 +132,7 @@ template <typename T>
 int launch_start(hipStream_t s, T *x, long long n, unsigned long long seed);
 template <typename T>
 int launch_combine(hipStream_t s, T *out, const T *basis, long long stride, int nvec,
-                   const double *coef, long long n, double *partial, unsigned *ticket,
+                   const double *coef, long long n, double *partial,
                    double *norm_out, double *scale_out);
 template <typename T>
 int launch_scale(hipStream_t s, T *x, long long n, const double *scale);
@@ -147,7 +147,7 @@ int launch_apply_c(hipStream_t s, const T *in, T *out, int nup_out, int ndn_out,
                    int spin, const uint32_t *map, int global_odd);
 template <typename T>
 int launch_observables(hipStream_t s, const T *gs, const HvOp &op, const double *scale,
-                       double *partial, unsigned *ticket, double *out);
+                       double *partial, double *out);
 int launch_cf(hipStream_t s, int nw, const double *omega, double e0, int np, const double *ap,
               const double *bp, int nm, const double *am, const double *bm, double *g,
               int accumulate);
@@ -179,6 +179,5 @@ constexpr int kBatchWide = 1024;
 constexpr int kBatchWideCols = 2;  // columns per long-column tile when they fit LDS
 
 // Number of per-block partial slots the reductions may need.
-long long max_blocks_needed(long long n);
 
 }  // namespace edg

@@ -1041,8 +1041,6 @@ template int launch_hv<double>(hipStream_t, const HvOp &, int, int, bool, const 
 
 // ----------------------------------------------------------- vector ops
 
-long long max_blocks_needed(long long n) { return 2048; }
-
 static inline int grid_for(long long n)
 {
     long long g = (n + (long long)kBlock * 8 - 1) / ((long long)kBlock * 8);
@@ -1052,9 +1050,7 @@ static inline int grid_for(long long n)
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_norm(const T *x, long long n, double *partial,
-                                                 unsigned *ticket, double *norm_out,
-                                                 double *scale_out, const int *stop)
+__global__ __launch_bounds__(kBlock) void k_norm(const T *x, long long n, double *partial, const int *stop)
 {
     if (stop && *stop) return;
     double v[1] = {0.0};
@@ -1067,19 +1063,18 @@ __global__ __launch_bounds__(kBlock) void k_norm(const T *x, long long n, double
 }
 
 template <typename T>
-int launch_norm(hipStream_t s, const T *x, long long n, double *partial, unsigned *ticket,
+int launch_norm(hipStream_t s, const T *x, long long n, double *partial,
                 double *norm_out, double *scale_out, const int *stop)
 {
     const int grid = grid_for(n);
-    hipLaunchKernelGGL(k_norm<T>, dim3(grid), dim3(kBlock), 0, s, x, n, partial, ticket,
-                       norm_out, scale_out, stop);
+    hipLaunchKernelGGL(k_norm<T>, dim3(grid), dim3(kBlock), 0, s, x, n, partial, stop);
     if (hipGetLastError() != hipSuccess) return -1;
     return launch_finish<1>(s, partial, FinArgs{FIN_NORM, grid, nullptr, norm_out, scale_out,
                                                 nullptr, nullptr, 0, 0});
 }
-template int launch_norm<float>(hipStream_t, const float *, long long, double *, unsigned *,
+template int launch_norm<float>(hipStream_t, const float *, long long, double *,
                                 double *, double *, const int *);
-template int launch_norm<double>(hipStream_t, const double *, long long, double *, unsigned *,
+template int launch_norm<double>(hipStream_t, const double *, long long, double *,
                                  double *, double *, const int *);
 
 // Lanczos step j, second half: alpha_j = s_j * sum(hv_part) (every
@@ -1180,8 +1175,7 @@ template int launch_start<double>(hipStream_t, double *, long long, unsigned lon
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_combine(T *out, const T *basis, long long stride,
                                                     int nvec, const double *coef, long long n,
-                                                    double *partial, unsigned *ticket,
-                                                    double *norm_out, double *scale_out)
+                                                    double *partial)
 {
     __shared__ double c[1024];
     for (int j = threadIdx.x; j < nvec && j < 1024; j += kBlock) c[j] = coef[j];
@@ -1200,22 +1194,22 @@ __global__ __launch_bounds__(kBlock) void k_combine(T *out, const T *basis, long
 
 template <typename T>
 int launch_combine(hipStream_t s, T *out, const T *basis, long long stride, int nvec,
-                   const double *coef, long long n, double *partial, unsigned *ticket,
+                   const double *coef, long long n, double *partial,
                    double *norm_out, double *scale_out)
 {
     if (nvec > 1024) return -2;
     const int grid = grid_for(n);
     hipLaunchKernelGGL(k_combine<T>, dim3(grid), dim3(kBlock), 0, s, out, basis, stride,
-                       nvec, coef, n, partial, ticket, norm_out, scale_out);
+                       nvec, coef, n, partial);
     if (hipGetLastError() != hipSuccess) return -1;
     return launch_finish<1>(s, partial, FinArgs{FIN_NORM, grid, nullptr, norm_out, scale_out,
                                                 nullptr, nullptr, 0, 0});
 }
 template int launch_combine<float>(hipStream_t, float *, const float *, long long, int,
-                                   const double *, long long, double *, unsigned *, double *,
+                                   const double *, long long, double *, double *,
                                    double *);
 template int launch_combine<double>(hipStream_t, double *, const double *, long long, int,
-                                    const double *, long long, double *, unsigned *, double *,
+                                    const double *, long long, double *, double *,
                                     double *);
 
 template <typename T>
@@ -1355,12 +1349,12 @@ template int launch_apply_c<double>(hipStream_t, const double *, double *, int, 
 
 // -------------------------------------------------------- observables
 
-// post_process.f90:63-112: out[m] = n(m,up), out[norb+m] = n(m,dn),
-// out[2 norb + m] = docc(m), with gs scaled by *scale.
+// post_process.f90:63-112: the per-workgroup partials of n(m,up), n(m,dn)
+// and docc(m), with gs scaled by *scale; the finisher writes out[m] =
+// n(m,up), out[norb+m] = n(m,dn), out[2 norb + m] = docc(m).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_observables(const T *gs, HvOp op,
-                                                        const double *scale, double *partial,
-                                                        unsigned *ticket, double *out)
+                                                        const double *scale, double *partial)
 {
     constexpr int NS = 3 * kMaxNorb;
     double v[NS];
@@ -1389,20 +1383,20 @@ __global__ __launch_bounds__(kBlock) void k_observables(const T *gs, HvOp op,
 }
 template <typename T>
 int launch_observables(hipStream_t s, const T *gs, const HvOp &op, const double *scale,
-                       double *partial, unsigned *ticket, double *out)
+                       double *partial, double *out)
 {
     const long long n = (long long)op.nup * op.ndn;
     const int grid = grid_for(n);
     hipLaunchKernelGGL(k_observables<T>, dim3(grid), dim3(kBlock), 0, s, gs, op, scale,
-                       partial, ticket, out);
+                       partial);
     if (hipGetLastError() != hipSuccess) return -1;
     return launch_finish<3 * kMaxNorb>(s, partial, FinArgs{FIN_OBS, grid, nullptr, out, nullptr,
                                                            nullptr, nullptr, 0, op.norb});
 }
 template int launch_observables<float>(hipStream_t, const float *, const HvOp &, const double *,
-                                       double *, unsigned *, double *);
+                                       double *, double *);
 template int launch_observables<double>(hipStream_t, const double *, const HvOp &,
-                                        const double *, double *, unsigned *, double *);
+                                        const double *, double *, double *);
 
 // --------------------------------------------------- continued fraction
 
