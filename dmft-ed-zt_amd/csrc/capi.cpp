@@ -489,6 +489,11 @@ int ground_state_t(edg_ctx *c, int nsector, const int *sectors, int *isector, do
     }
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     c->have_gs_ev = true;
+    if (bi == 0) {  // no finite E0 in any sector (NaN comparisons never win)
+        *isector = 0;
+        *e0 = best;
+        return fail(c, EDG_ERR_SOLVER, "no sector gave a finite ground-state energy");
+    }
     c->has_gs = true;
     c->gs_isector = bi;
     *isector = bi;

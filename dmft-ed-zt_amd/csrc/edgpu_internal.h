@@ -41,9 +41,10 @@ struct SpinTab {
     uint32_t *orb = nullptr;       // [norb*norb][n] orbital moves dst<-src
     int *maxcnt = nullptr;         // device scalar: max over cnt
     // compressed hop tables of the single-orbital kernels, [n][KC]:
-    // ccol = target rank, cidx = index into vsrc (bit 7: negative), 64 = zero;
-    // hrec = target rank | (value-table index << 16), the value table being
-    // [idx] = +v, [kMaxHops + 1 + idx] = -v, idx 64 = 0
+    // ccol = target rank, cidx = index into vsrc (bit 7: odd Jordan-Wigner
+    // parity, the element is -vsrc[idx]), 64 = zero; hrec = target rank |
+    // (value-table index << 16), the value table being [idx] = +v,
+    // [kMaxHops + 1 + idx] = -v (v the signed hybridization), idx 64 = 0
     int KC = 0;                    // 8 or 16 (0: not built)
     uint16_t *ccol = nullptr;
     uint8_t *cidx = nullptr;

@@ -156,9 +156,12 @@ __global__ __launch_bounds__(kBlock) void k_spin_table(int nsite, int norb, int 
             if (bi == bj) continue;
             const uint32_t q = p ^ ((1u << i) | (1u << j));
             const uint32_t between = p & ((1u << j) - 1u) & ~((2u << i) - 1u);
-            const double val = (__popc(between) & 1) ? -v : v;
+            const uint32_t odd = __popc(between) & 1u;
+            const double val = odd ? -v : v;
             const uint32_t c = rank_of(q);
-            t.hd[(long long)r * t.K + slot] = HopD{c, (uint32_t)(l * norb + m) | ((val < 0) << 7), val};
+            // pad: element index, bit 7 = odd Jordan-Wigner parity (the
+            // compressed tables negate the signed V_lm stored at that index)
+            t.hd[(long long)r * t.K + slot] = HopD{c, (uint32_t)(l * norb + m) | (odd << 7), val};
             t.hf[(long long)r * t.K + slot] = HopF{c, (float)val};
             ++slot;
         }
@@ -198,8 +201,9 @@ __global__ __launch_bounds__(kBlock) void k_spin_table(int nsite, int norb, int 
 }
 
 // Compressed [n][KC] copies of the first KC slots: 16-bit target rank and
-// 8-bit element index (bit 7 = negative sign, 64 = zero element); and the
-// same pair as one 32-bit record, rank | (value-table index << 16).
+// 8-bit element index (bit 7 = odd parity, the element being minus the
+// signed V at that index; 64 = zero element); and the same pair as one
+// 32-bit record, rank | (value-table index << 16).
 __global__ __launch_bounds__(kBlock) void k_compress(SpinTab t)
 {
     const int r = blockIdx.x * kBlock + threadIdx.x;

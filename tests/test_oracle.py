@@ -92,6 +92,25 @@ def test_hx_matches_jordan_wigner(oracle, dense, norb, nbath, nu, nd, sym):
     assert np.abs(H - H.T).max() == 0.0
 
 
+@pytest.mark.parametrize("norb,nbath,nu,nd", [(1, 5, 3, 3), (2, 3, 2, 3), (2, 4, 3, 3), (3, 2, 3, 2)])
+def test_hx_negative_hybridization_matches_jordan_wigner(oracle, dense, norb, nbath, nu, nd):
+    """V of either sign (one bath site coupled +V / -V to two orbitals): the
+    reference's sgn/sgn2 shortcuts must give parity * V, never parity * |V|."""
+    rng = np.random.default_rng(40 + 10 * norb + nbath)
+    em, ek, vmk = rand_params(rng, norb, nbath)
+    vmk *= rng.choice([-1.0, 1.0], vmk.shape)
+    vmk[0, 0, :] = -np.abs(vmk[0, 0, :])
+    if norb >= 2:
+        vmk[1, 0, :] = np.abs(vmk[0, 0, :])
+    m = oracle.Model(norb, nbath, U=2.3, J=0.4, mu=0.7)
+    b = oracle.Basis(norb + nbath, nu, nd)
+    H = oracle_dense(oracle, m, em, ek, vmk, b)
+    Hd, _ = dense.dense_h(norb, nbath, em, ek, vmk, m.U, m.Up, m.J, m.Jp, m.mu, nu, nd)
+    assert np.abs(H - Hd).max() < 1e-13
+    Ha = oracle_dense(oracle, m, em, ek, np.abs(vmk), b)
+    assert np.abs(H - Ha).max() > 0.1  # the signs are visible element-wise
+
+
 def test_hx_kanamori_fixture(oracle):
     g = np.load(os.path.join(HERE, "golden", "kanamori_ns4.npz"))
     m = oracle.Model(2, 2, U=float(g["U"]), J=float(g["J"]), Up=float(g["Up"]), Jp=float(g["Jp"]),
