@@ -45,6 +45,8 @@ constexpr int kPartialCap = 1 << 17;  // doubles of the shared partial-sum scrat
 constexpr size_t kLdsMax = 152 * 1024;
 constexpr int kClasses = 6;            // H.v launch classes, chain_class()
 constexpr int kChunk = 8;              // batched steps launched per chunk
+constexpr int kGraphAfter = 2;         // chunks of an unchanged chain set before one is captured
+constexpr size_t kDescMin = 1 << 20;   // initial bytes of the descriptor buffer
 constexpr int kAhead = 2;              // chunks the host may run ahead of the device
 constexpr int kChunkEvents = 4;        // > kAhead
 // Problems whose state vector takes kLargeVec bytes or more are admitted at
@@ -185,13 +187,15 @@ struct edg_batch {
     int cls_tiles[kClasses] = {}, cls_lds[kClasses] = {};
     int nax = 0;
     bool use_graphs = true;
-    hipGraphExec_t gexec[kChunk + 1] = {};  // [k]: a chunk of k steps, captured on first use
+    hipGraphExec_t gexec[kChunk + 1] = {};  // [k]: a chunk of k steps of the current chain set
     // replaced graphs may still have launches queued: destroyed once the
     // stream has passed the event recorded after their last launch
     std::vector<std::pair<hipGraphExec_t, hipEvent_t>> retired;
+    int since_rebuild = 0;  // chunks launched since the chain set last changed
     std::vector<double *> pinned_free;
     size_t pinned_n = 0;
     double st_steps = 0, st_chain_steps = 0, st_done = 0, st_graph = 0, st_rebuild = 0;
+    long long st_inst = 0, st_exec_destroyed = 0;  // graph instantiations / execs destroyed
     // H.v tile schedule by size class: the first entry with nsite <= nsite_max
     // applies to every chain of a problem
     struct Sched {
@@ -764,6 +768,25 @@ int chain_class(const BInst &I)
 }
 int class_cols(int c) { return c >= 4 ? 1 : (c >= 2 ? kBatchWideCols : 0); }
 
+// Destroy every graph exec, live and retired.  The caller has drained the
+// stream, so no launch of them is still queued.
+int drop_graphs(edg_batch *b)
+{
+    for (hipGraphExec_t &ge : b->gexec) {
+        if (!ge) continue;
+        if (hipGraphExecDestroy(ge) != hipSuccess) return fail(b, EDG_ERR_HIP, "graph exec destroy failed");
+        ge = nullptr;
+        b->st_exec_destroyed += 1;
+    }
+    for (auto &r : b->retired) {
+        if (hipGraphExecDestroy(r.first) != hipSuccess) return fail(b, EDG_ERR_HIP, "graph exec destroy failed");
+        (void)hipEventDestroy(r.second);
+        b->st_exec_destroyed += 1;
+    }
+    b->retired.clear();
+    return 0;
+}
+
 // Upload the descriptor table of the active chains grouped by launch class,
 // with the tile prefix sums of each class and the axpy prefix sums over all
 // chains.  Every chain's tiles start at a multiple of 8 (kXcds): workgroup b
@@ -825,9 +848,18 @@ int rebuild(edg_batch *b)
         b->hdesc_cap[f] = total * 2;
     }
     if (b->ddesc_cap < total) {
-        if (b->ddesc) BCHK(b, hipFreeAsync(b->ddesc, b->stream));
-        BCHK(b, hipMallocAsync((void **)&b->ddesc, total * 2, b->stream));
-        b->ddesc_cap = total * 2;
+        // One descriptor buffer for the engine's life, rewritten in stream
+        // order; it grows only when the chain set outgrows it.  Captured
+        // graphs hold its address, so before it is released the stream
+        // drains and every graph exec goes.
+        BCHK(b, hipStreamSynchronize(b->stream));
+        if (int rc = drop_graphs(b)) return rc;
+        if (b->ddesc) BCHK(b, hipFree(b->ddesc));
+        b->ddesc = nullptr;
+        b->ddesc_cap = 0;
+        const size_t cap = std::max(total * 2, kDescMin);
+        BCHK(b, hipMalloc((void **)&b->ddesc, cap));
+        b->ddesc_cap = cap;
     }
     char *h = b->hdesc[f];
     for (int i = 0; i < n; ++i) std::memcpy(h + sizeof(BInst) * i, &order[i]->bi, sizeof(BInst));
@@ -855,6 +887,7 @@ int rebuild(edg_batch *b)
     b->nax = ax[n];
     b->dirty = false;
     b->st_rebuild += 1;
+    b->since_rebuild = 0;
     for (hipGraphExec_t &ge : b->gexec) {
         if (!ge) continue;
         hipEvent_t ev;
@@ -864,12 +897,15 @@ int rebuild(edg_batch *b)
         ge = nullptr;
     }
     for (size_t i = 0; i < b->retired.size();) {
-        if (hipEventQuery(b->retired[i].second) == hipSuccess) {
-            (void)hipGraphExecDestroy(b->retired[i].first);
+        const hipError_t q = hipEventQuery(b->retired[i].second);
+        if (q == hipSuccess) {
+            BCHK(b, hipGraphExecDestroy(b->retired[i].first));
             (void)hipEventDestroy(b->retired[i].second);
             b->retired.erase(b->retired.begin() + i);
+            b->st_exec_destroyed += 1;
             continue;
         }
+        if (q != hipErrorNotReady) return fail(b, EDG_ERR_HIP, "event query of a retired graph failed");
         ++i;
     }
     return 0;
@@ -903,7 +939,13 @@ int run_steps(edg_batch *b, int k)
     if (k < 1 || k > kChunk) return fail(b, EDG_ERR_ARG, "chunk length out of range");
     if (b->dirty)
         if (int rc = rebuild(b)) return rc;
-    if (b->use_graphs) {
+    // A graph pays only when it is replayed: chunks right after a change of
+    // the chain set (chains start and finish all the time in a served
+    // stream) launch their kernels directly, and a chunk length is captured
+    // once the set has stayed unchanged for kGraphAfter chunks.
+    const bool graph = b->use_graphs && (b->gexec[k] || b->since_rebuild >= kGraphAfter);
+    b->since_rebuild += 1;
+    if (graph) {
         if (!b->gexec[k]) {
             hipGraph_t g = nullptr;
             BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -917,6 +959,7 @@ int run_steps(edg_batch *b, int k)
                 b->gexec[k] = nullptr;
                 return fail(b, EDG_ERR_HIP, "graph instantiation failed");
             }
+            b->st_inst += 1;
         }
         BCHK(b, hipGraphLaunch(b->gexec[k], b->stream));
         b->st_graph += 1;
@@ -1007,14 +1050,8 @@ int edg_batch_destroy(edg_batch *b)
     for (Job *J : b->running) drop(J);
     for (Job *J : b->finished) drop(J);
     for (double *p : b->pinned_free) (void)hipHostFree(p);
-    for (hipGraphExec_t ge : b->gexec)
-        if (ge) (void)hipGraphExecDestroy(ge);
-    for (auto &r : b->retired) {
-        (void)hipGraphExecDestroy(r.first);
-        (void)hipEventDestroy(r.second);
-    }
-    if (b->ddesc && b->stream) (void)hipFreeAsync(b->ddesc, b->stream);
-    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    (void)drop_graphs(b);
+    if (b->ddesc) (void)hipFree(b->ddesc);
     delete b->heap;  // its chunks go to the device cache for later queues
     for (int f = 0; f < 2; ++f) {
         if (b->hdesc[f]) (void)hipHostFree(b->hdesc[f]);
@@ -1034,8 +1071,7 @@ int edg_batch_release_memory(int device)
 {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return EDG_ERR_ARG;
-    devmem_release_cache(device);
-    return EDG_OK;
+    return devmem_release_cache(device) ? EDG_ERR_HIP : EDG_OK;
 }
 
 const char *edg_batch_last_error(const edg_batch *b) { return b ? b->err.c_str() : "null batch"; }
@@ -1195,6 +1231,17 @@ int edg_batch_stats(const edg_batch *b, double *stats)
     stats[5] = (double)b->running.size();
     stats[6] = b->st_graph;
     stats[7] = b->st_rebuild;
+    return EDG_OK;
+}
+
+int edg_batch_graph_stats(const edg_batch *b, long long *instantiated, long long *destroyed, int *live)
+{
+    if (!b) return EDG_ERR_ARG;
+    int n = (int)b->retired.size();
+    for (hipGraphExec_t ge : b->gexec) n += ge != nullptr;
+    if (instantiated) *instantiated = b->st_inst;
+    if (destroyed) *destroyed = b->st_exec_destroyed;
+    if (live) *live = n;
     return EDG_OK;
 }
 

@@ -134,7 +134,7 @@ bool dev_malloc(void **p, size_t bytes, int device)
     if (hipMalloc(p, bytes) == hipSuccess) return true;
     (void)hipGetLastError();
     *p = nullptr;
-    devmem_release_cache(device);
+    (void)devmem_release_cache(device);
     if (hipMalloc(p, bytes) == hipSuccess) return true;
     (void)hipGetLastError();
     *p = nullptr;

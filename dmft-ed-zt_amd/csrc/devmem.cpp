@@ -201,18 +201,23 @@ size_t devmem_cached_bytes(int device)
     return s;
 }
 
-void devmem_release_cache(int device)
+int devmem_release_cache(int device)
 {
-    if (device < 0 || device >= kMaxDevices) return;
+    if (device < 0 || device >= kMaxDevices) return 0;
     std::vector<DevChunk *> drop;
     {
         std::lock_guard<std::mutex> lk(g_mu);
         drop.swap(g_cache[device]);
     }
+    int failed = 0;
     for (DevChunk *c : drop) {
-        (void)hipFree(c->base);
+        if (hipFree(c->base) != hipSuccess) {
+            (void)hipGetLastError();
+            ++failed;
+        }
         delete c;
     }
+    return failed;
 }
 
 }  // namespace edg

@@ -40,7 +40,8 @@ private:
 
 // Bytes of idle chunks in the device's cache (free for the next queue).
 size_t devmem_cached_bytes(int device);
-// Return the device's idle cached chunks to the driver.
-void devmem_release_cache(int device);
+// Return the device's idle cached chunks to the driver; returns the number
+// of chunks whose hipFree failed.
+int devmem_release_cache(int device);
 
 }  // namespace edg

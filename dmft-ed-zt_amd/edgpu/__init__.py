@@ -55,7 +55,7 @@ EXPORTS = ["edg_default_params", "edg_create", "edg_destroy", "edg_last_error",
            "edg_batch_create", "edg_batch_destroy", "edg_batch_last_error",
            "edg_batch_set_graphs", "edg_batch_problem_bytes", "edg_batch_submit",
            "edg_batch_step", "edg_batch_run", "edg_batch_poll", "edg_batch_stats",
-           "edg_batch_set_schedule", "edg_batch_release_memory"]
+           "edg_batch_set_schedule", "edg_batch_release_memory", "edg_batch_graph_stats"]
 
 _lib = None
 
@@ -108,6 +108,7 @@ def load_library(path=LIB_PATH):
     L.edg_batch_stats.argtypes = [bh, dp]
     L.edg_batch_set_schedule.argtypes = [bh, C.c_int, ip, ip, ip]
     L.edg_batch_release_memory.argtypes = [C.c_int]
+    L.edg_batch_graph_stats.argtypes = [bh, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip]
     for name in EXPORTS:
         if name not in ("edg_default_params", "edg_last_error", "edg_batch_last_error"):
             getattr(L, name).restype = C.c_int
@@ -448,4 +449,8 @@ class Batch:
         self._chk(self._L.edg_batch_stats(self._h, st))
         keys = ("steps", "chain_steps", "finished", "bytes_in_use", "budget", "in_flight",
                 "graph_launches", "rebuilds")
-        return dict(zip(keys, list(st)))
+        out = dict(zip(keys, list(st)))
+        ni, nd, nl = C.c_longlong(0), C.c_longlong(0), C.c_int(0)
+        self._chk(self._L.edg_batch_graph_stats(self._h, C.byref(ni), C.byref(nd), C.byref(nl)))
+        out.update(graphs_instantiated=ni.value, graphs_destroyed=nd.value, graphs_live=nl.value)
+        return out

@@ -217,8 +217,10 @@ int edg_batch_destroy(edg_batch *b);
  * on it (no counterpart in the reference). */
 int edg_batch_release_memory(int device);
 const char *edg_batch_last_error(const edg_batch *b);
-/* Replay the batched step as a captured HIP graph (1, default) or launch
- * its kernels one by one (0). */
+/* Replay the batched steps as captured HIP graphs (1, default) or launch
+ * their kernels one by one (0).  With graphs, a chunk of steps is captured
+ * once the set of chains in flight has stayed unchanged for two chunks and
+ * replayed until it changes; the chunks in between launch directly. */
 int edg_batch_set_graphs(edg_batch *b, int use_graphs);
 /* H.v tile schedule by size class (set while the engine is idle): entry i
  * applies to every chain of problems with nsite <= nsite_max[i] (increasing);
@@ -251,6 +253,12 @@ int edg_batch_poll(edg_batch *b, long long *id, double *e0, double *G, int *nste
  * [3] device bytes in use, [4] budget, [5] problems in flight, [6] graph
  * launches, [7] descriptor rebuilds. */
 int edg_batch_stats(const edg_batch *b, double *stats);
+/* Graph lifecycle counters (no counterpart in the reference): graph execs
+ * instantiated and destroyed so far, and execs alive now (current plus
+ * retired ones whose last launch the stream has not yet passed).  A chunk
+ * is captured only after the chain set has stayed unchanged for two
+ * chunks, so instantiations stay below the descriptor rebuilds. */
+int edg_batch_graph_stats(const edg_batch *b, long long *instantiated, long long *destroyed, int *live);
 
 #ifdef __cplusplus
 }

@@ -14,6 +14,8 @@ Tolerances (written per test):
   alone vs in a mix               bit-identical (a chain's reductions depend
                                   only on its own tiles)
 """
+import time
+
 import numpy as np
 import pytest
 
@@ -174,6 +176,35 @@ def test_result_independent_of_batch(edgpu, oracle, graphs):
         assert st["graph_launches"] > 0
 
 
+def test_graph_lifecycle_bounded(edgpu, oracle):
+    """Graph replay against direct launches on a stable batch (equal
+    problems: the chain set stays unchanged for many chunks) and on a
+    churning one (uneven sizes and lengths admitted through a small budget,
+    so chains start and finish all the time): results bit-identical; graphs
+    are captured only after the chain set has been stable (instantiations <=
+    descriptor rebuilds), and the execs alive at the end stay bounded."""
+    omega = oracle.matsubara(64, 50.0)
+    stable = [problem(edgpu, 7, 1.0 + 0.2 * k, 70 + k, maxnstep=160, maxnstep_green=100) for k in range(6)]
+    churn = [problem(edgpu, nb, 0.5 + 0.4 * k, 80 + k, maxnstep=20 + 7 * (k % 5), maxnstep_green=10 + 5 * (k % 4))
+             for k, nb in enumerate([5, 9, 3, 7, 11, 5, 9, 7, 3, 11, 5, 7])]
+    probe = edgpu.Batch(omega, precision="fp32")
+    need = max(probe.problem_bytes(p) for p in churn)
+    probe.close()
+    for name, probs, budget, step in (("stable", stable, 0, 16), ("churn", churn, 3 * need, 5)):
+        print(f"graph lifecycle {name}", flush=True)
+        rg, sg, _ = run_batch(edgpu, probs, omega, precision="fp32", graphs=True, budget=budget, step=step)
+        re_, se, _ = run_batch(edgpu, probs, omega, precision="fp32", graphs=False, budget=budget, step=step)
+        for a, b in zip(rg, re_):
+            assert a["e0"] == b["e0"]
+            assert np.array_equal(a["G"], b["G"])
+        assert se["graph_launches"] == 0 and se["graphs_instantiated"] == 0
+        assert sg["graphs_instantiated"] <= sg["rebuilds"]
+        assert sg["graphs_instantiated"] <= sg["graph_launches"]
+        assert sg["graphs_live"] <= 16
+        if name == "stable":  # long unchanged stretches: each capture is replayed
+            assert sg["graph_launches"] >= 2 * sg["graphs_instantiated"] > 0
+
+
 def test_breakdown_and_missing_chains(edgpu, oracle, dense):
     """A sector smaller than maxnstep (the chain runs past its dimension on
     rounding noise, as the reference's does) and sectors whose c or c+ chain
@@ -249,13 +280,21 @@ def test_budget_stable_across_queues(edgpu, oracle):
     e1, _ = run(b1)
     b1.close()
     edgpu.release_batch_memory(0)
-    b2 = edgpu.Batch(omega, precision="fp32")
-    budget2 = b2.stats()["budget"]
-    b2.close()
+    # the HIP runtime returns memory freed after use on since-destroyed
+    # streams a few milliseconds later (measured: back within 10 ms), so
+    # the budget of a new queue is read until it shows the chunks
+    tol = 0.05 * 0.85 * peak + 64e6
+    for _ in range(40):
+        b2 = edgpu.Batch(omega, precision="fp32")
+        budget2 = b2.stats()["budget"]
+        b2.close()
+        if abs(budget2 - budget0) < tol:
+            break
+        time.sleep(0.05)
     print(f"budget {budget0:.4e} {budget1:.4e} {budget2:.4e}, peak in use {peak:.3e}", flush=True)
     assert e1 == e0
-    assert abs(budget1 - budget0) < 0.05 * 0.85 * peak + 64e6
-    assert abs(budget2 - budget0) < 0.05 * 0.85 * peak + 64e6
+    assert abs(budget1 - budget0) < tol
+    assert abs(budget2 - budget0) < tol
 
 
 def test_admission_errors(edgpu, oracle):
