@@ -9,26 +9,31 @@ single-orbital Bethe-bath impurity at Ns=14 (Nbath=13), half-filled sector
   spins) of 150 steps each and G(i w_n) on 2048 Matsubara points (beta=100).
 Vectors are FP32 with FP64 dot products, alpha/beta, tridiagonal solve and
 continued fraction.  The same solve in FP64 vectors is timed for the
-FP32-vs-FP64 speed-up and the errors |dE0|, max|dG| (fp32_errors_by_U: the
-same errors at U = 1, 2 and 4).
+FP32-vs-FP64 speed-up.  dE0, max_dG, max_d_occ, max_d_docc are the FP32
+solve's errors against the FP64 CPU oracle's solve of the same problem
+(tests/golden/ns14_oracle.npz); errors_vs_oracle holds them for both
+precisions at U = 1, 2 and 4.
 
 value = ED solves/s over the timed steps (inputs resident on the device).
 roofline = the H.v kernel: algorithmic bytes per launch (x read once + y
 written once, 2*4*D) / average launch time from HIP events on the solver's
 stream.  cpu_baseline = the FP64 C restatement of the reference algorithm
-(oracle/, OpenMP) timed on the host for a bounded number of Lanczos steps
-at the same size and scaled to one solve, on up to 16 host threads and
-(one_core) on one.
+(oracle/, OpenMP) timed on the host: the median of three runs of a few
+Lanczos steps at the same size, scaled to one solve, on up to 16 host
+threads and (one_core) on one; with --cpu-full also three whole oracle
+solves timed end to end.
 
 The same JSON line also carries:
   batched   512 independent single-orbital problems (Ns in {8,10,12}, random
             U, filling and bath) through the batched solver: solves/s;
-  served    a Poisson stream of 200 requests/s (Ns in {10,12,14}) replayed
-            against the wall clock for --serve-seconds: solves/s, p50/p99
-            latency (result time - arrival time);
+  served    a Poisson stream of 200 requests/s (Ns in {10,12,14}, four
+            chains each) replayed against the wall clock for
+            --serve-seconds (60): solves/s, p50/p99 latency (result time -
+            arrival time) overall and by Ns, and the queue depth over time
+            (backlog growth: whether the GPU keeps up with the offered rate);
   kanamori  the two-orbital Kanamori Ns=14 problem, FP32 vs FP64 vectors:
-            solve times, |dE0|, max|dG|, H.v time, and the oracle's ms per
-            Lanczos step on the host cores;
+            solve times, errors against the oracle's solve, H.v time, and
+            the oracle's ms per Lanczos step on the host cores;
   dmft_ns6  the Ns=6 problem through the DMFT loop (bath fit on the
             host) with the FP64 GPU solver and with the oracle: iterations,
             time per solve and per iteration.
@@ -72,14 +77,39 @@ def make_solver(edgpu, prec, U, mu):
 
 def solve(s, em, ek, vmk, om):
     gs = s.ground_state(em, ek, vmk, [(7, 7)])
+    occ, docc = s.observables()
     G, co = s.cluster_green_ftn(em, ek, vmk, om)
-    return gs.val, G, co
+    return gs.val, G, co, occ, docc
 
 
-def cpu_baseline(em, ek, vmk, U, mu, nsteps, threads):
+def oracle_fixture():
+    """The FP64 CPU oracle's solves of the Ns=14 problems (U = 1, 2, 4 and the
+    two-orbital Kanamori one), written by tests/golden/make_ns14.py."""
+    path = os.path.join(ROOT, "tests", "golden", "ns14_oracle.npz")
+    try:
+        return dict(np.load(path))
+    except OSError:
+        return None
+
+
+def errors_vs_oracle(ref, name, res):
+    """|dE0|, max|dG|, max|dn|, |d docc| of a solve (solve()'s tuple) against
+    the oracle's solve `name` of the fixture (None when the fixture lacks it)."""
+    if ref is None or f"{name}_E0" not in ref:
+        return None
+    e0, G, _, occ, docc = res
+    return dict(dE0=float(abs(e0 - float(ref[f"{name}_E0"]))),
+                max_dG=float(np.abs(G[:, :, 0] - ref[f"{name}_G"]).max()),
+                max_d_occ=float(np.abs(occ - ref[f"{name}_occ"]).max()),
+                max_d_docc=float(np.abs(docc - ref[f"{name}_docc"]).max()))
+
+
+def cpu_baseline(em, ek, vmk, U, mu, nsteps, threads, repeats=3):
     """Oracle (FP64 CPU restatement) Lanczos steps at the same size, scaled to
     the reference's work per solve: 2*200+1 ground-state H.v (two-pass
-    eigenvector, lanczos_solver.f90:20-90) + 4 x 150 chain H.v."""
+    eigenvector, lanczos_solver.f90:20-90) + 4 x 150 chain H.v.  Each figure
+    is the median of `repeats` timed runs (nsteps steps on `threads` host
+    threads; one step on one core)."""
     from oracle import oracle as O
     O.build()
     used = O.set_threads(threads)
@@ -87,24 +117,58 @@ def cpu_baseline(em, ek, vmk, U, mu, nsteps, threads):
     b = O.Basis(14, 7, 7)
     v0 = O.start_vector(1, b.ntot)
     O.lanczos_iteration(m, em, ek, vmk, b, v0, 2)  # warm caches / page in
-    t = time.perf_counter()
-    O.lanczos_iteration(m, em, ek, vmk, b, v0, nsteps)
-    dt = (time.perf_counter() - t) / nsteps
-    # the same on one core, fewer steps (the data is already paged in)
+    per = []
+    for _ in range(repeats):
+        t = time.perf_counter()
+        O.lanczos_iteration(m, em, ek, vmk, b, v0, nsteps)
+        per.append((time.perf_counter() - t) / nsteps)
+    dt = float(np.median(per))
     O.set_threads(1)
-    n1 = max(1, nsteps // 4)
-    t = time.perf_counter()
-    O.lanczos_iteration(m, em, ek, vmk, b, v0, n1)
-    dt1 = (time.perf_counter() - t) / n1
+    per1 = []
+    for _ in range(repeats):
+        t = time.perf_counter()
+        O.lanczos_iteration(m, em, ek, vmk, b, v0, 1)
+        per1.append(time.perf_counter() - t)
+    dt1 = float(np.median(per1))
     O.set_threads(threads)
     hv_per_solve = 2 * 200 + 1 + 4 * 150
     return dict(value=1.0 / (dt * hv_per_solve), unit="solves/s", cores=used, kind="port",
-                sample=f"{nsteps} FP64 Lanczos steps of the oracle at Ns=14 sector (7,7) "
-                       f"(D=11778624), {dt * 1e3:.1f} ms/step, scaled to {hv_per_solve} "
-                       f"steps per solve", ms_per_lanczos_step=dt * 1e3,
+                sample=f"median of {repeats} runs of {nsteps} FP64 Lanczos steps of the oracle at "
+                       f"Ns=14 sector (7,7) (D=11778624) on {used} threads, {dt * 1e3:.1f} ms/step, "
+                       f"scaled to the {hv_per_solve} H.v of one solve",
+                ms_per_lanczos_step=dt * 1e3, ms_per_step_runs=[x * 1e3 for x in per],
                 one_core=dict(value=1.0 / (dt1 * hv_per_solve), unit="solves/s",
-                              ms_per_lanczos_step=dt1 * 1e3, steps=n1),
+                              ms_per_lanczos_step=dt1 * 1e3, ms_per_step_runs=[x * 1e3 for x in per1]),
                 host_cpus=os.cpu_count())
+
+
+def cpu_full_solve(em, ek, vmk, U, mu, threads, repeats=3):
+    """The oracle's whole solve at Ns=14, timed end to end (no scaling): the
+    reference's ground_state (200 Lanczos steps, tridiagonal eigensolve,
+    second pass for the eigenvector, lanczos_solver.f90:20-90) and
+    cluster_green_ftn with four chains of 150 steps (both spins), on
+    `threads` host threads; median and all runs, seconds per phase."""
+    from oracle import oracle as O
+    O.build()
+    used = O.set_threads(threads)
+    m = O.Model(1, 13, U=U, mu=mu)
+    om = O.matsubara(2048, 100.0)
+    runs = []
+    for _ in range(repeats):
+        t = time.perf_counter()
+        gs = O.ground_state(m, em, ek, vmk, [(7, 7)], 200, seed=1)
+        t1 = time.perf_counter()
+        O.cluster_green_ftn(m, em, ek, vmk, gs, 150, om, both_spins=True)
+        t2 = time.perf_counter()
+        runs.append((t1 - t, t2 - t1))
+        log(f"  oracle solve: ground state {t1 - t:.1f} s, Green's function {t2 - t1:.1f} s")
+    tot = [a + b for a, b in runs]
+    k = int(np.argsort(tot)[len(tot) // 2])
+    return dict(value=1.0 / tot[k], unit="solves/s", cores=used, kind="port",
+                solve_s=tot[k], gs_phase_s=runs[k][0], green_phase_s=runs[k][1],
+                runs_s=[list(r) for r in runs],
+                sample=f"median of {repeats} whole oracle solves at Ns=14 sector (7,7): 200 GS "
+                       f"steps + second eigenvector pass + 4 chains x 150, on {used} threads")
 
 
 def cpu_kanamori_step_ms(W, nsteps, threads):
@@ -221,26 +285,46 @@ def bench_batched(edgpu, W, S, graphs=True):
 
 def bench_served(edgpu, W, S, seconds, graphs=True):
     """Poisson arrivals at 200 req/s for `seconds` (Ns 10/12/14 with weights
-    0.5/0.35/0.15, U~U[0,8]) replayed against the wall clock; latency =
-    result time - arrival time, solves/s over the whole run (arrivals + drain)."""
+    0.5/0.35/0.15, U~U[0,8], 200 GS steps + 4 chains of 150) replayed
+    against the wall clock; latency = result time - arrival time, solves/s
+    over the whole run (arrivals + drain), and the queue depth over time:
+    whether the GPU keeps up is read from the backlog's growth over the
+    second half of the arrival window."""
     from edgpu import autotune
     om = W.matsubara(2048, 100.0)
     req = W.served_requests(edgpu, rate=200.0, duration=seconds)
     sched, _ = autotune.tune_batch(edgpu, [p for _, p in req], om, sample=8, graphs=graphs)
-    r = S.serve(edgpu, req, om, drain_limit=300.0, log=log, schedule=sched, graphs=graphs)
+    r = S.serve(edgpu, req, om, drain_limit=600.0, log=log, schedule=sched, graphs=graphs)
+    depth = r["depth"]
+    win = [d for d in depth if d[0] <= seconds] or depth[:1]
+    half = [d for d in win if d[0] >= 0.5 * seconds]
+    slope = float(np.polyfit([d[0] for d in half], [d[3] for d in half], 1)[0]) if len(half) >= 3 else None
+    offered = r["submitted"] / seconds if seconds else 0.0
+    ns = np.array([req[k][1].nbath + 1 for k in r["index"]])
+    lat = r["latencies"]
+    by_ns = {str(n): dict(count=int((ns == n).sum()), p50_s=float(np.percentile(lat[ns == n], 50)),
+                          p99_s=float(np.percentile(lat[ns == n], 99)))
+             for n in (10, 12, 14) if (ns == n).any()}
+    every = max(1, int(round(5.0 / 1.0)))
     return dict(workload=f"Poisson 200 req/s for {seconds:g} s, Ns in {{10,12,14}} (0.5/0.35/0.15), "
-                         "U~U[0,8], half filling, 200 GS steps + 2 chains x 150 steps, FP32 vectors; "
-                         "admission by device bytes (stored Krylov basis + rings)",
-                schedule=[list(e) for e in sched],
-                submitted=r["submitted"], completed=r["completed"], wall_s=r["wall"],
-                solves_per_s=r["solves_per_s"], p50_s=r["p50"], p99_s=r["p99"],
-                offered_per_s=r["submitted"] / seconds if seconds else 0.0)
+                         "U~U[0,8], half filling, 200 GS steps + 4 chains x 150 steps (both spins), "
+                         "FP32 vectors; admission by device bytes (stored Krylov basis + rings)",
+                arrival_window_s=seconds, schedule=[list(e) for e in sched],
+                submitted=r["submitted"], completed=r["completed"], unfinished=r["unfinished"],
+                wall_s=r["wall"], drain_s=max(0.0, r["wall"] - seconds),
+                solves_per_s=r["solves_per_s"], p50_s=r["p50"], p99_s=r["p99"], by_ns=by_ns,
+                offered_per_s=offered,
+                completed_in_window_per_s=win[-1][2] / seconds if seconds else 0.0,
+                backlog_at_window_end=win[-1][3], backlog_max=max(d[3] for d in depth),
+                backlog_growth_per_s=slope,
+                keeps_up=bool(slope is not None and slope < 0.02 * offered),
+                depth_every_5s=[[round(d[0], 2), d[1], d[2], d[3]] for d in depth[::every]])
 
 
-def bench_kanamori(edgpu, W):
+def bench_kanamori(edgpu, W, ref):
     """Two-orbital Kanamori impurity, Ns=14 (6 bath sites per orbital),
     sector (7,7), U=3, J=U/4, U'=U-2J; FP32-vector vs FP64 solve with the
-    2x2 G per spin from 8 chains."""
+    2x2 G per spin from 8 chains, each against the oracle's solve."""
     em, ek, vmk, mu, J, Up = W.two_orbital_kanamori(6, 3.0, seed=1)
     om = W.matsubara(2048, 100.0)
     out = {}
@@ -249,19 +333,20 @@ def bench_kanamori(edgpu, W):
                          maxnstep_green=150, precision=prec, both_spins=True, seed=1)
         s.ground_state(em, ek, vmk, [(7, 7)])
         t = time.perf_counter()
-        gs = s.ground_state(em, ek, vmk, [(7, 7)])
-        G, _ = s.cluster_green_ftn(em, ek, vmk, om)
-        out[prec] = (time.perf_counter() - t, gs.val, G, s.observables())
+        res = solve(s, em, ek, vmk, om)
+        out[prec] = (time.perf_counter() - t, res)
         ms, by = s.time_hv(em, ek, vmk, 7, 7, warmup=3, reps=20)
         out[prec + "_hv"] = (ms, by)
         s.close()
-    t32, e32, g32, o32 = out["fp32"]
-    t64, e64, g64, o64 = out["fp64"]
+    t32, r32 = out["fp32"]
+    t64, r64 = out["fp64"]
     return dict(workload="two-orbital Kanamori, Ns=14 (6 bath sites per orbital), sector (7,7), "
                          "D=11778624, U=3, J=0.75, U'=1.5, Jp=J; 200 GS steps + 8 chains x 150",
                 fp32_solve_ms=t32 * 1e3, fp64_solve_ms=t64 * 1e3, fp32_vs_fp64_speedup=t64 / t32,
-                e0=e64, dE0=abs(e32 - e64), max_dG=float(np.abs(g32 - g64).max()),
-                max_d_occ=float(np.abs(o32[0] - o64[0]).max()),
+                e0=r64[0], errors_vs_oracle=dict(fp32=errors_vs_oracle(ref, "kan", r32),
+                                                 fp64=errors_vs_oracle(ref, "kan", r64)),
+                fp32_vs_fp64_gpu=dict(dE0=abs(r32[0] - r64[0]),
+                                      max_dG=float(np.abs(r32[1] - r64[1]).max())),
                 hv_fp32_us=out["fp32_hv"][0] * 1e3,
                 hv_fp32_gbs=out["fp32_hv"][1] / (out["fp32_hv"][0] * 1e-3) / 1e9)
 
@@ -272,11 +357,14 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--U", type=float, default=2.0)
-    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--serve-seconds", type=float, default=3.0,
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="seconds of Poisson arrivals for the served-stream figures (0: skip)")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="also time three whole oracle solves at Ns=14 end to end on the host "
+                         "(about 2 minutes each on 16 cores)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-kanamori", action="store_true")
     ap.add_argument("--no-graphs", action="store_true",
@@ -320,14 +408,17 @@ def main():
     t0 = time.perf_counter()
     for _ in range(args.steps):
         t = time.perf_counter()
-        e32, G32, co32 = solve(s32, em, ek, vmk, om)
+        res32 = solve(s32, em, ek, vmk, om)
         lat.append(time.perf_counter() - t)
     wall = time.perf_counter() - t0
+    e32, G32 = res32[0], res32[1]
     st = s32.last_solve_stats()
     info32 = s32.gs_info()
     steps_per_solve = st["n_steps"]
 
-    out = {}
+    ref = oracle_fixture()
+    uname = f"u{U:g}"
+    out = dict(errors_vs_oracle={f"{U:g}": dict(fp32=errors_vs_oracle(ref, uname, res32))})
     if not args.no_fp64:
         log("FP64 solves")
         s64 = make_solver(edgpu, "fp64", U, mu)
@@ -336,30 +427,32 @@ def main():
         lat64 = []
         for _ in range(max(1, min(args.steps, 3))):
             t = time.perf_counter()
-            e64, G64, co64 = solve(s64, em, ek, vmk, om)
+            res64 = solve(s64, em, ek, vmk, om)
             lat64.append(time.perf_counter() - t)
         hv64_ms, hv64_bytes = s64.time_hv(em, ek, vmk, 7, 7, warmup=10, reps=100)
-        out = dict(fp64_solve_ms=float(np.median(lat64)) * 1e3,
+        out.update(fp64_solve_ms=float(np.median(lat64)) * 1e3,
                    fp32_vs_fp64_speedup=float(np.median(lat64) / np.median(lat)),
-                   dE0=float(abs(e32 - e64)), max_dG=float(np.abs(G32 - G64).max()),
+                   fp32_vs_fp64_gpu=dict(dE0=float(abs(e32 - res64[0])),
+                                         max_dG=float(np.abs(G32 - res64[1]).max())),
                    hv_fp64_gbs=hv64_bytes / (hv64_ms * 1e-3) / 1e9, hv_fp64_us=hv64_ms * 1e3)
+        out["errors_vs_oracle"][f"{U:g}"]["fp64"] = errors_vs_oracle(ref, uname, res64)
         s64.close()
-        # the same comparison at the other interaction strengths of the Ns=14
-        # workload (one solve per precision each)
-        errs = {f"{U:g}": dict(dE0=out["dE0"], max_dG=out["max_dG"])}
-        for U2 in (1.0, 4.0):
+        # the other interaction strengths of the Ns=14 workload (one solve
+        # per precision each), against the oracle's solves
+        for U2 in (1.0, 2.0, 4.0):
             if U2 == U:
                 continue
-            log(f"FP32 vs FP64 at U={U2:g}")
+            log(f"FP32 and FP64 solves at U={U2:g}")
             em2, ek2, vmk2, mu2 = W.single_orbital(13, U2, seed=1)
-            r = {}
+            e = {}
             for prec in ("fp32", "fp64"):
                 s2 = make_solver(edgpu, prec, U2, mu2)
-                r[prec] = solve(s2, em2, ek2, vmk2, om)
+                e[prec] = errors_vs_oracle(ref, f"u{U2:g}", solve(s2, em2, ek2, vmk2, om))
                 s2.close()
-            errs[f"{U2:g}"] = dict(dE0=float(abs(r["fp32"][0] - r["fp64"][0])),
-                                   max_dG=float(np.abs(r["fp32"][1] - r["fp64"][1]).max()))
-        out["fp32_errors_by_U"] = errs
+            out["errors_vs_oracle"][f"{U2:g}"] = e
+    head = out["errors_vs_oracle"][f"{U:g}"]["fp32"] or {}
+    out.update(dE0=head.get("dE0"), max_dG=head.get("max_dG"), max_d_occ=head.get("max_d_occ"),
+               max_d_docc=head.get("max_d_docc"))
 
     extra = {}
     from edgpu import serve as S
@@ -372,7 +465,7 @@ def main():
     threads = min(os.cpu_count() or 1, 16)
     if not args.no_kanamori:
         log("two-orbital Kanamori")
-        extra["kanamori"] = bench_kanamori(edgpu, W)
+        extra["kanamori"] = bench_kanamori(edgpu, W, ref)
         if not args.no_cpu_baseline:
             ms = cpu_kanamori_step_ms(W, 3, threads)
             extra["kanamori"].update(cpu_ms_per_lanczos_step=ms, cpu_cores=threads,
@@ -384,6 +477,9 @@ def main():
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle)")
         cpu = cpu_baseline(em, ek, vmk, U, mu, args.cpu_steps, threads)
+        if args.cpu_full:
+            log("CPU baseline: whole oracle solves")
+            cpu["full_solve"] = cpu_full_solve(em, ek, vmk, U, mu, threads)
 
     lat_ms = np.array(lat) * 1e3
     res = {

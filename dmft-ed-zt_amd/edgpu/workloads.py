@@ -74,10 +74,12 @@ def batched_problems(edgpu, n=512, seed=7, ns_choices=(8, 10, 12), U=(0.0, 6.0),
 
 
 def served_requests(edgpu, rate=200.0, duration=60.0, seed=11, ns_weights=((10, 0.5), (12, 0.35), (14, 0.15)),
-                    U=(0.0, 8.0), maxnstep=200, maxnstep_green=150):
+                    U=(0.0, 8.0), maxnstep=200, maxnstep_green=150, both_spins=True):
     """Poisson arrivals at `rate` per second over `duration` seconds of
     half-filled single-orbital problems: Ns drawn with ns_weights, U ~ U[U],
-    seeded baths eps ~ U[-1,1], V ~ U[0.2,0.6].  Returns [(t_arrival, Problem)]."""
+    seeded baths eps ~ U[-1,1], V ~ U[0.2,0.6]; each solved with the
+    full-size step counts, 200 ground-state steps and four c+/c chains
+    (both spins) of 150.  Returns [(t_arrival, Problem)]."""
     rng = np.random.Generator(np.random.PCG64(seed))
     ns_vals = np.array([w[0] for w in ns_weights])
     p = np.array([w[1] for w in ns_weights], dtype=float)
@@ -96,5 +98,6 @@ def served_requests(edgpu, rate=200.0, duration=60.0, seed=11, ns_weights=((10, 
         ek = np.stack([e, e], axis=1)
         vmk = np.stack([vv, vv], axis=1)[None, :, :]
         out.append((t, edgpu.Problem(nb, u, u / 2.0, em, ek, vmk, ns // 2, ns // 2, maxnstep=maxnstep,
-                                     maxnstep_green=maxnstep_green, seed=seed * 1000003 + i)))
+                                     maxnstep_green=maxnstep_green, both_spins=both_spins,
+                                     seed=seed * 1000003 + i)))
         i += 1
