@@ -17,6 +17,7 @@
 
 namespace edg {
 int tridiag_lowest(int n, const double *d, const double *e, double *lambda, double *z);
+const char *noalloc_last_error();
 }
 
 using namespace edg;
@@ -865,7 +866,7 @@ int edg_destroy(edg_ctx *c)
     return EDG_OK;
 }
 
-const char *edg_last_error(const edg_ctx *c) { return c ? c->err.c_str() : "null context"; }
+const char *edg_last_error(const edg_ctx *c) { return c ? c->err.c_str() : noalloc_last_error(); }
 
 int edg_set_memory_budget(edg_ctx *c, unsigned long long bytes)
 {

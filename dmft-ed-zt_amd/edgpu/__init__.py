@@ -55,7 +55,8 @@ EXPORTS = ["edg_default_params", "edg_create", "edg_destroy", "edg_last_error",
            "edg_batch_create", "edg_batch_destroy", "edg_batch_last_error",
            "edg_batch_set_graphs", "edg_batch_problem_bytes", "edg_batch_submit",
            "edg_batch_step", "edg_batch_run", "edg_batch_poll", "edg_batch_stats",
-           "edg_batch_set_schedule", "edg_batch_release_memory", "edg_batch_graph_stats"]
+           "edg_batch_set_schedule", "edg_batch_release_memory", "edg_batch_graph_stats",
+           "edg_hx_noalloc"]
 
 _lib = None
 
@@ -83,6 +84,8 @@ def load_library(path=LIB_PATH):
     L.edg_spin_table.argtypes = [ctx, dp, dp, dp, C.c_int, C.c_int, ip, ip,
                                  C.POINTER(C.c_uint), ip, dp, ip, dp]
     L.edg_hx.argtypes = [ctx, dp, dp, dp, C.c_int, C.c_int, dp, dp, C.c_int]
+    L.edg_hx_noalloc.argtypes = [C.POINTER(_Params), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                 C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
     L.edg_lanczos_iteration.argtypes = [ctx, dp, dp, dp, C.c_int, C.c_int, dp, ip, dp, dp]
     L.edg_ground_state.argtypes = [ctx, dp, dp, dp, C.c_int, ip, ip, dp]
     L.edg_gs_info.argtypes = [ctx, ip, ip, C.POINTER(C.c_longlong), dp, dp, dp, ip]

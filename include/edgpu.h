@@ -67,7 +67,9 @@ void edg_default_params(edg_params *p);
 /* Create a solver context on HIP device `device`. */
 int edg_create(const edg_params *p, int device, edg_ctx **out);
 int edg_destroy(edg_ctx *ctx);
-/* Human-readable message of the last failing call on ctx (never NULL). */
+/* Human-readable message of the last failing call on ctx (never NULL);
+ * with ctx = NULL, that of the calling thread's last failing
+ * context-free call (edg_hx_noalloc). */
 const char *edg_last_error(const edg_ctx *ctx);
 /* Device-memory budget for admission (bytes); 0 = no limit. */
 int edg_set_memory_budget(edg_ctx *ctx, unsigned long long bytes);
@@ -107,6 +109,26 @@ int edg_spin_table(edg_ctx *ctx, const double *em, const double *ek, const doubl
  * device in the context's precision. */
 int edg_hx(edg_ctx *ctx, const double *em, const double *ek, const double *vmk, int ne_up,
            int ne_dn, const double *x, double *y, int py);
+
+/* edg_hx on device memory that allocates nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Needs no context: p gives the model (precision, norb, nbath, U, J, Up,
+ * Jp, mu as for edg_create); em, ek, vmk are DEVICE arrays in the reference
+ * layout (FP64, column-major); x and y are DEVICE sector vectors of
+ * C(nsite,ne_up)*C(nsite,ne_dn) elements in p->precision (float or
+ * double), not overlapping; y = H x (py = 0) or y += H x (py = 1).  The
+ * per-spin tables go to a static device array of 16 MiB per GPU, held until
+ * the process ends (every sector with norb <= 2 and nsite <= 15 fits; larger
+ * tables are refused with EDG_ERR_ARG).  The kernels are queued on `stream`
+ * (a hipStream_t; NULL = the null stream) on the current device and the call
+ * returns without waiting; it calls no hipMalloc, hipFree, memset or copy,
+ * so it may be captured in a HIP graph.  Calls that overlap on one GPU share
+ * its array and race: serialize them.  Kernels and results are those of
+ * edg_hx with a new context's default schedule.  Messages of failing calls:
+ * edg_last_error(NULL), per host thread. */
+int edg_hx_noalloc(const edg_params *p, const double *em, const double *ek, const double *vmk, int ne_up,
+                   int ne_dn, const void *x, void *y, int py, void *stream);
 
 /* lanczos_iteration (lanczos_solver.f90:92-173): plain 3-term Lanczos from
  * an unnormalized v_init for maxnstep steps.  a, b have maxnstep entries,
