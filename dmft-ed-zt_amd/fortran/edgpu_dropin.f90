@@ -1,8 +1,8 @@
 ! edgpu_dropin.f90 -- drop-in replacements, with the reference's argument
 ! lists, of the ED solver entry points of yw-choi/DMFT-ED-ZT, computed on the
 ! GPU through libedgpu.so.  Compiled INSIDE the reference's build (it uses
-! the reference's modules dmft_params, dmft_grid, ed_basis, eigpair, mpi,
-! utils); see INTEGRATION.md for the Makefile lines.
+! the reference's modules dmft_params, dmft_grid, ed_basis, eigpair, io_units,
+! mpi, utils); see INTEGRATION.md for the Makefile lines.
 !
 !   edgpu_ground_state      replaces ground_state       (ground_state.f90:1)
 !   edgpu_lanczos_iteration replaces lanczos_iteration  (lanczos_solver.f90:92)
@@ -115,19 +115,59 @@ contains
     end subroutine edgpu_lanczos_iteration
 
     ! cluster_green_ftn(em, ek, vmk, gs, G_cl) of dmft_green.f90:47-126.
-    ! The ground state computed by edgpu_ground_state is still on the device.
+    ! The ground state computed by edgpu_ground_state is still on the device;
+    ! when the held one is not gs (another sector or energy, or none), gs%vec
+    ! is gathered and installed first.  The master rank writes g_coeffs.dat
+    ! as the reference does (:66-69,89-109): a header "maxnstep norb", then
+    ! per orbital and spin a line "iorb ispin E0" and maxnstep rows
+    ! "ap bp am bm" (spin-1 coefficients repeated for spin 2 when the
+    ! `even` shortcut skips its chains, :76-80,102-109).
     subroutine edgpu_cluster_green_ftn(em, ek, vmk, gs, G_cl)
-        use dmft_params, only: norb, nbath, nspin, maxnstep
+        use mpi
+        use io_units, only: IO_G_COEFFS, FN_G_COEFFS
+        use dmft_params, only: norb, nbath, nspin, maxnstep, sectors
         use dmft_grid, only: nwloc, omega
         use eigpair, only: eigpair_t
+        use ed_basis, only: basis_t, generate_basis, dealloc_basis
         double precision, intent(in) :: em(norb,2), ek(nbath,2), vmk(norb,nbath,2)
         type(eigpair_t), intent(in) :: gs
         double complex, intent(out) :: G_cl(nwloc,norb,nspin)
-        double precision, allocatable :: coeffs(:)
+        double precision, allocatable :: coeffs(:,:,:,:), full(:)
+        integer(c_int) :: nu, nd, ns, rc
+        integer(c_long_long) :: dim
+        real(c_double) :: e0, er, res
+        type(basis_t) :: basis
+        integer :: iorb, ispin, i
         call ensure_ctx()
-        allocate(coeffs(maxnstep*4*2*norb))
+        rc = edg_gs_info(ctx, nu, nd, dim, er, e0, res, ns)
+        if (rc /= EDG_OK .or. nu /= sectors(gs%isector,1) .or. nd /= sectors(gs%isector,2) .or. &
+            e0 /= gs%val) then
+            call generate_basis(sectors(gs%isector,1), sectors(gs%isector,2), basis)
+            allocate(full(basis%ntot))
+            call mpi_allgatherv(gs%vec, basis%nloc, mpi_double_precision, full, basis%nlocals, &
+                                basis%offsets, mpi_double_precision, comm, mpierr)
+            call check(edg_set_gs(ctx, int(basis%ne_up, c_int), int(basis%ne_down, c_int), gs%val, full, &
+                                  int(basis%ntot, c_long_long)), "cluster_green_ftn")
+            deallocate(full)
+            call dealloc_basis(basis)
+        endif
+        allocate(coeffs(maxnstep,4,2,norb))
         call check(edg_cluster_green(ctx, em, ek, vmk, int(nwloc, c_int), omega, G_cl, coeffs), &
                    "cluster_green_ftn")
+        if (master) then
+            open(unit=IO_G_COEFFS, file=FN_G_COEFFS, status="replace")
+            write(IO_G_COEFFS,*) maxnstep, norb
+            do iorb = 1, norb
+                do ispin = 1, 2
+                    write(IO_G_COEFFS,"(2I4,1x,ES24.16)") iorb, ispin, gs%val
+                    do i = 1, maxnstep
+                        write(IO_G_COEFFS,"(4(ES24.16,1x))") coeffs(i,1,ispin,iorb), coeffs(i,2,ispin,iorb), &
+                            coeffs(i,3,ispin,iorb), coeffs(i,4,ispin,iorb)
+                    enddo
+                enddo
+            enddo
+            close(IO_G_COEFFS)
+        endif
         deallocate(coeffs)
     end subroutine edgpu_cluster_green_ftn
 
