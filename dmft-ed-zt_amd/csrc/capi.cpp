@@ -52,6 +52,7 @@ struct TabEntry {
 struct Scalars {
     double *alpha, *beta, *scale, *misc;  // misc[0] = |v_init|, misc[1..3] RQ sums
     int *flags;                            // flags[0] = stop, flags[1] = nstep
+    FusedCoef *fcoef;                      // [maxn + 1] coefficients of the fused steps
 };
 
 }  // namespace
@@ -64,6 +65,7 @@ struct edg_ctx {
     unsigned long long budget = 0;
     int cols = 4;
     bool acc64 = false;
+    bool fused = false;  // single-orbital Lanczos steps without the axpy pass (HV_FUSED)
     int ncu = 0;
 
     std::vector<double> em, ek, vmk;
@@ -295,7 +297,7 @@ HvArgs base_args(edg_ctx *c)
 // scales s_j (if not null); *vinit_norm receives |v_init|.
 template <typename T>
 int lanczos_enqueue(edg_ctx *c, const Lane &L, const HvOp &op, T *base, long long stride, bool stored,
-                    int maxn)
+                    int maxn, T *rbase)
 {
     const long long D = (long long)op.nup * op.ndn;
     const Scalars &S = *L.sc;
@@ -307,6 +309,36 @@ int lanczos_enqueue(edg_ctx *c, const Lane &L, const HvOp &op, T *base, long lon
     HIPCHK(c, hipMemcpyAsync(S.flags, fl, sizeof(fl), hipMemcpyHostToDevice, L.stream));
     if (launch_norm<T>(L.stream, slot(0), D, L.partial, &S.misc[0], &S.scale[0], nullptr))
         return fail(c, EDG_ERR_HIP, "norm launch failed");
+    if (rbase && c->fused && hv_fused_supported(op, c->cols, sizeof(T))) {
+        // one H.v launch per step on r'_{j-1} (ring of two at rbase) and its
+        // finisher; u_j goes to slot j (stored basis) or j % 3 (ring)
+        auto rslot = [&](int j) -> T * { return rbase + (long long)(j & 1) * stride; };
+        if (launch_fused_start(L.stream, S.scale, S.fcoef)) return fail(c, EDG_ERR_HIP, "fused start launch failed");
+        for (int j = 0; j < maxn; ++j) {
+            HvArgs ha{};
+            ha.x = j == 0 ? slot(0) : rslot(j - 1);
+            ha.y = rslot(j);
+            ha.p = j >= 1 ? slot(j - 1) : nullptr;
+            ha.p2 = j >= 2 ? slot(j - 2) : nullptr;
+            ha.u = j >= 1 ? slot(j) : nullptr;
+            ha.partial = L.partial;
+            ha.step = j;
+            ha.alpha = S.alpha;
+            ha.beta = S.beta;
+            ha.scale = S.scale;
+            ha.stop = S.flags;
+            ha.nstep = S.flags + 1;
+            ha.fcoef = S.fcoef;
+            const int hv_n = launch_hv<T>(L.stream, op, HV_FUSED, c->cols, c->acc64, ha);
+            if (hv_n < 0) return fail(c, EDG_ERR_HIP, "fused H.v launch failed");
+            ++c->n_hv;
+            ++c->n_steps;
+            if (launch_fused_finish(L.stream, L.partial, hv_n, j, S.alpha, S.beta, S.scale, S.flags,
+                                    S.flags + 1, S.fcoef))
+                return fail(c, EDG_ERR_HIP, "fused finisher launch failed");
+        }
+        return 0;
+    }
     // H.v partials and axpy partials live in separate halves of the buffer:
     // step j's H.v reads step j-1's axpy partials while writing its own
     double *hv_part = L.partial, *ax_part = L.partial + kPartialHalf;
@@ -369,9 +401,9 @@ int lanczos_finish(edg_ctx *c, const Lane &L, int maxn, int *nstep, double *a, d
 
 template <typename T>
 int run_lanczos(edg_ctx *c, const Lane &L, const HvOp &op, T *base, long long stride, bool stored, int maxn,
-                int *nstep, double *a, double *b, double *scales, double *vinit_norm)
+                T *rbase, int *nstep, double *a, double *b, double *scales, double *vinit_norm)
 {
-    if (int rc = lanczos_enqueue<T>(c, L, op, base, stride, stored, maxn)) return rc;
+    if (int rc = lanczos_enqueue<T>(c, L, op, base, stride, stored, maxn, rbase)) return rc;
     return lanczos_finish(c, L, maxn, nstep, a, b, scales, vinit_norm);
 }
 
@@ -415,15 +447,18 @@ int diag_sector(edg_ctx *c, int nu, int nd, unsigned long long seed, T *gs_out, 
     if (int rc = make_op(c, nu, nd, &op)) return rc;
     const int maxn = c->P.maxnstep;
     const long long D = (long long)op.nup * op.ndn, Dp = round_up(D, kPad);
-    const size_t need = (size_t)(maxn + 1) * Dp * sizeof(T);
+    // the stored Krylov basis (maxn + 1 slots), and the two residual slots
+    // of the fused step when it is on
+    const size_t need = (size_t)(maxn + (c->fused ? 3 : 1)) * Dp * sizeof(T);
     if (int rc = admit(c, need + 2 * Dp * sizeof(T))) return rc;
     if (int rc = ensure(c, c->basis, need)) return rc;
     T *base = (T *)c->basis.p;
     if (launch_start<T>(c->stream, base, D, seed)) return fail(c, EDG_ERR_HIP, "start launch failed");
     int nstep;
     std::vector<double> a(maxn), b(maxn), s(maxn + 1);
-    if (int rc = run_lanczos<T>(c, lane_of(c, 0), op, base, Dp, true, maxn, &nstep, a.data(), b.data(), s.data(),
-                                nullptr))
+    if (int rc = run_lanczos<T>(c, lane_of(c, 0), op, base, Dp, true, maxn,
+                                c->fused ? base + (long long)(maxn + 1) * Dp : nullptr, &nstep, a.data(),
+                                b.data(), s.data(), nullptr))
         return rc;
     std::vector<double> z(nstep);
     double lam;
@@ -532,8 +567,11 @@ int chain_enqueue(edg_ctx *c, const Lane &L, int iorb, int ispin, int pm, ChainR
     if (int rc = make_op(c, nu, nd, &op)) return rc;
     if (int rc = make_op(c, c->gs_nu, c->gs_nd, &op0)) return rc;
     const long long D = (long long)op.nup * op.ndn, Dp = round_up(D, kPad);
-    if (int rc = admit(c, 3 * Dp * sizeof(T))) return rc;
-    if (int rc = ensure(c, *L.ring, 3 * Dp * sizeof(T))) return rc;
+    // ring of three Krylov vectors, and the two residual slots of the fused
+    // step when it is on
+    const int nring = c->fused ? 5 : 3;
+    if (int rc = admit(c, nring * Dp * sizeof(T))) return rc;
+    if (int rc = ensure(c, *L.ring, nring * Dp * sizeof(T))) return rc;
     if (int rc = ensure(c, *L.mapbuf, sizeof(uint32_t) * 8192)) return rc;
     T *base = (T *)L.ring->p;
     const SpinTab &tout = ispin == 1 ? op.up : op.dn;
@@ -544,7 +582,7 @@ int chain_enqueue(edg_ctx *c, const Lane &L, int iorb, int ispin, int pm, ChainR
     if (launch_apply_c<T>(L.stream, (const T *)c->gs[c->gs_slot].p, base, op.nup, op.ndn, op0.nup, ispin,
                           (const uint32_t *)L.mapbuf->p, godd))
         return fail(c, EDG_ERR_HIP, "apply_c launch failed");
-    if (int rc = lanczos_enqueue<T>(c, L, op, base, Dp, false, maxn)) return rc;
+    if (int rc = lanczos_enqueue<T>(c, L, op, base, Dp, false, maxn, c->fused ? base + 3 * Dp : nullptr)) return rc;
     r->active = true;
     return 0;
 }
@@ -680,11 +718,12 @@ int lanczos_t(edg_ctx *c, int nu, int nd, const double *v_init, int *nstep, doub
     HvOp op;
     if (int rc = make_op(c, nu, nd, &op)) return rc;
     const long long D = (long long)op.nup * op.ndn, Dp = round_up(D, kPad);
-    if (int rc = admit(c, 3 * Dp * sizeof(T))) return rc;
-    if (int rc = ensure(c, c->ring, 3 * Dp * sizeof(T))) return rc;
+    const int nring = c->fused ? 5 : 3;
+    if (int rc = admit(c, nring * Dp * sizeof(T))) return rc;
+    if (int rc = ensure(c, c->ring, nring * Dp * sizeof(T))) return rc;
     if (int rc = upload<T>(c, (T *)c->ring.p, v_init, D)) return rc;
-    return run_lanczos<T>(c, lane_of(c, 0), op, (T *)c->ring.p, Dp, false, c->P.maxnstep, nstep, a, b, nullptr,
-                          nullptr);
+    return run_lanczos<T>(c, lane_of(c, 0), op, (T *)c->ring.p, Dp, false, c->P.maxnstep,
+                          c->fused ? (T *)c->ring.p + 3 * Dp : nullptr, nstep, a, b, nullptr, nullptr);
 }
 
 template <typename T>
@@ -815,25 +854,28 @@ int edg_create(const edg_params *p, int device, edg_ctx **out)
         return bad();
     if (!dev_malloc((void **)&c->partial, sizeof(double) * 16 * 8192 + 64, device)) return bad();
     const int mx = (p->maxnstep > p->maxnstep_green ? p->maxnstep : p->maxnstep_green) + 8;
-    if (!dev_malloc(&c->scal_mem, sizeof(double) * (3 * mx + 40) + 64, device)) return bad();
+    const size_t fco = ((sizeof(double) * (3 * mx + 40) + 255) / 256) * 256;  // FusedCoef table offset
+    if (!dev_malloc(&c->scal_mem, fco + sizeof(FusedCoef) * (mx + 1), device)) return bad();
     double *s = (double *)c->scal_mem;
     c->sc.alpha = s;
     c->sc.beta = s + mx;
     c->sc.scale = s + 2 * mx;
     c->sc.misc = s + 3 * mx;
     c->sc.flags = (int *)(s + 3 * mx + 32);
+    c->sc.fcoef = (FusedCoef *)((char *)c->scal_mem + fco);
     for (auto &e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) return bad();
     // the second stream of the Green's-function chains (default priority)
     if (hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess) return bad();
     if (!dev_malloc((void **)&c->partial2, sizeof(double) * 16 * 8192 + 64, device)) return bad();
-    if (!dev_malloc(&c->scal_mem2, sizeof(double) * (3 * mx + 40) + 64, device)) return bad();
+    if (!dev_malloc(&c->scal_mem2, fco + sizeof(FusedCoef) * (mx + 1), device)) return bad();
     double *s2 = (double *)c->scal_mem2;
     c->sc2.alpha = s2;
     c->sc2.beta = s2 + mx;
     c->sc2.scale = s2 + 2 * mx;
     c->sc2.misc = s2 + 3 * mx;
     c->sc2.flags = (int *)(s2 + 3 * mx + 32);
+    c->sc2.fcoef = (FusedCoef *)((char *)c->scal_mem2 + fco);
     for (auto &e : c->ev2)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return bad();
     *out = c;
@@ -882,6 +924,13 @@ int edg_set_tuning(edg_ctx *c, int cols_per_block, int fp64_accumulate)
         return EDG_ERR_ARG;
     c->cols = cols_per_block;
     c->acc64 = fp64_accumulate != 0;
+    return EDG_OK;
+}
+
+int edg_set_fused_steps(edg_ctx *c, int on)
+{
+    if (!c) return EDG_ERR_ARG;
+    c->fused = on != 0;
     return EDG_OK;
 }
 

@@ -52,6 +52,17 @@ struct SpinTab {
     const double *vsrc = nullptr;  // hybridizations of this spin, [l*norb + m]
 };
 
+// Coefficients of fused Lanczos step j (HV_FUSED), in FP64 and FP32:
+//   [0] s_j  [1] d = alpha_{j-1} - sigma_{j-1}  [2] d s_{j-1}
+//   [3] sigma_{j-1} s_{j-1}  [4] beta_{j-1} s_{j-2}  [5] sigma_j s_j
+//   [6] beta_j s_{j-1}
+// written by the finisher of step j-1 (by the start-vector norm for j = 0)
+// so that the kernel reads them straight into scalar registers.
+struct FusedCoef {
+    double d[8];
+    float f[8];
+};
+
 // Everything the H.v kernel needs besides the vectors.
 struct HvOp {
     int nup, ndn, norb;
@@ -63,8 +74,23 @@ struct HvOp {
 };
 
 // HV_PLAIN: y = Hx (y += Hx when HvArgs::accumulate; the launcher then runs
-// the HV_ACCUM instantiation, so that the plain form never reads y)
-enum HvMode { HV_PLAIN = 0, HV_LANCZOS = 1, HV_RQ = 2, HV_ACCUM = 3 };
+// the HV_ACCUM instantiation, so that the plain form never reads y).
+// HV_FUSED: one Lanczos step without a separate axpy pass (single-orbital
+// tile only).  Step j gathers from the shifted residual of step j-1,
+// r_{j-1} = (H - sigma_{j-1}) q_{j-1} - beta_{j-1} q_{j-2} (u_0 at j = 0),
+// where q_k = s_k u_k and u_k are the stored unnormalized Krylov vectors,
+// and per element forms (d = alpha_{j-1} - sigma_{j-1})
+//   u_j   = r_{j-1} - d q_{j-1}                                        (stored)
+//   H u_j = H r_{j-1} - d (r_{j-1} + sigma_{j-1} q_{j-1} + beta_{j-1} q_{j-2})
+//   r_j   = s_j H u_j - sigma_j q_j - beta_j q_{j-1}                   (stored)
+// with partial sums <u_j, r_j>, |r_j|^2, |u_j|^2, from which the finisher
+// takes alpha_j and beta_{j+1}.  H u_j is never applied to the stored u_j:
+// it is carried by the identity in the middle line, so an error in it is
+// multiplied by d / beta_j at every step.  Without the shift (sigma = 0)
+// that factor is alpha / beta, large for chains that start low in the
+// spectrum (measured FP64 errors of 5e-8 in G); with sigma_j = alpha_{j-1}
+// it is the step-to-step change of alpha over beta.
+enum HvMode { HV_PLAIN = 0, HV_LANCZOS = 1, HV_RQ = 2, HV_ACCUM = 3, HV_FUSED = 4 };
 
 struct HvArgs {
     const void *x;          // input vector (T)
@@ -83,6 +109,14 @@ struct HvArgs {
     double *beta;
     int *stop;              // skip when *stop != 0
     int *nstep;
+    // HV_FUSED: x = r'_{j-1} (u_0 at j = 0), p = u_{j-1}, p2 = u_{j-2},
+    // u = where u_j goes (null at j = 0), y = r'_j; alpha, beta, scale the
+    // chain's coefficient arrays (alpha_{j-1}, beta_{j-1}, beta_j, s_{j-2..j}
+    // written by the finishers of the earlier steps)
+    const void *p2;
+    void *u;
+    double *alpha;
+    struct FusedCoef *fcoef;  // [step]: the step's coefficients, written by the previous finisher
 };
 
 // One Lanczos chain of the batched engine (batch.hip / batch.cpp): its
@@ -120,6 +154,15 @@ int launch_hv(hipStream_t s, const HvOp &op, int mode, int cols, bool acc64, con
 // alpha[step] = scale[step] * sum(partial[0..nblk)) unless stopped.
 int launch_alpha(hipStream_t s, const double *partial, int nblk, const double *scale_j,
                  double *alpha_j, int *stop);
+// Finisher of an HV_FUSED step: alpha[step], beta[step + 1], scale[step + 1]
+// from the three sums per workgroup; beta < 1e-16 stops the chain
+// (*nstep = step + 1), as FIN_BETA does.
+int launch_fused_finish(hipStream_t s, const double *partial, int nblk, int step, double *alpha,
+                        double *beta, double *scale, int *stop, int *nstep, FusedCoef *fc);
+// fc[0] of a fused chain from scale[0] (after the start-vector norm).
+int launch_fused_start(hipStream_t s, const double *scale, FusedCoef *fc);
+// Whether launch_hv runs HV_FUSED for this operator and schedule.
+bool hv_fused_supported(const HvOp &op, int cols, size_t esize);
 template <typename T>
 int launch_norm(hipStream_t s, const T *x, long long n, double *partial,
                 double *norm_out, double *scale_out, const int *stop);

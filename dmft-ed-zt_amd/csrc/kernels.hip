@@ -16,26 +16,39 @@
 
 namespace edg {
 
-enum FinMode { FIN_ALPHA = 0, FIN_BETA = 1, FIN_NORM = 2, FIN_COPY = 3, FIN_OBS = 4 };
+enum FinMode { FIN_ALPHA = 0, FIN_BETA = 1, FIN_NORM = 2, FIN_COPY = 3, FIN_OBS = 4, FIN_FUSED = 5 };
 
 struct FinArgs {
     int mode;
     int nblk;
     const double *sx;   // FIN_ALPHA: alpha = sx * sum
-    double *out0;       // alpha / beta / norm / sums
-    double *out1;       // scale (1/beta, 1/norm)
-    int *stop;          // FIN_BETA: set when beta < 1e-16; FIN_ALPHA/BETA skip when set
+    double *out0;       // alpha / beta / norm / sums (FIN_FUSED: the alpha array)
+    double *out1;       // scale (1/beta, 1/norm) (FIN_FUSED: the beta array)
+    int *stop;          // FIN_BETA / FIN_FUSED: set when beta < 1e-16; FIN_ALPHA/BETA/FUSED skip when set
     int *nstep;
     int step;
     int norb;           // FIN_OBS
+    double *out2;       // FIN_FUSED: the scale array
+    FusedCoef *fc;      // FIN_FUSED: the coefficients of the next step go to fc[step + 1]
 };
+
+__device__ __forceinline__ void put_coef(FusedCoef *fc, const double (&v)[7])
+{
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+        fc->d[i] = v[i];
+        fc->f[i] = (float)v[i];
+    }
+    fc->d[7] = 0.0;
+    fc->f[7] = 0.0f;
+}
 
 constexpr int kFinBlock = 1024;
 
 template <int NS>
 __global__ __launch_bounds__(kFinBlock) void k_finish(const double *partial, FinArgs f)
 {
-    if ((f.mode == FIN_ALPHA || f.mode == FIN_BETA) && f.stop && *f.stop) return;
+    if ((f.mode == FIN_ALPHA || f.mode == FIN_BETA || f.mode == FIN_FUSED) && f.stop && *f.stop) return;
     double v[NS], tot[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) v[s] = 0.0;
@@ -68,6 +81,35 @@ __global__ __launch_bounds__(kFinBlock) void k_finish(const double *partial, Fin
 #pragma unroll
         for (int s = 0; s < NS; ++s) f.out0[s] = tot[s];
         break;
+    case FIN_FUSED:
+        if constexpr (NS >= 3) {
+            // tot = <u_j, r_j>, |r_j|^2, |u_j|^2 for the residual shifted by
+            // sigma_j = alpha_{j-1} (0 at j = 0), r_j = (H - sigma_j) q_j -
+            // beta_j q_{j-1}, q_j = s_j u_j: the reference's alpha_j =
+            // <q_j, H q_j - beta_j q_{j-1}> = s_j tot0 + sigma_j |q_j|^2 and
+            // beta_{j+1}^2 = |r_j - d q_j|^2 with d = alpha_j - sigma_j
+            const double sj = f.out2[f.step];
+            const double sg = f.step > 0 ? f.out0[f.step - 1] : 0.0;
+            const double qq = sj * sj * tot[2];
+            const double al = sj * tot[0] + sg * qq;
+            const double d = al - sg;
+            const double b2 = tot[1] - 2.0 * d * sj * tot[0] + d * d * qq;
+            const double b = b2 > 0.0 ? sqrt(b2) : 0.0;
+            f.out0[f.step] = al;
+            f.out1[f.step + 1] = b;
+            const double sn = b > 0.0 ? 1.0 / b : 0.0;
+            f.out2[f.step + 1] = sn;
+            if (b < 1.0e-16) {
+                *f.stop = 1;
+                *f.nstep = f.step + 1;
+            }
+            // step j+1: s_{j+1}, d = alpha_j - sigma_j, d s_j, sigma_j s_j,
+            // beta_j s_{j-1}, sigma_{j+1} s_{j+1} (sigma_{j+1} = alpha_j), beta_{j+1} s_j
+            const double bj = f.out1[f.step], sjm1 = f.step > 0 ? f.out2[f.step - 1] : 0.0;
+            const double v[7] = {sn, d, d * sj, sg * sj, bj * sjm1, al * sn, b * sj};
+            put_coef(f.fc + f.step + 1, v);
+        }
+        break;
     default:  // FIN_OBS: sums laid out [m], [kMaxNorb+m], [2 kMaxNorb+m]
         for (int m = 0; m < f.norb; ++m) {
             f.out0[m] = tot[m];
@@ -89,7 +131,7 @@ static int finish_hv(hipStream_t s, int mode, int nblk, const HvArgs &a)
 {
     if (mode == HV_RQ)
         if (launch_finish<3>(s, a.partial, FinArgs{FIN_COPY, nblk, nullptr, a.result, nullptr,
-                                                   nullptr, nullptr, 0, 0}))
+                                                   nullptr, nullptr, 0, 0, nullptr}))
             return -1;
     return nblk;
 }
@@ -98,7 +140,28 @@ int launch_alpha(hipStream_t s, const double *partial, int nblk, const double *s
                  double *alpha_j, int *stop)
 {
     return launch_finish<1>(s, partial, FinArgs{FIN_ALPHA, nblk, scale_j, alpha_j, nullptr, stop,
-                                                nullptr, 0, 0});
+                                                nullptr, 0, 0, nullptr});
+}
+
+int launch_fused_finish(hipStream_t s, const double *partial, int nblk, int step, double *alpha,
+                        double *beta, double *scale, int *stop, int *nstep, FusedCoef *fc)
+{
+    return launch_finish<3>(s, partial, FinArgs{FIN_FUSED, nblk, nullptr, alpha, beta, stop, nstep, step,
+                                                0, scale, fc});
+}
+
+__global__ void k_fused_start(const double *scale, FusedCoef *fc)
+{
+    if (threadIdx.x == 0) {
+        const double v[7] = {scale[0], 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        put_coef(fc, v);
+    }
+}
+
+int launch_fused_start(hipStream_t s, const double *scale, FusedCoef *fc)
+{
+    hipLaunchKernelGGL(k_fused_start, dim3(1), dim3(64), 0, s, scale, fc);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 __device__ __forceinline__ uint32_t binom_tab(const uint32_t (*B)[16], int n, int k)
@@ -908,6 +971,7 @@ static int launch_hv1_c(hipStream_t s, const HvOp &op, int mode, const HvArgs &a
     if (mode == HV_PLAIN) f = k_hv1<T, ACC, C, HV_PLAIN, KH>;
     else if (mode == HV_ACCUM) f = k_hv1<T, ACC, C, HV_ACCUM, KH>;
     else if (mode == HV_LANCZOS) f = k_hv1<T, ACC, C, HV_LANCZOS, KH>;
+    else if (mode == HV_FUSED) f = k_hv1<T, ACC, C, HV_FUSED, KH>;
     else f = k_hv1<T, double, C, HV_RQ, KH>;
     if (lds > 64 * 1024) {
         if (hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -985,6 +1049,13 @@ static int launch_hv_a(hipStream_t s, const HvOp &op, int mode, int cols, const 
 {
     const bool kan = op.norb >= 2;
     const int km = op.up.kmax > op.dn.kmax ? op.up.kmax : op.dn.kmax;
+    if (mode == HV_FUSED) {  // the single-orbital tile only
+        if (op.norb != 1 || op.sched >= 16) return -4;
+        if (op.up.KC == 8 && km <= 7) return launch_hv1_k<T, ACC, 7>(s, op, mode, cols, a);
+        if (op.up.KC == 8 && km <= 8) return launch_hv1_k<T, ACC, 8>(s, op, mode, cols, a);
+        if (op.up.KC == 16 && km <= 16) return launch_hv1_k<T, ACC, 16>(s, op, mode, cols, a);
+        return -4;
+    }
     // two orbitals: the specialized kernel (tables with <= 16 slots, LDS for a column and its target)
     if (op.norb == kNorb2 && op.up.KC >= 8 && op.up.KC == op.dn.KC && km <= 16 && op.up.hrec &&
         hv2_lds(op.nup, 1, sizeof(T)) <= kLdsBudget && op.sched < 16) {
@@ -1042,6 +1113,13 @@ int launch_hv(hipStream_t s, const HvOp &op_in, int mode, int cols, bool acc64, 
 }
 template int launch_hv<float>(hipStream_t, const HvOp &, int, int, bool, const HvArgs &);
 template int launch_hv<double>(hipStream_t, const HvOp &, int, int, bool, const HvArgs &);
+
+bool hv_fused_supported(const HvOp &op, int cols, size_t esize)
+{
+    const int km = op.up.kmax > op.dn.kmax ? op.up.kmax : op.dn.kmax;
+    return op.norb == 1 && cols < 16 && ((op.up.KC == 8 && km <= 8) || (op.up.KC == 16 && km <= 16)) &&
+           (size_t)op.nup * esize <= kLdsBudget;
+}
 
 // ----------------------------------------------------------- vector ops
 

@@ -56,7 +56,7 @@ EXPORTS = ["edg_default_params", "edg_create", "edg_destroy", "edg_last_error",
            "edg_batch_set_graphs", "edg_batch_problem_bytes", "edg_batch_submit",
            "edg_batch_step", "edg_batch_run", "edg_batch_poll", "edg_batch_stats",
            "edg_batch_set_schedule", "edg_batch_release_memory", "edg_batch_graph_stats",
-           "edg_hx_noalloc"]
+           "edg_hx_noalloc", "edg_set_fused_steps"]
 
 _lib = None
 
@@ -81,6 +81,7 @@ def load_library(path=LIB_PATH):
     L.edg_set_memory_budget.argtypes = [ctx, C.c_ulonglong]
     L.edg_device_count.argtypes = [ip]
     L.edg_set_tuning.argtypes = [ctx, C.c_int, C.c_int]
+    L.edg_set_fused_steps.argtypes = [ctx, C.c_int]
     L.edg_spin_table.argtypes = [ctx, dp, dp, dp, C.c_int, C.c_int, ip, ip,
                                  C.POINTER(C.c_uint), ip, dp, ip, dp]
     L.edg_hx.argtypes = [ctx, dp, dp, dp, C.c_int, C.c_int, dp, dp, C.c_int]
@@ -209,6 +210,10 @@ class Solver:
 
     def set_tuning(self, cols_per_block, fp64_accumulate=False):
         self._chk(self._L.edg_set_tuning(self._h, int(cols_per_block), int(bool(fp64_accumulate))))
+
+    def set_fused_steps(self, on=True):
+        """One H.v pass per Lanczos step, or H.v plus an axpy pass (default)."""
+        self._chk(self._L.edg_set_fused_steps(self._h, int(bool(on))))
 
     # ---- reference interface
     def spin_table(self, em, ek, vmk, ispin, ne):

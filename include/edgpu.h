@@ -88,6 +88,20 @@ int edg_device_count(int *n);
  * counterpart in the reference (its hx is one scalar loop); set by the
  * autotuner or a caller that has measured. */
 int edg_set_tuning(edg_ctx *ctx, int cols_per_block, int fp64_accumulate);
+/* Form of the Lanczos step (no counterpart in the reference, whose steps
+ * are the three-term recurrence of lanczos_solver.f90:147-170 in separate
+ * vector passes).  on = 0 (default): an H.v pass plus an axpy / norm pass
+ * per step.  on = 1: for one orbital with a one-workgroup-per-column-group
+ * schedule, a single H.v pass per step that gathers from the previous
+ * step's shifted residual (H - alpha_{j-2}) q_{j-1} - beta_{j-1} q_{j-2},
+ * forms the new Krylov vector and the next residual in the same pass, and
+ * takes alpha and beta from three FP64 sums reduced by a one-workgroup
+ * finisher; it holds two more vectors per chain.  Both run the same
+ * recurrence and differ in rounding only.  Measured on MI355X at Ns=14
+ * (FP32), the fused form saves 14% per chain step alone but loses on the
+ * whole solve (62 against 55 ms), whose two concurrent chains overlap the
+ * two-pass form's H.v and axpy passes. */
+int edg_set_fused_steps(edg_ctx *ctx, int on);
 
 /* Per-spin basis and hop tables of one (spin, N) sector half, built on the
  * device: ed_basis.f90:42-114 (generate_basis), the hybridization part of
