@@ -77,7 +77,7 @@ struct edg_ctx {
     Scalars sc{};
     void *scal_mem = nullptr;
 
-    DevBuf basis, ring, gs[2], stage, tmp, coefbuf, gbuf, omegabuf, mapbuf;
+    DevBuf basis, ring, gs[2], stage, tmp, coefbuf, gbuf, omegabuf, mapbuf, sigbuf;
     int gs_slot = 0;
 
     bool has_gs = false;
@@ -891,7 +891,7 @@ int edg_destroy(edg_ctx *c)
     for (auto &kv : c->tabs)
         for (void *p : kv.second.allocs) (void)hipFree(p);
     DevBuf *bufs[] = {&c->dparams, &c->basis, &c->ring, &c->gs[0], &c->gs[1], &c->stage, &c->tmp,
-                      &c->coefbuf, &c->gbuf, &c->omegabuf, &c->mapbuf, &c->ring2, &c->mapbuf2};
+                      &c->coefbuf, &c->gbuf, &c->omegabuf, &c->mapbuf, &c->ring2, &c->mapbuf2, &c->sigbuf};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (c->partial) (void)hipFree(c->partial);
@@ -1056,6 +1056,29 @@ int edg_cluster_green(edg_ctx *c, const double *em, const double *ek, const doub
     if (int rc = set_params(c, em, ek, vmk)) return rc;
     return c->P.precision == EDG_FP32 ? cluster_green_t<float>(c, nw, omega, G_cl, coeffs)
                                       : cluster_green_t<double>(c, nw, omega, G_cl, coeffs);
+}
+
+int edg_self_energy(edg_ctx *c, const double *em, const double *ek, const double *vmk, int nw,
+                    const double *omega, const double *G_cl, double *Sigma, double *Z)
+{
+    if (!c || nw < 1 || !omega || !G_cl || !Sigma) return EDG_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (int rc = set_params(c, em, ek, vmk)) return rc;
+    const int norb = c->P.norb, nspin = c->P.nspin;
+    const size_t ng = (size_t)nw * norb * nspin;
+    // omega | G | Sigma | Z on the device
+    const size_t zoff = 8 * ((size_t)nw + 4 * ng);
+    if (int rc = ensure(c, c->sigbuf, zoff + 8 * (size_t)norb * nspin)) return rc;
+    char *b = (char *)c->sigbuf.p;
+    double *om = (double *)b, *g = om + nw, *sg = g + 2 * ng, *z = (double *)(b + zoff);
+    HIPCHK(c, hipMemcpyAsync(om, omega, 8 * (size_t)nw, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(g, G_cl, 16 * ng, hipMemcpyHostToDevice, c->stream));
+    if (launch_sigma(c->stream, nw, norb, c->P.nbath, nspin, om, (const double *)c->dparams.p, c->P.mu, g, sg, z))
+        return fail(c, EDG_ERR_HIP, "self-energy launch failed");
+    HIPCHK(c, hipMemcpyAsync(Sigma, sg, 16 * ng, hipMemcpyDeviceToHost, c->stream));
+    if (Z) HIPCHK(c, hipMemcpyAsync(Z, z, 8 * (size_t)norb * nspin, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return EDG_OK;
 }
 
 int edg_observables(edg_ctx *c, double *occ, double *docc)

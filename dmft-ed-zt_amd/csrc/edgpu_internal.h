@@ -192,6 +192,8 @@ int launch_apply_c(hipStream_t s, const T *in, T *out, int nup_out, int ndn_out,
 template <typename T>
 int launch_observables(hipStream_t s, const T *gs, const HvOp &op, const double *scale,
                        double *partial, double *out);
+int launch_sigma(hipStream_t s, int nw, int norb, int nbath, int nspin, const double *omega,
+                 const double *params, double mu, const double *G, double *Sigma, double *Z);
 int launch_cf(hipStream_t s, int nw, const double *omega, double e0, int np, const double *ap,
               const double *bp, int nm, const double *am, const double *bm, double *g,
               int accumulate);

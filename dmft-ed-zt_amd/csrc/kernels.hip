@@ -1547,6 +1547,48 @@ __global__ __launch_bounds__(128) void k_cf(int nw, const double *omega, double 
     }
 }
 
+// Self-energy of post_process.f90 dump_sigma (:121-185) and the
+// quasiparticle weight of :40-58, one thread per (w_n, orbital, spin):
+//   D(i w) = sum_l vmk(m,l,s)^2 / (i w - ek(l,s))   (dmft_green.f90:24-45)
+//   Sigma  = i w + mu - em(m,s) - D - 1/G_cl
+//   Z(m,s) = 1 / (1 - Im Sigma(w_1) / w_1)
+// params = em(norb,2) | ek(nbath,2) | vmk(norb,nbath,2), column-major;
+// G and Sigma interleaved complex (nw, norb, nspin), column-major.
+__global__ __launch_bounds__(kBlock) void k_sigma(int nw, int norb, int nbath, int nspin, const double *omega,
+                                                 const double *params, double mu, const double *G,
+                                                 double *Sigma, double *Z)
+{
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= (long long)nw * norb * nspin) return;
+    const int iw = (int)(t % nw), m = (int)((t / nw) % norb), s = (int)(t / ((long long)nw * norb));
+    const double *em = params, *ek = params + 2 * norb, *vmk = params + 2 * norb + 2 * nbath;
+    const double w = omega[iw];
+    double dr = 0.0, di = 0.0;
+    for (int l = 0; l < nbath; ++l) {
+        const double v = vmk[(size_t)s * nbath * norb + (size_t)l * norb + m];
+        const double e = ek[(size_t)s * nbath + l];
+        const double q = v * v / (w * w + e * e);  // v^2 / (i w - e) = v^2 (-e - i w) / (w^2 + e^2)
+        dr -= e * q;
+        di -= w * q;
+    }
+    const double gr = G[2 * t], gi = G[2 * t + 1];
+    const double g2 = gr * gr + gi * gi;
+    const double sr = mu - em[(size_t)s * norb + m] - dr - gr / g2;
+    const double si = w - di + gi / g2;
+    Sigma[2 * t] = sr;
+    Sigma[2 * t + 1] = si;
+    if (iw == 0) Z[(size_t)s * norb + m] = 1.0 / (1.0 - si / w);
+}
+
+int launch_sigma(hipStream_t s, int nw, int norb, int nbath, int nspin, const double *omega,
+                 const double *params, double mu, const double *G, double *Sigma, double *Z)
+{
+    const long long n = (long long)nw * norb * nspin;
+    hipLaunchKernelGGL(k_sigma, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, nw, norb, nbath,
+                       nspin, omega, params, mu, G, Sigma, Z);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_cf(hipStream_t s, int nw, const double *omega, double e0, int np, const double *ap,
               const double *bp, int nm, const double *am, const double *bm, double *g,
               int accumulate)

@@ -56,7 +56,7 @@ EXPORTS = ["edg_default_params", "edg_create", "edg_destroy", "edg_last_error",
            "edg_batch_set_graphs", "edg_batch_problem_bytes", "edg_batch_submit",
            "edg_batch_step", "edg_batch_run", "edg_batch_poll", "edg_batch_stats",
            "edg_batch_set_schedule", "edg_batch_release_memory", "edg_batch_graph_stats",
-           "edg_hx_noalloc", "edg_set_fused_steps"]
+           "edg_hx_noalloc", "edg_set_fused_steps", "edg_self_energy"]
 
 _lib = None
 
@@ -94,6 +94,7 @@ def load_library(path=LIB_PATH):
     L.edg_set_gs.argtypes = [ctx, C.c_int, C.c_int, C.c_double, dp, C.c_longlong]
     L.edg_cluster_green.argtypes = [ctx, dp, dp, dp, C.c_int, dp, dp, dp]
     L.edg_observables.argtypes = [ctx, dp, dp]
+    L.edg_self_energy.argtypes = [ctx, dp, dp, dp, C.c_int, dp, dp, dp, dp]
     L.edg_apply_c.argtypes = [ctx, C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_int, dp]
     L.edg_time_hv.argtypes = [ctx, dp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
     L.edg_last_solve_stats.argtypes = [ctx, dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
@@ -311,6 +312,22 @@ class Solver:
         docc = np.zeros(self.norb)
         self._chk(self._L.edg_observables(self._h, _d(occ), _d(docc)))
         return occ.reshape(self.norb, 2, order="F"), docc
+
+    def self_energy(self, em, ek, vmk, omega, G_cl):
+        """Sigma(nw,norb,nspin) and Z(norb,nspin) of post_process.f90 (dump_sigma
+        :121-185, Z :40-58), computed on the device from G_cl and the bath."""
+        em, ek, vmk = self._params(em, ek, vmk)
+        om = np.ascontiguousarray(omega, dtype=np.float64)
+        nw = om.size
+        G = np.asarray(G_cl, dtype=np.complex128)
+        if G.shape != (nw, self.norb, self.nspin):
+            raise EdgpuError(3, f"G_cl must have shape {(nw, self.norb, self.nspin)}")
+        Gi = np.ascontiguousarray(G.reshape(-1, order="F")).view(np.float64)
+        S = np.zeros(2 * G.size)
+        Z = np.zeros(self.norb * self.nspin)
+        self._chk(self._L.edg_self_energy(self._h, _d(em), _d(ek), _d(vmk), nw, _d(om), _d(Gi), _d(S), _d(Z)))
+        Sc = (S[0::2] + 1j * S[1::2]).reshape(nw, self.norb, self.nspin, order="F")
+        return Sc, Z.reshape(self.norb, self.nspin, order="F")
 
     def apply_c(self, ne_up, ne_dn, vec, pm, iorb, ispin):
         """ed_operator.f90:13-57 apply_c (pm=1 create, pm=2 destroy)."""

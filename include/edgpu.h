@@ -185,6 +185,18 @@ int edg_cluster_green(edg_ctx *ctx, const double *em, const double *ek, const do
  * docc(norb) of the held ground state (post_process.f90:63-112). */
 int edg_observables(edg_ctx *ctx, double *occ, double *docc);
 
+/* Self-energy and quasiparticle weight of post_process.f90 (dump_sigma
+ * :121-185, Z :40-58), computed on the device from G_cl(nw,norb,nspin)
+ * (interleaved complex, as returned by edg_cluster_green) and the bath:
+ *   Sigma(iw,iorb,ispin) = i w + mu - em - D_cl - 1/G_cl,
+ *   D_cl = sum_l vmk^2/(i w - ek) (dmft_green.f90:24-45),
+ *   Z(iorb,ispin) = 1/(1 - Im Sigma(1)/omega(1)) (norb x nspin; may be NULL).
+ * Every orbital's Sigma is kept: the reference's dump_sigma clears the whole
+ * array inside its orbital loop (:161), so its Z is 1 for all but the last
+ * orbital when norb > 1. */
+int edg_self_energy(edg_ctx *ctx, const double *em, const double *ek, const double *vmk, int nw,
+                    const double *omega, const double *G_cl, double *Sigma, double *Z);
+
 /* apply_c (ed_operator.f90:13-76): vec_out = c+_{iorb,ispin} vec (pm = 1)
  * or c_{iorb,ispin} vec (pm = 2), from sector (ne_up, ne_dn) to the sector
  * with one more / one fewer electron of spin ispin. */
