@@ -66,6 +66,8 @@ struct edg_ctx {
     int cols = 4;
     bool acc64 = false;
     bool fused = false;  // single-orbital Lanczos steps without the axpy pass (HV_FUSED)
+    int reorth = 0;      // reorthogonalization of the Green's-function chains: 0 none, 1 partial, 2 full
+    long long ro_events = 0, ro_steps = 0;
     int ncu = 0;
 
     std::vector<double> em, ek, vmk;
@@ -97,6 +99,7 @@ struct edg_ctx {
     Scalars sc2{};
     void *scal_mem2 = nullptr;
     DevBuf ring2, mapbuf2;
+    DevBuf robuf, robuf2;  // reorthogonalization state of the two lanes
     hipEvent_t ev2[2] = {nullptr, nullptr};  // [0] main stream ready, [1] second stream done
 };
 
@@ -113,13 +116,13 @@ struct Lane {
     hipStream_t stream;
     double *partial;
     const Scalars *sc;
-    DevBuf *ring, *mapbuf;
+    DevBuf *ring, *mapbuf, *robuf;
 };
 
 Lane lane_of(edg_ctx *c, int k)
 {
-    if (k == 0) return Lane{c->stream, c->partial, &c->sc, &c->ring, &c->mapbuf};
-    return Lane{c->s2, c->partial2, &c->sc2, &c->ring2, &c->mapbuf2};
+    if (k == 0) return Lane{c->stream, c->partial, &c->sc, &c->ring, &c->mapbuf, &c->robuf};
+    return Lane{c->s2, c->partial2, &c->sc2, &c->ring2, &c->mapbuf2, &c->robuf2};
 }
 
 #define HIPCHK(c, call)                                                                 \
@@ -295,9 +298,14 @@ HvArgs base_args(edg_ctx *c)
 // are queued on the lane's stream without waiting; lanczos_finish waits for
 // them and fills the host arrays a, b (maxn entries, zero-padded) and the
 // scales s_j (if not null); *vinit_norm receives |v_init|.
+// Layout of a lane's reorthogonalization buffer (doubles): omega 3 rows,
+// coef, flags (8 ints), then the partial dots (kReorthDots per vector).
+int reorth_ld(int maxn) { return (maxn + 2 + 7) / 8 * 8; }
+size_t reorth_bytes(int maxn) { return sizeof(double) * (size_t)reorth_ld(maxn) * (4 + kReorthDots) + 64; }
+
 template <typename T>
 int lanczos_enqueue(edg_ctx *c, const Lane &L, const HvOp &op, T *base, long long stride, bool stored,
-                    int maxn, T *rbase)
+                    int maxn, T *rbase, bool ro = false)
 {
     const long long D = (long long)op.nup * op.ndn;
     const Scalars &S = *L.sc;
@@ -309,7 +317,7 @@ int lanczos_enqueue(edg_ctx *c, const Lane &L, const HvOp &op, T *base, long lon
     HIPCHK(c, hipMemcpyAsync(S.flags, fl, sizeof(fl), hipMemcpyHostToDevice, L.stream));
     if (launch_norm<T>(L.stream, slot(0), D, L.partial, &S.misc[0], &S.scale[0], nullptr))
         return fail(c, EDG_ERR_HIP, "norm launch failed");
-    if (rbase && c->fused && hv_fused_supported(op, c->cols, sizeof(T))) {
+    if (rbase && !ro && c->fused && hv_fused_supported(op, c->cols, sizeof(T))) {
         // one H.v launch per step on r'_{j-1} (ring of two at rbase) and its
         // finisher; u_j goes to slot j (stored basis) or j % 3 (ring)
         auto rslot = [&](int j) -> T * { return rbase + (long long)(j & 1) * stride; };
@@ -343,6 +351,28 @@ int lanczos_enqueue(edg_ctx *c, const Lane &L, const HvOp &op, T *base, long lon
     // step j's H.v reads step j-1's axpy partials while writing its own
     double *hv_part = L.partial, *ax_part = L.partial + kPartialHalf;
     int ax_n = 0;
+    ReorthArgs ra{};
+    if (ro) {  // a stored basis, reorthogonalized after each axpy (reorth.hip)
+        if (!stored) return fail(c, EDG_ERR_ARG, "reorthogonalization needs the stored Krylov basis");
+        if (int rc = ensure(c, *L.robuf, reorth_bytes(maxn))) return rc;
+        const int ld = reorth_ld(maxn);
+        double *rb = (double *)L.robuf->p;
+        ra.mode = c->reorth;
+        ra.ld = ld;
+        ra.eps = sizeof(T) == 4 ? 5.9604644775390625e-08 : 1.1102230246251565e-16;  // 2^-24, 2^-53
+        ra.alpha = S.alpha;
+        ra.beta = S.beta;
+        ra.scale = S.scale;
+        ra.ax_part = ax_part;
+        ra.omega = rb;
+        ra.coef = rb + 3 * (size_t)ld;
+        ra.flags = (int *)(rb + 4 * (size_t)ld);
+        ra.dpart = rb + 4 * (size_t)ld + 4;
+        ra.stop = S.flags;
+        static const double kOne = 1.0;  // omega_{0,0}
+        HIPCHK(c, hipMemsetAsync(rb, 0, sizeof(double) * (4 * (size_t)ld + 4), L.stream));
+        HIPCHK(c, hipMemcpyAsync(rb, &kOne, sizeof(double), hipMemcpyHostToDevice, L.stream));
+    }
     for (int j = 0; j < maxn; ++j) {
         HvArgs ha{};
         ha.x = slot(j);
@@ -368,6 +398,12 @@ int lanczos_enqueue(edg_ctx *c, const Lane &L, const HvOp &op, T *base, long lon
         ax_n = launch_axpy_norm<T>(L.stream, slot(j + 1), slot(j), D, hv_part, hv_n, S.scale, S.alpha,
                                    j, ax_part, S.flags);
         if (ax_n < 0) return fail(c, EDG_ERR_HIP, "axpy launch failed");
+        if (ro) {
+            ra.step = j;
+            ra.ax_n = ax_n;
+            if (launch_reorth<T>(L.stream, ra, slot(j + 1), base, stride, D, kReorthDots))
+                return fail(c, EDG_ERR_HIP, "reorthogonalization launch failed");
+        }
     }
     return 0;
 }
@@ -568,8 +604,9 @@ int chain_enqueue(edg_ctx *c, const Lane &L, int iorb, int ispin, int pm, ChainR
     if (int rc = make_op(c, c->gs_nu, c->gs_nd, &op0)) return rc;
     const long long D = (long long)op.nup * op.ndn, Dp = round_up(D, kPad);
     // ring of three Krylov vectors, and the two residual slots of the fused
-    // step when it is on
-    const int nring = c->fused ? 5 : 3;
+    // step when it is on; the whole Krylov basis when it is reorthogonalized
+    const bool ro = c->reorth != 0;
+    const long long nring = ro ? maxn + 1 : (c->fused ? 5 : 3);
     if (int rc = admit(c, nring * Dp * sizeof(T))) return rc;
     if (int rc = ensure(c, *L.ring, nring * Dp * sizeof(T))) return rc;
     if (int rc = ensure(c, *L.mapbuf, sizeof(uint32_t) * 8192)) return rc;
@@ -582,7 +619,8 @@ int chain_enqueue(edg_ctx *c, const Lane &L, int iorb, int ispin, int pm, ChainR
     if (launch_apply_c<T>(L.stream, (const T *)c->gs[c->gs_slot].p, base, op.nup, op.ndn, op0.nup, ispin,
                           (const uint32_t *)L.mapbuf->p, godd))
         return fail(c, EDG_ERR_HIP, "apply_c launch failed");
-    if (int rc = lanczos_enqueue<T>(c, L, op, base, Dp, false, maxn, c->fused ? base + 3 * Dp : nullptr)) return rc;
+    if (int rc = lanczos_enqueue<T>(c, L, op, base, Dp, ro, maxn, !ro && c->fused ? base + 3 * Dp : nullptr, ro))
+        return rc;
     r->active = true;
     return 0;
 }
@@ -595,6 +633,13 @@ int chain_finish(edg_ctx *c, const Lane &L, const ChainRun &r, double *aa, doubl
     double nv;
     if (int rc = lanczos_finish(c, L, r.maxn, nstep, aa, bb, nullptr, &nv)) return rc;
     bb[0] = nv;
+    if (c->reorth && L.robuf->p) {  // reorthogonalized steps of this chain
+        int fl[8];
+        HIPCHK(c, hipMemcpy(fl, (const char *)L.robuf->p + sizeof(double) * 4 * reorth_ld(r.maxn), sizeof(fl),
+                            hipMemcpyDeviceToHost));
+        c->ro_events += fl[2];
+        c->ro_steps += fl[5];
+    }
     return 0;
 }
 
@@ -602,6 +647,7 @@ template <typename T>
 int cluster_green_t(edg_ctx *c, int nw, const double *omega, double *G_cl, double *coeffs)
 {
     if (!c->has_gs) return fail(c, EDG_ERR_STATE, "no ground state held: call edg_ground_state first");
+    c->ro_events = c->ro_steps = 0;
     const int norb = c->P.norb, nspin = c->P.nspin, maxn = c->P.maxnstep_green;
     const bool even = !(nspin == 2 || c->gs_nu != c->gs_nd);
     if (int rc = ensure(c, c->omegabuf, sizeof(double) * nw)) return rc;
@@ -891,7 +937,8 @@ int edg_destroy(edg_ctx *c)
     for (auto &kv : c->tabs)
         for (void *p : kv.second.allocs) (void)hipFree(p);
     DevBuf *bufs[] = {&c->dparams, &c->basis, &c->ring, &c->gs[0], &c->gs[1], &c->stage, &c->tmp,
-                      &c->coefbuf, &c->gbuf, &c->omegabuf, &c->mapbuf, &c->ring2, &c->mapbuf2, &c->sigbuf};
+                      &c->coefbuf, &c->gbuf, &c->omegabuf, &c->mapbuf, &c->ring2, &c->mapbuf2, &c->sigbuf,
+                      &c->robuf, &c->robuf2};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (c->partial) (void)hipFree(c->partial);
@@ -924,6 +971,21 @@ int edg_set_tuning(edg_ctx *c, int cols_per_block, int fp64_accumulate)
         return EDG_ERR_ARG;
     c->cols = cols_per_block;
     c->acc64 = fp64_accumulate != 0;
+    return EDG_OK;
+}
+
+int edg_set_reorthogonalization(edg_ctx *c, int mode)
+{
+    if (!c || mode < 0 || mode > 2) return EDG_ERR_ARG;
+    c->reorth = mode;
+    return EDG_OK;
+}
+
+int edg_reorth_stats(edg_ctx *c, long long *events, long long *steps)
+{
+    if (!c) return EDG_ERR_ARG;
+    if (events) *events = c->ro_events;
+    if (steps) *steps = c->ro_steps;
     return EDG_OK;
 }
 

@@ -59,6 +59,20 @@ __device__ __forceinline__ void block_partial(double (&mine)[NS], double *partia
 
 // Sum of n partials in a fixed order: thread t adds partials t, t+BS, ...,
 // then a fixed block tree; every workgroup of size BS gets the same value.
+// Maximum of v over the workgroup; s holds BS / 64 doubles of LDS (it may be
+// reused by the next call).
+template <int BS>
+__device__ __forceinline__ double block_max(double v, double *s)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x / 64] = v;
+    __syncthreads();
+    double m = s[0];
+    for (int i = 1; i < BS / 64; ++i) m = fmax(m, s[i]);
+    return m;
+}
+
 template <int BS>
 __device__ __forceinline__ double sum_partials(const double *part, int n)
 {

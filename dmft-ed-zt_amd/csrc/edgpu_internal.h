@@ -63,6 +63,28 @@ struct FusedCoef {
     float f[8];
 };
 
+// Reorthogonalization of one Lanczos chain after its axpy (reorth.hip).
+struct ReorthArgs {
+    int mode;              // 1 = partial (omega recurrence), 2 = full (every step)
+    int step;              // j: r = unnormalized u_{j+1}, basis slots 0..j
+    int ld;                // row length of omega / dpart / coef (>= maxn + 2)
+    int ax_n;              // partials of |r|^2 the next H.v reduces (the axpy's count)
+    double eps;            // unit roundoff of the stored vectors
+    const double *alpha, *beta, *scale;
+    double *ax_part;       // |r|^2 partials: read, and rewritten when the step reorthogonalizes
+    double *omega;         // 3 rows of ld: omega rows j-1, j, j+1 (ring by j % 3)
+    int *flags;            // [0] this step reorthogonalizes, [2] count, [3 + j % 2] step j is forced,
+                           // [5] steps run
+    const int *stop;       // the chain's stop flag
+    double *dpart;         // [vector][workgroup] partial dots
+    double *coef;          // [ld] c_k = s_k^2 <u_k, r>
+};
+// Workgroups of the partial dots; the buffer holds kReorthDots * ld doubles.
+constexpr int kReorthDots = 2048;
+template <typename T>
+int launch_reorth(hipStream_t s, const ReorthArgs &a, T *r, const T *base, long long stride, long long n,
+                  int ndot);
+
 // Everything the H.v kernel needs besides the vectors.
 struct HvOp {
     int nup, ndn, norb;

@@ -56,7 +56,8 @@ EXPORTS = ["edg_default_params", "edg_create", "edg_destroy", "edg_last_error",
            "edg_batch_set_graphs", "edg_batch_problem_bytes", "edg_batch_submit",
            "edg_batch_step", "edg_batch_run", "edg_batch_poll", "edg_batch_stats",
            "edg_batch_set_schedule", "edg_batch_release_memory", "edg_batch_graph_stats",
-           "edg_hx_noalloc", "edg_set_fused_steps", "edg_self_energy"]
+           "edg_hx_noalloc", "edg_set_fused_steps", "edg_self_energy", "edg_set_reorthogonalization",
+           "edg_reorth_stats"]
 
 _lib = None
 
@@ -82,6 +83,8 @@ def load_library(path=LIB_PATH):
     L.edg_device_count.argtypes = [ip]
     L.edg_set_tuning.argtypes = [ctx, C.c_int, C.c_int]
     L.edg_set_fused_steps.argtypes = [ctx, C.c_int]
+    L.edg_set_reorthogonalization.argtypes = [ctx, C.c_int]
+    L.edg_reorth_stats.argtypes = [ctx, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L.edg_spin_table.argtypes = [ctx, dp, dp, dp, C.c_int, C.c_int, ip, ip,
                                  C.POINTER(C.c_uint), ip, dp, ip, dp]
     L.edg_hx.argtypes = [ctx, dp, dp, dp, C.c_int, C.c_int, dp, dp, C.c_int]
@@ -211,6 +214,17 @@ class Solver:
 
     def set_tuning(self, cols_per_block, fp64_accumulate=False):
         self._chk(self._L.edg_set_tuning(self._h, int(cols_per_block), int(bool(fp64_accumulate))))
+
+    def set_reorthogonalization(self, mode):
+        """Green's-function chains: 0 plain Lanczos (reference), 1 partial
+        (omega recurrence), 2 full FP64 reorthogonalization."""
+        m = {"none": 0, "partial": 1, "full": 2}.get(mode, mode)
+        self._chk(self._L.edg_set_reorthogonalization(self._h, int(m)))
+
+    def reorth_stats(self):
+        ev, st = C.c_longlong(0), C.c_longlong(0)
+        self._chk(self._L.edg_reorth_stats(self._h, C.byref(ev), C.byref(st)))
+        return dict(events=ev.value, steps=st.value)
 
     def set_fused_steps(self, on=True):
         """One H.v pass per Lanczos step, or H.v plus an axpy pass (default)."""

@@ -13,7 +13,8 @@ PKG = os.path.dirname(HERE)
 REPO = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(HERE, "libedgpu.so")
-SOURCES = ["kernels.hip", "batch.hip", "noalloc.hip", "capi.cpp", "batch.cpp", "tridiag.cpp", "devmem.cpp"]
+SOURCES = ["kernels.hip", "batch.hip", "noalloc.hip", "reorth.hip", "capi.cpp", "batch.cpp", "tridiag.cpp",
+           "devmem.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -102,6 +102,21 @@ int edg_set_tuning(edg_ctx *ctx, int cols_per_block, int fp64_accumulate);
  * whole solve (62 against 55 ms), whose two concurrent chains overlap the
  * two-pass form's H.v and axpy passes. */
 int edg_set_fused_steps(edg_ctx *ctx, int on);
+/* FP64 reorthogonalization of the Green's-function chains (the north star's
+ * option; the reference runs the plain three-term recurrence,
+ * lanczos_solver.f90:147-170, called at dmft_green.f90:218,248).  mode 0
+ * (default): none.  mode 1: partial reorthogonalization, Simon's omega
+ * recurrence (Math. Comp. 42 (1984) 115) estimating |<q_{j+1},q_k>| on the
+ * device each step; when an estimate exceeds sqrt(eps) of the vector
+ * storage, that step and the next are orthogonalized against every stored
+ * chain vector by classical Gram-Schmidt with FP64 dot products and
+ * updates.  mode 2: every step.  Each chain then keeps its whole Krylov
+ * basis on the device (maxnstep_green + 1 vectors) instead of three.  The
+ * ground-state Lanczos run is not affected. */
+int edg_set_reorthogonalization(edg_ctx *ctx, int mode);
+/* Steps of the last edg_cluster_green that were reorthogonalized, and steps
+ * that ran the test (mode 1) or reorthogonalized (mode 2). */
+int edg_reorth_stats(edg_ctx *ctx, long long *events, long long *steps);
 
 /* Per-spin basis and hop tables of one (spin, N) sector half, built on the
  * device: ed_basis.f90:42-114 (generate_basis), the hybridization part of
