@@ -34,6 +34,9 @@ The same JSON line also carries:
   kanamori  the two-orbital Kanamori Ns=14 problem, FP32 vs FP64 vectors:
             solve times, errors against the oracle's solve, H.v time, and
             the oracle's ms per Lanczos step on the host cores;
+  batched_kanamori  two-orbital problems through the batched solver (all 8
+            chains of a problem in shared launches) against the same
+            problems solved one by one: 4 at Ns=14, 48 at Ns=10;
   dmft_ns6  the Ns=6 problem through the DMFT loop (bath fit on the
             host) with the FP64 GPU solver and with the oracle: iterations,
             time per solve and per iteration.
@@ -351,6 +354,62 @@ def bench_kanamori(edgpu, W, ref):
                 hv_fp32_gbs=out["fp32_hv"][1] / (out["fp32_hv"][0] * 1e-3) / 1e9)
 
 
+def bench_batched_kanamori(edgpu, W, ref):
+    """Two-orbital Kanamori problems through the batched solver (every
+    problem's 8 c+/c chains in the shared launches), FP32 vectors, against
+    the same problems solved one by one on the single-problem path:
+    4 problems at Ns=14 (U = 3 with the oracle fixture's bath, and U = 2, 4, 5)
+    and 48 at Ns=10 (4 bath sites per orbital, U~U[1,5])."""
+    om = W.matsubara(2048, 100.0)
+
+    def kan(nb_per_orb, U, seed):
+        em, ek, vmk, mu, J, Up = W.two_orbital_kanamori(nb_per_orb, U, seed=seed)
+        ns = 2 * nb_per_orb + 2
+        return edgpu.Problem(2 * nb_per_orb, U, mu, em, ek, vmk, ns // 2, ns // 2, maxnstep=200,
+                             maxnstep_green=150, both_spins=True, seed=seed, norb=2, J=J, Up=Up, Jp=J)
+
+    def single(prs):
+        t = time.perf_counter()
+        for pr in prs:
+            s = edgpu.Solver(norb=2, nbath=pr.nbath, U=pr.U, J=pr.J, Up=pr.Up, Jp=pr.Jp, mu=pr.mu,
+                             maxnstep=200, maxnstep_green=150, precision="fp32", both_spins=True, seed=pr.seed)
+            em = pr.em.reshape(2, 2, order="F")
+            ek = pr.ek.reshape(pr.nbath, 2, order="F")
+            vmk = pr.vmk.reshape(2, pr.nbath, 2, order="F")
+            s.ground_state(em, ek, vmk, [(pr.nup, pr.ndn)])
+            s.cluster_green_ftn(em, ek, vmk, om)
+            s.close()
+        return time.perf_counter() - t
+
+    def batched(prs):
+        b = edgpu.Batch(om, precision="fp32")
+        t = time.perf_counter()
+        ids = [b.submit(p) for p in prs]
+        b.run()
+        wall = time.perf_counter() - t
+        res = {r["id"]: r for r in b.poll()}
+        b.close()
+        return wall, [res[i] for i in ids]
+
+    rng = np.random.Generator(np.random.PCG64(5))
+    big = [kan(6, 3.0, 1), kan(6, 2.0, 2), kan(6, 4.0, 3), kan(6, 5.0, 4)]
+    small = [kan(4, float(rng.uniform(1.0, 5.0)), 10 + k) for k in range(48)]
+    batched(small[:4])  # warm-up: tables, LDS attributes, memory chunks
+    out = {}
+    for name, prs in (("ns14", big), ("ns10", small)):
+        wb, res = batched(prs)
+        ws = single(prs)
+        out[name] = dict(problems=len(prs), batched_wall_s=wb, batched_solves_per_s=len(prs) / wb,
+                         single_path_wall_s=ws, single_path_solves_per_s=len(prs) / ws,
+                         batched_vs_single=ws / wb)
+        if name == "ns14" and ref is not None and "kan_E0" in ref:
+            out[name]["errors_vs_oracle_u3"] = dict(dE0=float(abs(res[0]["e0"] - float(ref["kan_E0"]))),
+                                                    max_dG=float(np.abs(res[0]["G"][:, :, 0] - ref["kan_G"]).max()))
+    out["workload"] = ("two-orbital Kanamori, half filling, 200 GS steps + 8 chains x 150 steps, FP32 vectors, "
+                       "2048 Matsubara points; 4 problems at Ns=14 (sector (7,7)) and 48 at Ns=10 (sector (5,5))")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -466,6 +525,8 @@ def main():
     if not args.no_kanamori:
         log("two-orbital Kanamori")
         extra["kanamori"] = bench_kanamori(edgpu, W, ref)
+        log("batched two-orbital Kanamori")
+        extra["batched_kanamori"] = bench_batched_kanamori(edgpu, W, ref)
         if not args.no_cpu_baseline:
             ms = cpu_kanamori_step_ms(W, 3, threads)
             extra["kanamori"].update(cpu_ms_per_lanczos_step=ms, cpu_cores=threads,

@@ -1,5 +1,5 @@
-// Batched single-orbital ED solver and request loop (C ABI: edg_batch_* in
-// include/edgpu.h).
+// Batched one- and two-orbital ED solver and request loop (C ABI: edg_batch_*
+// in include/edgpu.h).
 //
 // Each problem moves through the stages of edg_ground_state +
 // edg_cluster_green:
@@ -8,7 +8,8 @@
 //   GS_READ  alpha/beta on their way to the host; then the tridiagonal
 //            eigenvector, gs = sum_j c_j q_j, the FP64 Rayleigh quotient and
 //            the c+/c start vectors (per-problem launches)
-//   CHAINS   the c+/c Lanczos chains (ring of 3 vectors each), batched
+//   CHAINS   the c+/c Lanczos chains of every orbital and spin (ring of 3
+//            vectors each), batched
 //   CH_READ  chain coefficients on their way to the host; then the continued
 //            fractions on the device
 //   CF_READ  G(i w) on its way to the host; then DONE and memory released.
@@ -43,7 +44,8 @@ namespace {
 constexpr long long kPadB = 64;       // element alignment of vectors
 constexpr int kPartialCap = 1 << 17;  // doubles of the shared partial-sum scratch
 constexpr size_t kLdsMax = 152 * 1024;
-constexpr int kClasses = 6;            // H.v launch classes, chain_class()
+constexpr int kClasses = 9;            // H.v launch classes, chain_class()
+constexpr int kMaxChains = 8;          // c+/c chains per problem: 2 orbitals x 2 spins x 2
 constexpr int kChunk = 8;              // batched steps launched per chunk
 constexpr int kGraphAfter = 2;         // chunks of an unchanged chain set before one is captured
 constexpr size_t kDescMin = 1 << 20;   // initial bytes of the descriptor buffer
@@ -66,14 +68,36 @@ long long binom_ll(int n, int k)
     return r;
 }
 
-// Largest hop count of one spin half with ne electrons, single orbital:
-// impurity occupied -> the empty bath sites, impurity empty -> the occupied
-// ones (zero hybridizations only lower it).
-int kmax_single(int nbath, int ne)
+// Largest hop count of one spin half with ne electrons on norb impurity +
+// nbath bath sites: a configuration with a occupied impurity orbitals has
+// a (nbath - (ne - a)) + (norb - a)(ne - a) pairs of unlike occupation
+// (zero hybridizations only lower it).
+// Exact largest hop count of one spin half over its ne-electron patterns
+// (impurity orbital m = bit m, bath site l = bit norb + l): pairs (m, l)
+// with V_ml != 0 and unlike occupation.  At most 2^15 patterns.
+int kmax_exact(int norb, int nbath, int ne, const double *v_s)
+{
+    const int ns = norb + nbath;
+    int best = 0;
+    for (uint32_t p = 0; p < (1u << ns); ++p) {
+        if (__builtin_popcount(p) != ne) continue;
+        int k = 0;
+        for (int m = 0; m < norb; ++m)
+            for (int l = 0; l < nbath; ++l)
+                if (v_s[l * norb + m] != 0.0 && (((p >> m) ^ (p >> (norb + l))) & 1u)) ++k;
+        best = std::max(best, k);
+    }
+    return best;
+}
+
+int kmax_hops(int norb, int nbath, int ne)
 {
     int k = 0;
-    if (ne >= 1 && ne - 1 <= nbath) k = std::max(k, nbath - (ne - 1));
-    if (ne <= nbath) k = std::max(k, ne);
+    for (int a = 0; a <= norb; ++a) {
+        const int bo = ne - a;
+        if (bo < 0 || bo > nbath) continue;
+        k = std::max(k, a * (nbath - bo) + (norb - a) * bo);
+    }
     return k;
 }
 
@@ -85,7 +109,7 @@ struct Chain {
     bool exists = false;
     bool pending = false;  // started on the side stream, not yet in the batched steps
     bool active = false;
-    int ispin = 0, pm = 0;
+    int iorb = 0, ispin = 0, pm = 0;
     long long D = 0;
 };
 
@@ -116,16 +140,17 @@ struct Job {
     size_t bytes = 0;
     bool large = false;  // state vector of kLargeVec bytes or more
     Arena ar;
-    double *dparams = nullptr;  // em(1,2) | ek(nbath,2) | vmk(1,nbath,2)
+    int norb = 1;
+    double *dparams = nullptr;  // em(norb,2) | ek(nbath,2) | vmk(norb,nbath,2)
     SpinTab tab[2][3];          // [spin][ne - N_spin + 1]
     bool has_tab[2][3] = {};
     void *gs = nullptr;
     double *misc = nullptr;     // [1..3] RQ sums, [4] |combine|, [5] 1/|combine|, [8+k] |v_init| of chain k
     double *coef = nullptr;     // device coefficient staging
-    double *gdev = nullptr;     // 2 spins x 2 nw
-    uint32_t *map[4] = {};
+    double *gdev = nullptr;     // norb x 2 spins x 2 nw
+    uint32_t *map[kMaxChains] = {};
     Chain gsc;
-    Chain ch[4];                // (up,+), (up,-), (dn,+), (dn,-)
+    Chain ch[kMaxChains];       // chain 4 (iorb - 1) + k: (up,+), (up,-), (dn,+), (dn,-)
     double *host = nullptr;     // pinned readback / upload area
     size_t host_n = 0;
     hipEvent_t ev = nullptr;
@@ -136,7 +161,7 @@ struct Job {
     double e0 = 0.0;
     int gs_nstep = 0;
     std::vector<double> G;
-    std::vector<double> coeffs;  // (mg, 4, 2) column-major: ap bp am bm per spin
+    std::vector<double> coeffs;  // (mg, 4, 2, norb) column-major: ap bp am bm per spin and orbital
 };
 
 }  // namespace
@@ -184,7 +209,8 @@ struct edg_batch {
     const int *d_axoff = nullptr, *d_axown = nullptr;
     const int *d_hvoff[kClasses] = {}, *d_hvown[kClasses] = {};
     int cls_start[kClasses] = {}, cls_n[kClasses] = {};
-    int cls_tiles[kClasses] = {}, cls_lds[kClasses] = {};
+    int cls_tiles[kClasses] = {};
+    size_t cls_lds[kClasses] = {};  // bytes
     int nax = 0;
     bool use_graphs = true;
     hipGraphExec_t gexec[kChunk + 1] = {};  // [k]: a chunk of k steps of the current chain set
@@ -218,7 +244,8 @@ int fail(edg_batch *b, int code, const std::string &msg)
         if (e_ != hipSuccess) return fail(b, EDG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-int nsite_of(const edg_problem &p) { return 1 + p.nbath; }
+int norb_of(const edg_problem &p) { return p.norb >= 2 ? 2 : 1; }
+int nsite_of(const edg_problem &p) { return norb_of(p) + p.nbath; }
 int green_len(const edg_problem &p) { return p.maxnstep_green > 0 ? p.maxnstep_green : p.maxnstep; }
 
 // Chains a problem runs: spin-down chains only when spin symmetry does not
@@ -229,12 +256,21 @@ bool runs_spin2(const edg_batch *b, const edg_problem &p)
     return !even || p.both_spins;
 }
 
-size_t tab_bytes(int n, int nbath)
+size_t tab_bytes(int n, int norb, int nbath)
 {
-    const size_t K = std::max(nbath, kMinSlots);
+    const size_t K = std::max(norb * nbath, kMinSlots);
+    const size_t orb = norb >= 2 ? 4 * (size_t)norb * norb * n + 256 : 0;
     return 4 * (size_t)n + K * n * (sizeof(HopF) + sizeof(HopD)) + n + 4 + 8 * (size_t)n + 4 * (size_t)n +
-           n + 16 * (size_t)n * (2 + 1 + 4) + 16 * 256;
+           n + 16 * (size_t)n * (2 + 1 + 4) + orb + 16 * 256;
 }
+
+// Pinned host area of a problem (doubles): the misc sums, the raw arrays of
+// each chain read back, the coefficient staging and G; the last 16 doubles
+// hold the flag pairs uploaded to the chains.
+size_t host_chain(int k, int mg) { return 16 + (size_t)k * (3 * mg + 4); }
+size_t host_coef(int mg) { return host_chain(kMaxChains, mg); }
+size_t host_g(int mg) { return host_coef(mg) + 16 * (size_t)mg; }  // (ap bp am bm) x 2 spins x 2 orbitals
+constexpr int kMaxLen = 1000;  // longest Lanczos run of a problem
 
 size_t state_bytes(int maxn, int hvn, int axn)
 {
@@ -252,8 +288,12 @@ int wide_cols(const edg_batch *b, long long nup)
 {
     return (size_t)kBatchWideCols * nup * b->es <= kLdsMax ? kBatchWideCols : 1;
 }
-Shape chain_shape(const edg_batch *b, int nsite, long long nup)
+// Two-orbital chains: the single-problem default of kBatchKanColsF32
+// columns per tile with FP32 vectors (FP64 accumulation), one with FP64.
+int kan_cols(const edg_batch *b) { return b->fp64 ? 1 : kBatchKanColsF32; }
+Shape chain_shape(const edg_batch *b, int nsite, long long nup, int norb = 1)
 {
+    if (norb >= 2) return {1, kan_cols(b)};
     for (const auto &e : b->sched)
         if (nsite <= e.nsite_max) {
             const int wide = e.wide < 0 ? (nup >= kBatchWide) : e.wide;
@@ -265,9 +305,9 @@ Shape chain_shape(const edg_batch *b, int nsite, long long nup)
     return {wide, wide ? wide_cols(b, nup) : 1};
 }
 
-int chain_hvn(const edg_batch *b, int nsite, long long nup, long long ndn)
+int chain_hvn(const edg_batch *b, int nsite, long long nup, long long ndn, int norb = 1)
 {
-    const Shape sh = chain_shape(b, nsite, nup);
+    const Shape sh = chain_shape(b, nsite, nup, norb);
     return (int)((ndn + sh.cb - 1) / sh.cb);
 }
 
@@ -283,52 +323,54 @@ int chain_axn(long long D)
 // Device bytes of a problem in flight.
 size_t problem_bytes(const edg_batch *b, const edg_problem &p)
 {
-    const int ns = nsite_of(p);
-    size_t bytes = 64 * 1024 + 8 * (size_t)(2 + 4 * p.nbath);
+    const int ns = nsite_of(p), no = norb_of(p);
+    size_t bytes = 64 * 1024 + 8 * (size_t)(2 * no + 2 * p.nbath + 2 * no * p.nbath);
     for (int s = 0; s < 2; ++s) {
         const int n0 = s == 0 ? p.nup : p.ndn;
         for (int d = -1; d <= 1; ++d)
-            if (n0 + d >= 0 && n0 + d <= ns) bytes += tab_bytes((int)binom_ll(ns, n0 + d), p.nbath);
+            if (n0 + d >= 0 && n0 + d <= ns) bytes += tab_bytes((int)binom_ll(ns, n0 + d), no, p.nbath);
     }
     const long long nu = binom_ll(ns, p.nup), nd = binom_ll(ns, p.ndn);
     const long long D = nu * nd, Dp = rup(D, kPadB);
     bytes += ((size_t)p.maxnstep + 1) * Dp * b->es + Dp * b->es;
-    bytes += state_bytes(p.maxnstep, chain_hvn(b, ns, nu, nd), chain_axn(D));
+    bytes += state_bytes(p.maxnstep, chain_hvn(b, ns, nu, nd, no), chain_axn(D));
     const int mg = green_len(p);
-    bytes += 64 * 8 + 8 * (8 * (size_t)std::max(mg, p.maxnstep) + 64) + 8 * 4 * (size_t)b->nw;
-    for (int k = 0; k < 4 && !p.gs_only; ++k) {
-        const int ispin = k < 2 ? 1 : 2, pm = (k & 1) ? 2 : 1;
+    bytes += 64 * 8 + 8 * (8 * (size_t)no * std::max(mg, p.maxnstep) + 64) + 8 * 4 * (size_t)no * b->nw;
+    for (int k = 0; k < 4 * no && !p.gs_only; ++k) {
+        const int ispin = (k & 3) < 2 ? 1 : 2, pm = (k & 1) ? 2 : 1;
         const int cu = p.nup + (ispin == 1 ? (pm == 1 ? 1 : -1) : 0);
         const int cd = p.ndn + (ispin == 2 ? (pm == 1 ? 1 : -1) : 0);
         if (cu < 0 || cd < 0 || cu > ns || cd > ns) continue;
         const long long cnu = binom_ll(ns, cu), cnd = binom_ll(ns, cd), Dc = cnu * cnd;
         bytes += 3 * rup(Dc, kPadB) * b->es + 4 * (size_t)(ispin == 1 ? cnu : cnd) + 256;
-        bytes += state_bytes(mg, chain_hvn(b, ns, cnu, cnd), chain_axn(Dc));
+        bytes += state_bytes(mg, chain_hvn(b, ns, cnu, cnd, no), chain_axn(Dc));
     }
     return bytes + 64 * 256;
 }
 
 int validate(const edg_batch *b, const edg_problem &p)
 {
-    const int ns = nsite_of(p);
-    if (p.nbath < 1 || ns > 15) return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "nbath must be 1..14");
+    const int ns = nsite_of(p), no = norb_of(p);
+    if (p.norb < 0 || p.norb > 2) return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "norb must be 0, 1 or 2");
+    if (p.nbath < 1 || ns > 15)
+        return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "nbath must be 1..15 - norb");
     if (!p.ek || !p.vmk) return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "ek / vmk missing");
     if (p.nup < 0 || p.ndn < 0 || p.nup > ns || p.ndn > ns)
         return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "sector outside 0..nsite");
     if (p.gs_only != 0 && p.gs_only != 1) return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "gs_only must be 0 or 1");
-    if (p.maxnstep < 1 || p.maxnstep > 1000 || green_len(p) > 1000)
+    if (p.maxnstep < 1 || p.maxnstep > kMaxLen || green_len(p) > kMaxLen)
         return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "Lanczos lengths must be 1..1000");
     for (int s = 0; s < 2; ++s) {
         const int n0 = s == 0 ? p.nup : p.ndn;
         for (int d = -1; d <= 1; ++d) {
             const int ne = n0 + d;
             if (ne < 0 || ne > ns) continue;
-            if (kmax_single(p.nbath, ne) > 16)
+            if (kmax_hops(no, p.nbath, ne) > 16)
                 return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "more than 16 hops per configuration");
             const long long n = binom_ll(ns, ne);
-            const Shape sh = chain_shape(b, ns, n);
-            if ((size_t)sh.cb * n * b->es > kLdsMax)
-                return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "sector column exceeds LDS");
+            const Shape sh = chain_shape(b, ns, n, no);
+            const size_t lds = no >= 2 ? hv2_lds((int)n, sh.cb, b->es) : (size_t)sh.cb * n * b->es;
+            if (lds > kLdsMax) return fail(const_cast<edg_batch *>(b), EDG_ERR_ARG, "sector column exceeds LDS");
         }
     }
     return 0;
@@ -349,14 +391,19 @@ double *pinned_get(edg_batch *b)
     return p;
 }
 
+// U' and J' of a two-orbital problem (dmft_params.f90:90-96 defaults).
+double kan_up(const edg_problem &p) { return p.Up < 0 ? p.U - 2.0 * p.J : p.Up; }
+double kan_jp(const edg_problem &p) { return p.Jp < 0 ? p.J : p.Jp; }
+
 // Per-spin tables of one (spin, N) half; kc = hop slots per row, the same
-// for every table of the problem so that its chains share one launch class.
-int build_table(edg_batch *b, Job *J, int spin, int ne, SpinTab &t, int kc)
+// for every table of the problem so that its chains share one launch class;
+// km = the table's largest hop count.
+int build_table(edg_batch *b, Job *J, int spin, int ne, SpinTab &t, int kc, int km)
 {
-    const int ns = nsite_of(J->p), nbath = J->p.nbath;
+    const int ns = nsite_of(J->p), nbath = J->p.nbath, norb = J->norb;
     t = SpinTab{};
     t.n = (int)binom_ll(ns, ne);
-    t.K = std::max(nbath, kMinSlots);
+    t.K = std::max(norb * nbath, kMinSlots);
     const size_t n = t.n, K = t.K;
     t.pattern = (uint32_t *)J->ar.take(4 * n);
     t.hf = (HopF *)J->ar.take(K * n * sizeof(HopF));
@@ -366,18 +413,20 @@ int build_table(edg_batch *b, Job *J, int spin, int ne, SpinTab &t, int kc)
     t.energy = (double *)J->ar.take(8 * n);
     t.energyf = (float *)J->ar.take(4 * n);
     t.mask = (uint8_t *)J->ar.take(n);
-    t.kmax = kmax_single(nbath, ne);
+    if (norb >= 2) t.orb = (uint32_t *)J->ar.take(4 * (size_t)norb * norb * n);
+    t.kmax = km;
     t.KC = kc;
     t.ccol = (uint16_t *)J->ar.take(16 * n * 2);
     t.cidx = (uint8_t *)J->ar.take(16 * n);
     t.hrec = (uint32_t *)J->ar.take(16 * n * 4);
     if (J->ar.overflow) return fail(b, EDG_ERR_NOMEM, "problem arena overflow (tables)");
-    const double *em_s = J->dparams + (spin - 1);
-    const double *ek_s = J->dparams + 2 + (size_t)(spin - 1) * nbath;
-    const double *v_s = J->dparams + 2 + 2 * (size_t)nbath + (size_t)(spin - 1) * nbath;
+    const double *em_s = J->dparams + (size_t)(spin - 1) * norb;
+    const double *ek_s = J->dparams + 2 * norb + (size_t)(spin - 1) * nbath;
+    const double *v_s = J->dparams + 2 * norb + 2 * (size_t)nbath + (size_t)(spin - 1) * norb * nbath;
     t.vsrc = v_s;
     BCHK(b, hipMemsetAsync(t.maxcnt, 0, sizeof(int), b->side));
-    if (launch_spin_table(b->side, ns, 1, nbath, ne, em_s, ek_s, v_s, J->p.mu, 0.0, t))
+    const double UpmJ = norb >= 2 ? kan_up(J->p) - J->p.J : 0.0;
+    if (launch_spin_table(b->side, ns, norb, nbath, ne, em_s, ek_s, v_s, J->p.mu, UpmJ, t))
         return fail(b, EDG_ERR_HIP, "spin table launch failed");
     if (launch_compress(b->side, t)) return fail(b, EDG_ERR_HIP, "compress launch failed");
     return 0;
@@ -390,15 +439,15 @@ HvOp make_op(const edg_batch *b, const Job *J, int nu, int nd)
     const SpinTab &td = J->tab[1][nd - J->p.ndn + 1];
     op.nup = tu.n;
     op.ndn = td.n;
-    op.norb = 1;
-    op.nvup = J->p.nbath;
+    op.norb = J->norb;
+    op.nvup = J->norb * J->p.nbath;
     op.ncu = b->ncu;
     op.sched = 0;
     op.up = tu;
     op.dn = td;
     op.U = J->p.U;
-    op.Up = 0.0;
-    op.Jp = 0.0;
+    op.Up = J->norb >= 2 ? kan_up(J->p) : 0.0;
+    op.Jp = J->norb >= 2 ? kan_jp(J->p) : 0.0;
     return op;
 }
 
@@ -413,12 +462,13 @@ int init_chain(edg_batch *b, Job *J, Chain &C, const HvOp &op, int maxn, bool st
     I.stride = Dp;
     I.base = J->ar.take((size_t)I.nslot * Dp * b->es);
     I.maxn = maxn;
-    const Shape sh = chain_shape(b, nsite_of(J->p), op.nup);
+    const Shape sh = chain_shape(b, nsite_of(J->p), op.nup, J->norb);
     I.wide = sh.wide;
     I.cb = sh.cb;
-    I.hvn = chain_hvn(b, nsite_of(J->p), op.nup, op.ndn);
+    I.hvn = chain_hvn(b, nsite_of(J->p), op.nup, op.ndn, J->norb);
     I.axn = chain_axn(D);
-    if ((size_t)I.cb * op.nup * b->es > kLdsMax) return fail(b, EDG_ERR_ARG, "tile exceeds LDS");
+    const size_t lds = J->norb >= 2 ? hv2_lds(op.nup, I.cb, b->es) : (size_t)I.cb * op.nup * b->es;
+    if (lds > kLdsMax) return fail(b, EDG_ERR_ARG, "tile exceeds LDS");
     char *st = (char *)J->ar.take(state_bytes(maxn, I.hvn, I.axn));
     I.step = (int *)st;
     I.flags = (int *)(st + 8);
@@ -436,12 +486,12 @@ int init_chain(edg_batch *b, Job *J, Chain &C, const HvOp &op, int maxn, bool st
 }
 
 // flags = {0, maxn}, scale[0] = 1/|slot 0| (norm into norm_out).  Chain
-// number slot (0 = ground state, 1..4 = c+/c) owns its own pinned flag pair,
+// number slot (0 = ground state, 1..8 = c+/c) owns its own pinned flag pair,
 // so no upload waits for an earlier one.
 int start_chain(edg_batch *b, Job *J, Chain &C, double *norm_out, int slot)
 {
     BInst &I = C.bi;
-    int *h = (int *)(J->host + J->host_n - 8) + 2 * slot;
+    int *h = (int *)(J->host + J->host_n - 16) + 2 * slot;
     h[0] = 0;
     h[1] = I.maxn;
     BCHK(b, hipMemcpyAsync(I.flags, h, 2 * sizeof(int), hipMemcpyHostToDevice, b->side));
@@ -471,20 +521,28 @@ int admit(edg_batch *b, Job *J)
     J->host_n = b->pinned_n;
     BCHK(b, hipEventCreateWithFlags(&J->ev, hipEventDisableTiming));
     BCHK(b, hipEventCreateWithFlags(&J->start_ev, hipEventDisableTiming));
-    const int nbath = p.nbath, ns = nsite_of(p);
-    J->dparams = (double *)J->ar.take(8 * (2 + 4 * (size_t)nbath));
+    const int nbath = p.nbath, ns = nsite_of(p), no = J->norb;
+    const size_t npar = 2 * (size_t)no + 2 * (size_t)nbath + 2 * (size_t)no * nbath;
+    J->dparams = (double *)J->ar.take(8 * npar);
     if (J->ar.overflow) return fail(b, EDG_ERR_NOMEM, "problem arena overflow");
     double *h = J->host;
-    h[0] = p.em[0];
-    h[1] = p.em[1];
-    std::memcpy(h + 2, J->ek.data(), sizeof(double) * 2 * nbath);
-    std::memcpy(h + 2 + 2 * nbath, J->vmk.data(), sizeof(double) * 2 * nbath);
-    BCHK(b, hipMemcpyAsync(J->dparams, h, 8 * (2 + 4 * (size_t)nbath), hipMemcpyHostToDevice, b->side));
-    int kc = 8;
+    if (no >= 2) std::memcpy(h, p.em2, sizeof(double) * 4);
+    else {
+        h[0] = p.em[0];
+        h[1] = p.em[1];
+    }
+    std::memcpy(h + 2 * no, J->ek.data(), sizeof(double) * 2 * nbath);
+    std::memcpy(h + 2 * no + 2 * nbath, J->vmk.data(), sizeof(double) * 2 * no * nbath);
+    BCHK(b, hipMemcpyAsync(J->dparams, h, 8 * npar, hipMemcpyHostToDevice, b->side));
+    // exact hop counts (zero hybridizations skipped, as the tables do)
+    int km[2][3] = {}, kc = 8;
     for (int s = 0; s < 2; ++s) {
         const int n0 = s == 0 ? p.nup : p.ndn;
         for (int d = -1; d <= 1; ++d)
-            if (n0 + d >= 0 && n0 + d <= ns && kmax_single(nbath, n0 + d) > 8) kc = 16;
+            if (n0 + d >= 0 && n0 + d <= ns) {
+                km[s][d + 1] = kmax_exact(no, nbath, n0 + d, J->vmk.data() + (size_t)s * no * nbath);
+                if (km[s][d + 1] > 8) kc = 16;
+            }
     }
     for (int s = 0; s < 2; ++s) {
         const int n0 = s == 0 ? p.nup : p.ndn;
@@ -492,12 +550,12 @@ int admit(edg_batch *b, Job *J)
             const int ne = n0 + d;
             J->has_tab[s][d + 1] = ne >= 0 && ne <= ns;
             if (J->has_tab[s][d + 1])
-                if (int rc = build_table(b, J, s + 1, ne, J->tab[s][d + 1], kc)) return rc;
+                if (int rc = build_table(b, J, s + 1, ne, J->tab[s][d + 1], kc, km[s][d + 1])) return rc;
         }
     }
     J->misc = (double *)J->ar.take(64 * 8);
-    J->coef = (double *)J->ar.take(8 * (8 * (size_t)std::max(green_len(p), p.maxnstep) + 64));
-    J->gdev = (double *)J->ar.take(8 * 4 * (size_t)b->nw);
+    J->coef = (double *)J->ar.take(8 * (8 * (size_t)no * std::max(green_len(p), p.maxnstep) + 64));
+    J->gdev = (double *)J->ar.take(8 * 4 * (size_t)no * b->nw);
     const HvOp op = make_op(b, J, p.nup, p.ndn);
     const long long Dp = rup((long long)op.nup * op.ndn, kPadB);
     J->gs = J->ar.take(Dp * b->es);
@@ -581,15 +639,17 @@ int after_gs(edg_batch *b, Job *J)
     const int hrc = b->fp64 ? launch_hv<double>(b->side, op, HV_RQ, 2, true, ha)
                             : launch_hv<float>(b->side, op, HV_RQ, 2, true, ha);
     if (hrc < 0) return fail(b, EDG_ERR_HIP, "Rayleigh-quotient launch failed");
-    // c+/c start vectors (dmft_green.f90:202-257), one ring per chain
+    // c+/c start vectors of every orbital (dmft_green.f90:71-118, 202-257),
+    // one ring per chain
     const int ns_site = nsite_of(p), mg = green_len(p);
     const bool spin2 = runs_spin2(b, p);
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < kMaxChains; ++k) {
         Chain &C = J->ch[k];
-        C.ispin = k < 2 ? 1 : 2;
+        C.iorb = k / 4 + 1;
+        C.ispin = (k & 3) < 2 ? 1 : 2;
         C.pm = (k & 1) ? 2 : 1;
         C.exists = false;
-        if (p.gs_only) continue;
+        if (p.gs_only || C.iorb > J->norb) continue;
         if (C.ispin == 2 && !spin2) continue;
         const int nu = p.nup + (C.ispin == 1 ? (C.pm == 1 ? 1 : -1) : 0);
         const int nd = p.ndn + (C.ispin == 2 ? (C.pm == 1 ? 1 : -1) : 0);
@@ -600,7 +660,7 @@ int after_gs(edg_batch *b, Job *J)
         J->map[k] = (uint32_t *)J->ar.take(4 * (size_t)tout.n);
         if (J->ar.overflow) return fail(b, EDG_ERR_NOMEM, "problem arena overflow (map)");
         const int ne_in = C.ispin == 1 ? p.nup : p.ndn;
-        if (launch_cmap(b->side, tout.pattern, tout.n, ns_site, ne_in, 0, C.pm, J->map[k]))
+        if (launch_cmap(b->side, tout.pattern, tout.n, ns_site, ne_in, C.iorb - 1, C.pm, J->map[k]))
             return fail(b, EDG_ERR_HIP, "cmap launch failed");
         const int godd = C.ispin == 2 ? (p.nup & 1) : 0;
         const int arc = b->fp64 ? launch_apply_c<double>(b->side, (const double *)J->gs, (double *)C.bi.base,
@@ -619,7 +679,7 @@ int after_gs(edg_batch *b, Job *J)
 
 bool chains_running(const Job *J)
 {
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < kMaxChains; ++k)
         if (J->ch[k].exists && (J->ch[k].active || J->ch[k].pending)) return true;
     return false;
 }
@@ -628,73 +688,83 @@ bool chains_running(const Job *J)
 int after_chains(edg_batch *b, Job *J)
 {
     const edg_problem &p = J->p;
-    const int mg = green_len(p);
+    const int mg = green_len(p), no = J->norb;
     const double *h = J->host;
-    const double *rq = h;  // misc[0..7] copied first
+    const double *rq = h;  // misc[0..15] copied first
     J->e0 = rq[1] / rq[3];
-    // per chain: ap/bp (k even) or am/bm (k odd), b[0] = |v_init|
-    std::vector<double> co[2];
-    int nsv[2][2] = {{0, 0}, {0, 0}};
-    for (int s = 0; s < 2; ++s) co[s].assign(4 * (size_t)mg, 0.0);
-    for (int k = 0; k < 4; ++k) {
+    // per orbital and spin: ap/bp (pm 1) or am/bm (pm 2), b[0] = |v_init|;
+    // co[o][s] holds the (mg, 4) block of g_coeffs.dat
+    std::vector<double> co[2][2];
+    int nsv[2][2][2] = {};
+    for (int o = 0; o < no; ++o)
+        for (int s = 0; s < 2; ++s) co[o][s].assign(4 * (size_t)mg, 0.0);
+    for (int k = 0; k < 4 * no; ++k) {
         const Chain &C = J->ch[k];
         if (!C.exists) continue;
-        const double *raw = h + 16 + (size_t)k * (3 * mg + 4);
-        const int s = C.ispin - 1, q = C.pm == 1 ? 0 : 2;
-        const int ns = chain_ab(raw, mg, co[s].data() + q * mg, co[s].data() + (q + 1) * mg);
-        co[s][(q + 1) * mg] = rq[8 + k];
-        nsv[s][C.pm - 1] = ns;
+        const double *raw = h + host_chain(k, mg);
+        const int o = C.iorb - 1, s = C.ispin - 1, q = C.pm == 1 ? 0 : 2;
+        const int ns = chain_ab(raw, mg, co[o][s].data() + q * mg, co[o][s].data() + (q + 1) * mg);
+        co[o][s][(q + 1) * mg] = rq[8 + k];
+        nsv[o][s][C.pm - 1] = ns;
     }
     const bool spin2 = runs_spin2(b, p);
-    if (!spin2) {  // the reference reuses the spin-1 coefficients (:102-109)
-        co[1] = co[0];
-        nsv[1][0] = nsv[0][0];
-        nsv[1][1] = nsv[0][1];
-    }
-    J->coeffs.resize(8 * (size_t)mg);
-    std::memcpy(J->coeffs.data(), co[0].data(), sizeof(double) * 4 * mg);
-    std::memcpy(J->coeffs.data() + 4 * mg, co[1].data(), sizeof(double) * 4 * mg);
-    double *hc = J->host + 16 + 4 * (size_t)(3 * mg + 4);
+    if (!spin2)  // the reference reuses the spin-1 coefficients (:102-109)
+        for (int o = 0; o < no; ++o) {
+            co[o][1] = co[o][0];
+            nsv[o][1][0] = nsv[o][0][0];
+            nsv[o][1][1] = nsv[o][0][1];
+        }
+    J->coeffs.resize(8 * (size_t)mg * no);
+    for (int o = 0; o < no; ++o)
+        for (int s = 0; s < 2; ++s)
+            std::memcpy(J->coeffs.data() + ((size_t)o * 2 + s) * 4 * mg, co[o][s].data(), sizeof(double) * 4 * mg);
+    double *hc = J->host + host_coef(mg);
     const int nspin_run = spin2 ? 2 : 1;
-    for (int s = 0; s < nspin_run; ++s) {
-        std::memcpy(hc + (size_t)s * 4 * mg, co[s].data(), sizeof(double) * 4 * mg);
-        const double *dc = J->coef + (size_t)s * 4 * mg;
-        BCHK(b, hipMemcpyAsync((void *)dc, hc + (size_t)s * 4 * mg, sizeof(double) * 4 * mg, hipMemcpyHostToDevice,
-                               b->side));
-        if (launch_cf(b->side, b->nw, b->omega, J->e0, nsv[s][0], dc, dc + mg, nsv[s][1], dc + 2 * mg, dc + 3 * mg,
-                      J->gdev + (size_t)s * 2 * b->nw, 0))
-            return fail(b, EDG_ERR_HIP, "continued fraction launch failed");
-    }
-    double *hg = J->host + 16 + 4 * (size_t)(3 * mg + 4) + 8 * (size_t)mg;
-    BCHK(b, hipMemcpyAsync(hg, J->gdev, sizeof(double) * 2 * b->nw * nspin_run, hipMemcpyDeviceToHost, b->side));
+    for (int o = 0; o < no; ++o)
+        for (int s = 0; s < nspin_run; ++s) {
+            const size_t q = ((size_t)o * 2 + s) * 4 * mg;
+            std::memcpy(hc + q, co[o][s].data(), sizeof(double) * 4 * mg);
+            const double *dc = J->coef + q;
+            BCHK(b, hipMemcpyAsync((void *)dc, hc + q, sizeof(double) * 4 * mg, hipMemcpyHostToDevice, b->side));
+            if (launch_cf(b->side, b->nw, b->omega, J->e0, nsv[o][s][0], dc, dc + mg, nsv[o][s][1], dc + 2 * mg,
+                          dc + 3 * mg, J->gdev + ((size_t)o * 2 + s) * 2 * b->nw, 0))
+                return fail(b, EDG_ERR_HIP, "continued fraction launch failed");
+        }
+    double *hg = J->host + host_g(mg);
+    BCHK(b, hipMemcpyAsync(hg, J->gdev, sizeof(double) * 4 * b->nw * no, hipMemcpyDeviceToHost, b->side));
     BCHK(b, hipEventRecord(J->ev, b->side));
     J->st = ST_CF_READ;
     return 0;
 }
 
-// G_cl assembly (dmft_green.f90:111-118) and release.
+// G_cl(nw, norb, nspin) assembly (dmft_green.f90:111-118) and release.
 int finish(edg_batch *b, Job *J)
 {
     const edg_problem &p = J->p;
-    const int mg = green_len(p), nw = b->nw;
-    const double *hg = J->host + 16 + 4 * (size_t)(3 * mg + 4) + 8 * (size_t)mg;
+    const int mg = green_len(p), nw = b->nw, no = J->norb;
+    const double *hg = J->host + host_g(mg);
     const bool even = !(b->nspin == 2 || p.nup != p.ndn);
     const bool spin2 = runs_spin2(b, p);
-    J->G.assign(2 * (size_t)nw * b->nspin, 0.0);
-    for (int iw = 0; iw < nw; ++iw) {
-        const double r1 = hg[2 * iw], i1 = hg[2 * iw + 1];
-        const double r2 = spin2 ? hg[2 * nw + 2 * iw] : r1, i2 = spin2 ? hg[2 * nw + 2 * iw + 1] : i1;
-        if (b->nspin == 1 && even) {
-            J->G[2 * iw] = r1;
-            J->G[2 * iw + 1] = i1;
-        } else if (b->nspin == 1) {
-            J->G[2 * iw] = 0.5 * (r1 + r2);
-            J->G[2 * iw + 1] = 0.5 * (i1 + i2);
-        } else {
-            J->G[2 * iw] = r1;
-            J->G[2 * iw + 1] = i1;
-            J->G[2 * (size_t)nw + 2 * iw] = r2;
-            J->G[2 * (size_t)nw + 2 * iw + 1] = i2;
+    J->G.assign(2 * (size_t)nw * no * b->nspin, 0.0);
+    for (int o = 0; o < no; ++o) {
+        const double *g1 = hg + (size_t)o * 4 * nw, *g2 = spin2 ? g1 + 2 * nw : g1;
+        double *o1 = J->G.data() + 2 * (size_t)o * nw;
+        double *o2 = J->G.data() + 2 * ((size_t)no * nw + (size_t)o * nw);
+        for (int iw = 0; iw < nw; ++iw) {
+            const double r1 = g1[2 * iw], i1 = g1[2 * iw + 1];
+            const double r2 = g2[2 * iw], i2 = g2[2 * iw + 1];
+            if (b->nspin == 1 && even) {
+                o1[2 * iw] = r1;
+                o1[2 * iw + 1] = i1;
+            } else if (b->nspin == 1) {
+                o1[2 * iw] = 0.5 * (r1 + r2);
+                o1[2 * iw + 1] = 0.5 * (i1 + i2);
+            } else {
+                o1[2 * iw] = r1;
+                o1[2 * iw + 1] = i1;
+                o2[2 * iw] = r2;
+                o2[2 * iw + 1] = i2;
+            }
         }
     }
     // every GPU user of the region is done: the host saw J->ev after the last
@@ -724,7 +794,8 @@ int progress(edg_batch *b)
                 continue;
             }
             J->start_pending = false;
-            Chain *cs[5] = {&J->gsc, &J->ch[0], &J->ch[1], &J->ch[2], &J->ch[3]};
+            Chain *cs[kMaxChains + 1] = {&J->gsc};
+            for (int k = 0; k < kMaxChains; ++k) cs[k + 1] = &J->ch[k];
             for (Chain *C : cs)
                 if (C->pending) {
                     C->pending = false;
@@ -743,8 +814,8 @@ int progress(edg_batch *b)
             // misc (RQ sums, start norms) was written before start_ev
             BCHK(b, hipMemcpyAsync(J->host, J->misc, 16 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
             const int mg = green_len(J->p);
-            for (int k = 0; k < 4; ++k)
-                if (J->ch[k].exists) read_chain(b, J, J->ch[k], 16 + (size_t)k * (3 * mg + 4));
+            for (int k = 0; k < kMaxChains; ++k)
+                if (J->ch[k].exists) read_chain(b, J, J->ch[k], host_chain(k, mg));
             BCHK(b, hipEventRecord(J->ev, b->stream));
             J->st = ST_CH_READ;
         } else if (J->st == ST_CH_READ && hipEventQuery(J->ev) == hipSuccess) {
@@ -760,10 +831,17 @@ int progress(edg_batch *b)
     return 0;
 }
 
-// Launch class of a chain: short columns, long columns two per tile, long
-// columns one per tile (x 8 / 16 hop slots).
+// Launch class of a chain: single orbital with short columns, long columns
+// two per tile, long columns one per tile (x 8 / 16 hop slots); two orbitals
+// with 8, 12 or 16 hop slots processed (kan_slots).
+int kan_slots(const BInst &I)
+{
+    const int km = std::max(I.op.up.kmax, I.op.dn.kmax);
+    return I.op.up.KC == 8 ? 8 : (km <= 12 ? 12 : 16);
+}
 int chain_class(const BInst &I)
 {
+    if (I.op.norb >= 2) return 6 + (kan_slots(I) == 8 ? 0 : (kan_slots(I) == 12 ? 1 : 2));
     return (I.wide ? (I.cb > 1 ? 2 : 4) : 0) + (I.op.up.KC == 16 ? 1 : 0);
 }
 int class_cols(int c) { return c >= 4 ? 1 : (c >= 2 ? kBatchWideCols : 0); }
@@ -805,16 +883,17 @@ int rebuild(edg_batch *b)
     start[kClasses] = (int)order.size();
     const int n = start[kClasses];
     std::vector<int> hv[kClasses], ax(n + 1);
-    int lds[kClasses];
+    size_t lds[kClasses];  // dynamic LDS bytes of the class's launch
     ax[0] = 0;
     for (int c = 0; c < kClasses; ++c) {
-        lds[c] = 1;
+        lds[c] = 16;
         hv[c].assign(start[c + 1] - start[c] + 1, 0);
         for (int i = start[c]; i < start[c + 1]; ++i) {
             const BInst &I = order[i]->bi;
             const int tiles = (I.hvn + 7) / 8 * 8;  // every class: XCD-aligned tile offsets
             hv[c][i - start[c] + 1] = hv[c][i - start[c]] + tiles;
-            lds[c] = std::max(lds[c], I.cb * I.op.nup);
+            const size_t l = I.op.norb >= 2 ? hv2_lds(I.op.nup, I.cb, b->es) : (size_t)I.cb * I.op.nup * b->es;
+            lds[c] = std::max(lds[c], l);
         }
     }
     for (int i = 0; i < n; ++i) ax[i + 1] = ax[i] + order[i]->bi.axn;
@@ -920,9 +999,13 @@ int enqueue_chunk(edg_batch *b, int k)
     for (int sub = 0; sub < k; ++sub) {
         for (int c = 0; c < kClasses; ++c) {
             if (!b->cls_n[c]) continue;
-            if (launch_batch_hv(b->stream, b->fp64, class_cols(c), (c & 1) ? 16 : 8, b->d_inst + b->cls_start[c],
-                                b->d_hvoff[c], b->d_hvown[c], b->cls_tiles[c], b->cls_lds[c], sub))
-                return fail(b, EDG_ERR_HIP, "batched H.v launch failed");
+            const int rc = c >= 6 ? launch_batch_hv2(b->stream, b->fp64, c == 6 ? 8 : (c == 7 ? 12 : 16),
+                                                     b->d_inst + b->cls_start[c],
+                                                     b->d_hvoff[c], b->d_hvown[c], b->cls_tiles[c], b->cls_lds[c], sub)
+                                  : launch_batch_hv(b->stream, b->fp64, class_cols(c), (c & 1) ? 16 : 8,
+                                                    b->d_inst + b->cls_start[c], b->d_hvoff[c], b->d_hvown[c],
+                                                    b->cls_tiles[c], (int)(b->cls_lds[c] / b->es), sub);
+            if (rc) return fail(b, EDG_ERR_HIP, "batched H.v launch failed");
         }
         if (launch_batch_axpy(b->stream, b->fp64, b->d_inst, b->d_axoff, b->d_axown, b->nax, sub))
             return fail(b, EDG_ERR_HIP, "batched axpy launch failed");
@@ -1000,7 +1083,7 @@ int edg_batch_create(int device, int precision, int nspin, int nw, const double 
     b->es = b->fp64 ? 8 : 4;
     b->nspin = nspin;
     b->nw = nw;
-    b->pinned_n = 16 + 4 * (3 * 1000 + 4) + 8 * 1000 + 4 * (size_t)nw + 8 * 1000 + 64;
+    b->pinned_n = host_g(kMaxLen) + 8 * (size_t)nw + 16;
     int rc = 0;
     auto chk = [&](hipError_t e) {
         if (e != hipSuccess && !rc) rc = EDG_ERR_HIP;
@@ -1113,8 +1196,9 @@ int edg_batch_submit(edg_batch *b, const edg_problem *p, long long *id)
     if (int rc = validate(b, *p)) return rc;
     Job *J = new Job;
     J->p = *p;
+    J->norb = norb_of(*p);
     J->ek.assign(p->ek, p->ek + 2 * (size_t)p->nbath);
-    J->vmk.assign(p->vmk, p->vmk + 2 * (size_t)p->nbath);
+    J->vmk.assign(p->vmk, p->vmk + 2 * (size_t)J->norb * p->nbath);
     J->p.ek = nullptr;
     J->p.vmk = nullptr;
     J->bytes = problem_bytes(b, *p);

@@ -1,7 +1,9 @@
 // Batched Lanczos steps over many independent chains (uneven sector sizes)
-// for the single-orbital impurity (norb == 1).  Launches of k_bhv (chains
-// with short columns, several columns per tile) and of k_bhvw (long columns,
-// the single-chain tile) run the H.v half of step j_i for every chain i, one launch of k_baxpy the axpy
+// of single-orbital and two-orbital impurities.  Launches of k_bhv
+// (single-orbital chains with short columns, several columns per tile), of
+// k_bhvw (long columns, the single-chain tile) and of k_bhv2 (two-orbital
+// chains, the single-chain Kanamori tile) run the H.v half of step j_i for
+// every chain i, one launch of k_baxpy the axpy
 // half, one launch of k_badvance moves every chain to its next step; each
 // chain keeps its own step index, alpha/beta arrays and partial sums, so
 // chains at different steps (admitted at different times) share the
@@ -178,6 +180,40 @@ __global__ __launch_bounds__(kHvBlock) void k_bhvw(const BInst *__restrict__ ins
 #include "hv1_body.inc"
 }
 
+// H.v tile of two-orbital (Kanamori) chains: the single-problem two-orbital
+// tile (hv2_body.inc: the C columns and their orbital-move target columns
+// staged in LDS, neighbour-column loads for the down hops) over C columns
+// per 512-thread workgroup, XCD-aligned tile offsets as in k_bhvw.
+template <typename T, typename ACC, int C, int SLOTS>
+__global__ __launch_bounds__(kHvBlock) void k_bhv2(const BInst *__restrict__ inst, const int *__restrict__ hv_off,
+                                                   const int *__restrict__ own, int sub)
+{
+    const int i = own[blockIdx.x];  // chain of this workgroup
+    const BInst &I = inst[i];
+    const int t = blockIdx.x - hv_off[i];
+    if (I.flags[0] || t >= I.hvn) return;
+    const int j = uniform(*I.step) + sub;
+    HvArgs la{};
+    la.x = uniform(slot_of<T>(I, j));
+    la.y = uniform(slot_of<T>(I, j + 1));
+    la.p = j > 0 ? uniform(slot_of<T>(I, j - 1)) : nullptr;
+    la.partial = I.hvpart;
+    la.step = j;
+    la.prev_part = I.axpart;
+    la.prev_n = I.axn;
+    la.scale = I.scale;
+    la.beta = I.beta;
+    la.stop = I.flags;
+    la.nstep = I.flags + 1;
+    const HvOp op = I.op;
+    const HvArgs &a = la;
+    constexpr int MODE = HV_LANCZOS, KH = SLOTS;
+    const int idn0 = xcd_group(t, I.hvn) * C;
+    const int slot = t;
+    const bool lead = t == 0;
+#include "hv2_body.inc"
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_baxpy(const BInst *__restrict__ inst, const int *__restrict__ ax_off, const int *__restrict__ own,
                         int sub)
@@ -239,7 +275,24 @@ static int set_lds_ks(size_t max_lds)
     return 0;
 }
 
-int batch_set_lds(size_t max_lds) { return set_lds_ks<8>(max_lds) || set_lds_ks<16>(max_lds) ? -1 : 0; }
+template <int KS>
+static int set_lds_kan(size_t max_lds)
+{
+    const void *fs[2] = {(const void *)k_bhv2<float, double, kBatchKanColsF32, KS>,
+                         (const void *)k_bhv2<double, double, 1, KS>};
+    for (const void *f : fs)
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds) != hipSuccess)
+            return -1;
+    return 0;
+}
+
+int batch_set_lds(size_t max_lds)
+{
+    return set_lds_ks<8>(max_lds) || set_lds_ks<16>(max_lds) || set_lds_kan<8>(max_lds) ||
+                   set_lds_kan<12>(max_lds) || set_lds_kan<16>(max_lds)
+               ? -1
+               : 0;
+}
 
 template <int C>
 static void (*wide_kernel(bool fp64, int ks))(const BInst *, const int *, const int *, int)
@@ -262,6 +315,19 @@ int launch_batch_hv(hipStream_t s, bool fp64, int wide_cols, int ks, const BInst
         else f = fp64 ? k_bhv<double, double, 16> : k_bhv<float, float, 16>;
     }
     hipLaunchKernelGGL(f, dim3(ntiles), dim3(wide ? kHvBlock : kBlock), lds, s, inst, hv_off, own, sub);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_batch_hv2(hipStream_t s, bool fp64, int kh, const BInst *inst, const int *hv_off, const int *own,
+                     int ntiles, size_t lds_bytes, int sub)
+{
+    if (ntiles <= 0) return 0;
+    void (*f)(const BInst *, const int *, const int *, int);
+    if (kh == 8) f = fp64 ? k_bhv2<double, double, 1, 8> : k_bhv2<float, double, kBatchKanColsF32, 8>;
+    else if (kh == 12) f = fp64 ? k_bhv2<double, double, 1, 12> : k_bhv2<float, double, kBatchKanColsF32, 12>;
+    else if (kh == 16) f = fp64 ? k_bhv2<double, double, 1, 16> : k_bhv2<float, double, kBatchKanColsF32, 16>;
+    else return -1;
+    hipLaunchKernelGGL(f, dim3(ntiles), dim3(kHvBlock), lds_bytes, s, inst, hv_off, own, sub);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

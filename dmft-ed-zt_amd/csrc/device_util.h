@@ -193,7 +193,6 @@ __device__ __forceinline__ void gstore(T *p, const T &v)
     }
 }
 
-__host__ __device__ constexpr size_t align16(size_t n) { return (n + 15) & ~size_t(15); }
 
 
 template <typename ACC>

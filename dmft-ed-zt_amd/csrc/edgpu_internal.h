@@ -85,6 +85,18 @@ template <typename T>
 int launch_reorth(hipStream_t s, const ReorthArgs &a, T *r, const T *base, long long stride, long long n,
                   int ndot);
 
+__host__ __device__ constexpr size_t align16(size_t n) { return (n + 15) & ~size_t(15); }
+
+constexpr int kNorb2 = 2;
+constexpr int kPairs2 = kNorb2 * (kNorb2 - 1);
+
+// Dynamic LDS of the two-orbital H.v tile: the C columns and their C target
+// columns.
+__host__ __device__ constexpr size_t hv2_lds(int nup, int C, size_t esize)
+{
+    return 2 * align16((size_t)nup * C * esize);
+}
+
 // Everything the H.v kernel needs besides the vectors.
 struct HvOp {
     int nup, ndn, norb;
@@ -233,6 +245,13 @@ int launch_cf(hipStream_t s, int nw, const double *omega, double e0, int np, con
 // k to every chain's step after a chunk of k steps (sub = 0 .. k-1).
 int launch_batch_hv(hipStream_t s, bool fp64, int wide_cols, int ks, const BInst *inst, const int *hv_off,
                     const int *own, int ntiles, int lds_elems, int sub);
+// Two-orbital chains: kBatchKanColsF32 (FP32 vectors, FP64 accumulation) or
+// 1 (FP64) columns per 512-thread tile, lds_bytes = hv2_lds of the largest
+// nup among them; kh = hop slots processed per row (8 with 8-slot tables,
+// 12 or 16 with 16-slot tables), at least every table's largest hop count.
+int launch_batch_hv2(hipStream_t s, bool fp64, int kh, const BInst *inst, const int *hv_off, const int *own,
+                     int ntiles, size_t lds_bytes, int sub);
+constexpr int kBatchKanColsF32 = 2;
 int launch_batch_axpy(hipStream_t s, bool fp64, const BInst *inst, const int *ax_off, const int *own,
                       int nblocks, int sub);
 int launch_batch_advance(hipStream_t s, const BInst *inst, int ninst, int k);

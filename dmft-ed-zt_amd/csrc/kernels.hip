@@ -510,211 +510,14 @@ __global__ __launch_bounds__(kHvBlock) void k_hv1(HvOp op, HvArgs a)
 // coalesced loads of neighbour columns at wave-uniform offsets.  The
 // diagonal adds U n_m,up n_m,dn and U' for unlike orbitals of opposite spin
 // to the one-spin energies (which carry U' - J for like spins).
-constexpr int kNorb2 = 2;
-constexpr int kPairs2 = kNorb2 * (kNorb2 - 1);
-
-// Dynamic LDS of k_hv2: the C columns and their C target columns.
-__host__ __device__ constexpr size_t hv2_lds(int nup, int C, size_t esize)
-{
-    return 2 * align16((size_t)nup * C * esize);
-}
-
 template <typename T, typename ACC, int C, int MODE, int KH>
 __global__ __launch_bounds__(kHvBlock) void k_hv2(HvOp op, HvArgs a)
 {
-    constexpr int KS = KH <= 8 ? 8 : 16;
-    constexpr int kStage = C == 1 ? 4 : 2;  // rows per thread per staging pass
     if (a.stop && *a.stop) return;
-    using Hop = typename HopOf<ACC>::type;
-    extern __shared__ __align__(64) unsigned char smem[];
-    const int nup = op.nup, ndn = op.ndn;
-    T *xs = reinterpret_cast<T *>(smem);                                        // the columns
-    T *xt = reinterpret_cast<T *>(smem + align16((size_t)nup * C * sizeof(T)));  // their targets
-    __shared__ ACC s_v[2 * (kMaxHops + 1)];
-    __shared__ int s_kcol[C];
-    __shared__ Row<ACC, C> s_kval[kPairs2];  // +-Jp of pair p for column c, 0: no move
-    __shared__ Row<ACC, C> s_dval[KH];       // down-hop elements, slot-major
-
-    const int idn0 = xcd_group(blockIdx.x, (ndn + C - 1) / C) * C;
-    const T *x = static_cast<const T *>(a.x);
-    const unsigned char *xb = static_cast<const unsigned char *>(a.x);
-    for (int e = threadIdx.x; e <= kMaxHops; e += kHvBlock) {
-        const ACC v = e < op.nvup ? (ACC)op.up.vsrc[e] : ACC(0);
-        s_v[e] = v;
-        s_v[kMaxHops + 1 + e] = -v;
-    }
-    // down orbital moves: pair p = (i, j), up move j -> i; spin flip: down
-    // i -> j; pair hop: down j -> i (at most one is valid).  Columns past
-    // ndn (last group) repeat column ndn-1 and are not stored.
-    if (threadIdx.x < C) {
-        const int c = threadIdx.x;
-        const int idn = idn0 + c < ndn ? idn0 + c : ndn - 1;
-        int col = -1;
-#pragma unroll
-        for (int p = 0; p < kPairs2; ++p) {
-            const int i = p / (kNorb2 - 1), jj = p % (kNorb2 - 1), j = jj >= i ? jj + 1 : jj;
-            const uint32_t tsf = op.dn.orb[(long long)(j * kNorb2 + i) * ndn + idn];
-            const uint32_t tph = op.dn.orb[(long long)(i * kNorb2 + j) * ndn + idn];
-            const uint32_t t = tsf != kInvalid ? tsf : tph;
-            s_kval[p].v[c] = t != kInvalid ? ((t >> 31) ? -(ACC)op.Jp : (ACC)op.Jp) : ACC(0);
-            if (t != kInvalid) col = (int)(t & 0x7FFFFFFFu);
-        }
-        s_kcol[c] = col;
-    }
-    // down hops of the C columns (wave-uniform)
-    uint32_t coff[C][KH];
-    ACC ed[C];
-    int md[C];
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-        // wave-uniform (kept out of the divergent block above, whose equal
-        // expression the compiler would otherwise share and move to VGPRs)
-        const int idn = uniform(idn0 + c < ndn ? idn0 + c : ndn - 1);
-        const Hop *hr = HopOf<ACC>::tab(op.dn) + (long long)idn * op.dn.K;
-#pragma unroll
-        for (int k = 0; k < KH; ++k) {
-            const Hop h = gload(hr + k);
-            coff[c][k] = h.col * (uint32_t)nup * (uint32_t)sizeof(T);
-            if (threadIdx.x == 0) s_dval[k].v[c] = (ACC)h.val;
-        }
-        ed[c] = energy_of<ACC>(op.dn, idn);
-        md[c] = gload(op.dn.mask + idn);
-    }
-    __syncthreads();  // s_kcol
-    // stage the columns and their target columns, row-interleaved
-    for (int i0 = threadIdx.x; i0 < nup; i0 += kStage * kHvBlock) {
-        T v[C][kStage], w[C][kStage];
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-            const int idn = idn0 + c < ndn ? idn0 + c : ndn - 1;
-            const T *xc = x + (long long)idn * nup;
-            const int tc = s_kcol[c];
-#pragma unroll
-            for (int r = 0; r < kStage; ++r) {
-                const int i = i0 + r * kHvBlock;
-                const int ic = i < nup ? i : nup - 1;
-                v[c][r] = xc[ic];
-                w[c][r] = tc >= 0 ? x[(long long)tc * nup + ic] : T(0);
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < kStage; ++r) {
-            const int i = i0 + r * kHvBlock;
-            if (i < nup) {
-                Row<T, C> rv, rw;
-#pragma unroll
-                for (int c = 0; c < C; ++c) {
-                    rv.v[c] = v[c][r];
-                    rw.v[c] = w[c][r];
-                }
-                *reinterpret_cast<Row<T, C> *>(xs + i * C) = rv;
-                *reinterpret_cast<Row<T, C> *>(xt + i * C) = rw;
-            }
-        }
-    }
-    __syncthreads();
-
-    double sums[3] = {0.0, 0.0, 0.0};
-    double sx = 1.0, bsp = 0.0;
-    if constexpr (MODE == HV_LANCZOS) {
-        if (!lanczos_scales<kHvBlock>(a, sx, bsp, blockIdx.x == 0)) return;
-    }
-    T *y = static_cast<T *>(a.y);
-    const T *pv = static_cast<const T *>(a.p);
-    const bool has_p = MODE == HV_LANCZOS && pv != nullptr;
-    const ACC U = (ACC)op.U, Up = (ACC)op.Up;
-    for (int iup = threadIdx.x; iup < nup; iup += kHvBlock) {
-        // the LDS tables are re-read each row: hoisted out of the loop, the
-        // C * KH down-hop elements alone would hold 4 * C * KH VGPRs (FP64)
-        __asm__ volatile("" ::: "memory");
-        const uint32_t ib = (uint32_t)iup * (uint32_t)sizeof(T);
-        T g[C][KH];
-#pragma unroll
-        for (int c = 0; c < C; ++c)
-#pragma unroll
-            for (int k = 0; k < KH; ++k) g[c][k] = *reinterpret_cast<const T *>(xb + (coff[c][k] + ib));
-        const Row<uint32_t, KS> hrow = *reinterpret_cast<const Row<uint32_t, KS> *>(op.up.hrec + (long long)iup * KS);
-        uint32_t tu[kPairs2];
-#pragma unroll
-        for (int p = 0; p < kPairs2; ++p) {
-            const int i = p / (kNorb2 - 1), jj = p % (kNorb2 - 1), j = jj >= i ? jj + 1 : jj;
-            tu[p] = op.up.orb[(long long)(i * kNorb2 + j) * nup + iup];  // up move j -> i
-        }
-        const int mu = gload(op.up.mask + iup);
-        const ACC eu = energy_of<ACC>(op.up, iup);
-        T old[C];
-        if constexpr (MODE == HV_ACCUM || MODE == HV_LANCZOS) {
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-                const int idn = idn0 + c < ndn ? idn0 + c : ndn - 1;
-                const long long e = (long long)idn * nup + iup;
-                if constexpr (MODE == HV_ACCUM) old[c] = gload(y + e);
-                else old[c] = has_p ? gload(pv + e) : T(0);
-            }
-        }
-        const Row<T, C> xr = *reinterpret_cast<const Row<T, C> *>(xs + iup * C);
-        ACC acc[C];
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-            const int both = __popc(mu & md[c]);
-            const ACC diag = eu + ed[c] + U * (ACC)both + Up * (ACC)(__popc(mu) * __popc(md[c]) - both);
-            acc[c] = diag * (ACC)xr.v[c];
-        }
-#pragma unroll
-        for (int k = 0; k < KH; ++k) {
-            const uint32_t w = hrow.v[k];
-            const ACC val = s_v[w >> 16];
-            const Row<T, C> src = *reinterpret_cast<const Row<T, C> *>(xs + (w & 0xFFFFu) * C);
-#pragma unroll
-            for (int c = 0; c < C; ++c) acc[c] += val * (ACC)src.v[c];
-        }
-#pragma unroll
-        for (int k = 0; k < KH; ++k) {
-            const Row<ACC, C> dv = s_dval[k];
-#pragma unroll
-            for (int c = 0; c < C; ++c) acc[c] += dv.v[c] * (ACC)g[c][k];
-        }
-#pragma unroll
-        for (int p = 0; p < kPairs2; ++p) {
-            const bool ok = tu[p] != kInvalid;
-            const uint32_t ru = ok ? (tu[p] & 0x7FFFFFFFu) : 0u;
-            const bool neg = (tu[p] >> 31) != 0;
-            const Row<ACC, C> kv = s_kval[p];
-            const Row<T, C> src = *reinterpret_cast<const Row<T, C> *>(xt + ru * C);
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-                const ACC su = ok ? (neg ? -kv.v[c] : kv.v[c]) : ACC(0);
-                acc[c] += su * (ACC)src.v[c];
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-            if (idn0 + c >= ndn) break;
-            const long long e = (long long)(idn0 + c) * nup + iup;
-            if constexpr (MODE == HV_PLAIN) {
-                y[e] = (T)acc[c];
-            } else if constexpr (MODE == HV_ACCUM) {
-                y[e] = (T)((ACC)old[c] + acc[c]);
-            } else if constexpr (MODE == HV_LANCZOS) {
-                double r = sx * (double)acc[c];
-                if (has_p) r -= bsp * (double)old[c];
-                const T rt = (T)r;
-                y[e] = rt;
-                sums[0] += (double)xr.v[c] * (double)rt;
-            } else {
-                const double xv = (double)xr.v[c], hv = (double)acc[c];
-                sums[0] += xv * hv;
-                sums[1] += hv * hv;
-                sums[2] += xv * xv;
-            }
-        }
-    }
-    if constexpr (MODE == HV_LANCZOS) {
-        double mine[1] = {sums[0]};
-        block_partial<1, kHvBlock>(mine, a.partial);
-    } else if constexpr (MODE == HV_RQ) {
-        block_partial<3, kHvBlock>(sums, a.partial);
-    }
+    const int idn0 = xcd_group(blockIdx.x, (op.ndn + C - 1) / C) * C;
+    const int slot = blockIdx.x;
+    const bool lead = blockIdx.x == 0;
+#include "hv2_body.inc"
 }
 
 template <typename T, typename ACC, int C, int KH>

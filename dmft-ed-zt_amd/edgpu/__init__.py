@@ -44,7 +44,8 @@ class _Problem(C.Structure):
                 ("em", C.c_double * 2), ("ek", C.POINTER(C.c_double)),
                 ("vmk", C.POINTER(C.c_double)), ("nup", C.c_int), ("ndn", C.c_int),
                 ("maxnstep", C.c_int), ("maxnstep_green", C.c_int), ("both_spins", C.c_int),
-                ("seed", C.c_ulonglong), ("gs_only", C.c_int)]
+                ("seed", C.c_ulonglong), ("gs_only", C.c_int), ("norb", C.c_int),
+                ("J", C.c_double), ("Up", C.c_double), ("Jp", C.c_double), ("em2", C.c_double * 4)]
 
 
 EXPORTS = ["edg_default_params", "edg_create", "edg_destroy", "edg_last_error",
@@ -368,15 +369,20 @@ class Solver:
 
 
 class Problem:
-    """One single-orbital impurity problem for Batch (edg_problem): bath of
-    nbath sites with ek(nbath,2), vmk(1,nbath,2), em(1,2), U, mu, sector."""
+    """One impurity problem for Batch (edg_problem): norb = 1 or 2 (Kanamori,
+    with J, Up, Jp; Up < 0 gives U - 2J, Jp < 0 gives J), bath of nbath sites
+    with ek(nbath,2), vmk(norb,nbath,2), em(norb,2), U, mu, sector."""
 
     def __init__(self, nbath, U, mu, em, ek, vmk, nup, ndn, maxnstep=200, maxnstep_green=0,
-                 both_spins=False, seed=1, gs_only=False):
+                 both_spins=False, seed=1, gs_only=False, norb=1, J=0.0, Up=-1.0, Jp=-1.0):
         self.nbath, self.U, self.mu = int(nbath), float(U), float(mu)
-        self.em = _fortran(em, (1, 2))
+        self.norb = int(norb)
+        if self.norb not in (1, 2):
+            raise ValueError("norb must be 1 or 2")
+        self.J, self.Up, self.Jp = float(J), float(Up), float(Jp)
+        self.em = _fortran(em, (self.norb, 2))
         self.ek = _fortran(ek, (self.nbath, 2))
-        self.vmk = _fortran(vmk, (1, self.nbath, 2))
+        self.vmk = _fortran(vmk, (self.norb, self.nbath, 2))
         self.nup, self.ndn = int(nup), int(ndn)
         self.maxnstep, self.maxnstep_green = int(maxnstep), int(maxnstep_green)
         self.both_spins, self.seed = bool(both_spins), int(seed)
@@ -385,7 +391,12 @@ class Problem:
     def _struct(self):
         p = _Problem()
         p.nbath, p.U, p.mu = self.nbath, self.U, self.mu
-        p.em[0], p.em[1] = self.em[0], self.em[1]
+        if self.norb == 1:
+            p.em[0], p.em[1] = self.em[0], self.em[1]
+        else:
+            for k in range(4):
+                p.em2[k] = self.em[k]
+        p.norb, p.J, p.Up, p.Jp = self.norb, self.J, self.Up, self.Jp
         p.ek, p.vmk = _d(self.ek), _d(self.vmk)
         p.nup, p.ndn = self.nup, self.ndn
         p.maxnstep, p.maxnstep_green = self.maxnstep, self.maxnstep_green
@@ -403,11 +414,11 @@ def release_batch_memory(device=0):
 
 
 class Batch:
-    """Batched solver / request loop over many single-orbital problems
+    """Batched solver / request loop over many one- and two-orbital problems
     (edg_batch_*).  submit() queues a Problem and returns its id; step()
     admits queued problems as device memory allows and advances every
     Lanczos chain in flight by batched launches; poll() returns finished
-    results as dicts (id, e0, G (nw, nspin) complex, nstep)."""
+    results as dicts (id, e0, G, nstep, coeffs)."""
 
     def __init__(self, omega, precision="fp32", nspin=1, budget_bytes=0, device=0, graphs=True):
         L = load_library()
@@ -461,15 +472,16 @@ class Batch:
         i = C.c_longlong(0)
         p = prob._struct()
         self._chk(self._L.edg_batch_submit(self._h, C.byref(p), C.byref(i)))
-        self._ng[i.value] = prob.maxnstep_green or prob.maxnstep
+        self._ng[i.value] = (prob.maxnstep_green or prob.maxnstep, prob.norb)
         return i.value
 
     def poll(self):
-        """Finished problems: id, e0, G (nw, nspin) complex, nstep and
-        coeffs[ispin] = (ap, bp, am, bm) (g_coeffs.dat content)."""
+        """Finished problems: id, e0, G, nstep and the g_coeffs.dat content.
+        Single orbital: G (nw, nspin) complex and coeffs[ispin] = (ap, bp, am,
+        bm); two orbitals: G (nw, 2, nspin) (G_cl) and coeffs[(iorb, ispin)]."""
         out = []
-        G = np.zeros(2 * self.nw * self.nspin)
-        co = np.zeros(8 * 1000)
+        G = np.zeros(2 * self.nw * 2 * self.nspin)
+        co = np.zeros(8 * 1000 * 2)
         while True:
             i, e0, ns = C.c_longlong(0), C.c_double(0), C.c_int(0)
             r = self._L.edg_batch_poll(self._h, C.byref(i), C.byref(e0), _d(G), C.byref(ns), _d(co))
@@ -477,10 +489,16 @@ class Batch:
                 self._chk(r)
             if r == 0:
                 return out
-            ng = self._ng.pop(i.value)
-            Gc = (G[0::2] + 1j * G[1::2]).reshape(self.nw, self.nspin, order="F").copy()
-            c = co[:8 * ng].reshape(ng, 4, 2, order="F")
-            coeffs = {s + 1: tuple(c[:, q, s].copy() for q in range(4)) for s in range(2)}
+            ng, no = self._ng.pop(i.value)
+            n = self.nw * no * self.nspin
+            Gc = (G[0:2 * n:2] + 1j * G[1:2 * n:2]).reshape(self.nw, no, self.nspin, order="F").copy()
+            c = co[:8 * ng * no].reshape(ng, 4, 2, no, order="F")
+            if no == 1:
+                Gc = Gc[:, 0, :]
+                coeffs = {s + 1: tuple(c[:, q, s, 0].copy() for q in range(4)) for s in range(2)}
+            else:
+                coeffs = {(o + 1, s + 1): tuple(c[:, q, s, o].copy() for q in range(4))
+                          for o in range(no) for s in range(2)}
             out.append(dict(id=i.value, e0=e0.value, G=Gc, nstep=ns.value, coeffs=coeffs))
 
     def stats(self):

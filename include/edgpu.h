@@ -235,8 +235,9 @@ int edg_last_solve_stats(edg_ctx *ctx, double *phase_ms, long long *n_hv, long l
 /* ---------------------------------------------------------------------
  * Batched solver and request loop.  No counterpart in the reference (it
  * solves one impurity problem per call); for callers with many independent
- * single-orbital problems: the candidate sectors and parameter points of a
- * DMFT step or scan, or a stream of requests.  Every problem runs the same
+ * one- or two-orbital problems: the candidate sectors and parameter points
+ * of a DMFT step or scan, the c+/c chains of every orbital and spin, or a
+ * stream of requests.  Every problem runs the same
  * algorithm as edg_ground_state + edg_cluster_green (stored-basis Lanczos
  * ground state, FP64 Rayleigh-quotient E0, c+/c chains, continued
  * fraction); the Lanczos steps of all problems in flight share batched
@@ -246,9 +247,12 @@ int edg_last_solve_stats(edg_ctx *ctx, double *phase_ms, long long *n_hv, long l
  * --------------------------------------------------------------------- */
 typedef struct edg_batch edg_batch;
 
-/* One single-orbital impurity problem (norb = 1): bath of nbath sites with
- * ek(nbath,2) and vmk(1,nbath,2) column-major as for edg_ground_state,
- * em(1,2), U and mu, in the sector (nup, ndn). */
+/* One impurity problem: norb = 1 (or 0) a single orbital with em(1,2) in
+ * em[]; norb = 2 the two-orbital Kanamori impurity with em(2,2) in em2[]
+ * and the couplings J, Up, Jp (Up < 0: U - 2J, Jp < 0: J, as
+ * dmft_params.f90:90-96).  Bath of nbath sites with ek(nbath,2) and
+ * vmk(norb,nbath,2) column-major as for edg_ground_state, U and mu, in the
+ * sector (nup, ndn). */
 typedef struct {
     int nbath;
     double U;
@@ -263,6 +267,9 @@ typedef struct {
     int both_spins;      /* also run the spin-down chains when spin symmetry would skip them */
     unsigned long long seed;
     int gs_only;         /* 1: ground state only (E0, no chains, G = 0), e.g. for a sector sweep */
+    int norb;            /* impurity orbitals: 0 or 1, or 2 */
+    double J, Up, Jp;    /* norb = 2: Hund coupling, U' and pair hopping */
+    double em2[4];       /* norb = 2: em(2,2) column-major */
 } edg_problem;
 
 /* precision: EDG_FP64 / EDG_FP32 vector storage; nspin 1 or 2 (G_cl spins);
@@ -307,10 +314,12 @@ int edg_batch_step(edg_batch *b, int max_steps, int *pending);
 /* Run until every submitted problem is finished. */
 int edg_batch_run(edg_batch *b);
 /* Pop one finished problem: returns 1 and fills id, e0 (FP64 Rayleigh
- * quotient), G (G_cl(nw,1,nspin) as interleaved re/im), nstep (ground-state
- * Lanczos steps) and coeffs (ng x 4 x 2, ng = maxnstep_green or maxnstep:
- * (ap, bp, am, bm) per spin as written to g_coeffs.dat, dmft_green.f90:89-109),
- * or returns 0 when none is ready.  Pointers may be NULL. */
+ * quotient), G (G_cl(nw,norb,nspin) as interleaved re/im), nstep
+ * (ground-state Lanczos steps) and coeffs (ng x 4 x 2 x norb, ng =
+ * maxnstep_green or maxnstep: (ap, bp, am, bm) per spin and orbital as
+ * written to g_coeffs.dat, dmft_green.f90:89-109; the layout of
+ * edg_cluster_green), or returns 0 when none is ready.  Pointers may be
+ * NULL. */
 int edg_batch_poll(edg_batch *b, long long *id, double *e0, double *G, int *nstep, double *coeffs);
 /* stats[0] batched steps, [1] chain-steps, [2] problems finished,
  * [3] device bytes in use, [4] budget, [5] problems in flight, [6] graph
