@@ -57,6 +57,12 @@ constexpr int kChunkEvents = 4;        // > kAhead
 // problem admitted one at a time against 44.8 all at once)
 constexpr size_t kLargeVec = size_t(32) << 20;
 constexpr int kMaxLarge = 1;
+// Chains of kDirectMin elements or more run their steps as launches of the
+// single-problem kernels (edg_cluster_green's, arguments from the host)
+// instead of the batched ones: one chain's H.v then fills the GPU, and the
+// batched tile's descriptor reads (chain index, step and stop flag before
+// any load of the vectors) cost 25-30% per step at Ns=14.
+constexpr long long kDirectMin = 4ll << 20;
 
 long long rup(long long n, long long m) { return (n + m - 1) / m * m; }
 
@@ -105,7 +111,9 @@ enum Stage { ST_QUEUED, ST_GS, ST_GS_READ, ST_CHAINS, ST_CH_READ, ST_CF_READ, ST
 
 struct Chain {
     BInst bi{};
-    int launched = 0;  // steps launched
+    bool direct = false;  // steps as single-problem launches (D >= kDirectMin)
+    int ax_n = 0;         // partial sums of the last axpy of a direct chain
+    int launched = 0;     // steps launched
     bool exists = false;
     bool pending = false;  // started on the side stream, not yet in the batched steps
     bool active = false;
@@ -194,9 +202,10 @@ struct edg_batch {
     std::deque<Job *> finished;
     long long next_id = 1;
     double *partial = nullptr;
-    // chains in the batched launches
-    std::vector<Chain *> active;
+    // chains in the batched launches, and the direct ones
+    std::vector<Chain *> active, direct;
     bool dirty = true;
+    bool moved = false;  // a direct chain joined or left
     char *ddesc = nullptr;
     size_t ddesc_cap = 0;
     char *hdesc[2] = {nullptr, nullptr};
@@ -320,6 +329,15 @@ int chain_axn(long long D)
     return (int)std::min<long long>(std::max<long long>(g, 1), 1024);
 }
 
+// Partial-sum slots of a chain: the batched tiles and axpy workgroups, or
+// for a direct chain bounds of the single-problem kernels' grids (one
+// workgroup per sector column at most, kAxpyMaxGrid axpy workgroups).
+int chain_hv_parts(const edg_batch *b, int nsite, long long nup, long long ndn, int norb)
+{
+    return nup * ndn >= kDirectMin ? (int)ndn : chain_hvn(b, nsite, nup, ndn, norb);
+}
+int chain_ax_parts(long long D) { return D >= kDirectMin ? std::max(chain_axn(D), kAxpyMaxGrid) : chain_axn(D); }
+
 // Device bytes of a problem in flight.
 size_t problem_bytes(const edg_batch *b, const edg_problem &p)
 {
@@ -333,7 +351,7 @@ size_t problem_bytes(const edg_batch *b, const edg_problem &p)
     const long long nu = binom_ll(ns, p.nup), nd = binom_ll(ns, p.ndn);
     const long long D = nu * nd, Dp = rup(D, kPadB);
     bytes += ((size_t)p.maxnstep + 1) * Dp * b->es + Dp * b->es;
-    bytes += state_bytes(p.maxnstep, chain_hvn(b, ns, nu, nd, no), chain_axn(D));
+    bytes += state_bytes(p.maxnstep, chain_hv_parts(b, ns, nu, nd, no), chain_ax_parts(D));
     const int mg = green_len(p);
     bytes += 64 * 8 + 8 * (8 * (size_t)no * std::max(mg, p.maxnstep) + 64) + 8 * 4 * (size_t)no * b->nw;
     for (int k = 0; k < 4 * no && !p.gs_only; ++k) {
@@ -343,7 +361,7 @@ size_t problem_bytes(const edg_batch *b, const edg_problem &p)
         if (cu < 0 || cd < 0 || cu > ns || cd > ns) continue;
         const long long cnu = binom_ll(ns, cu), cnd = binom_ll(ns, cd), Dc = cnu * cnd;
         bytes += 3 * rup(Dc, kPadB) * b->es + 4 * (size_t)(ispin == 1 ? cnu : cnd) + 256;
-        bytes += state_bytes(mg, chain_hvn(b, ns, cnu, cnd, no), chain_axn(Dc));
+        bytes += state_bytes(mg, chain_hv_parts(b, ns, cnu, cnd, no), chain_ax_parts(Dc));
     }
     return bytes + 64 * 256;
 }
@@ -465,8 +483,10 @@ int init_chain(edg_batch *b, Job *J, Chain &C, const HvOp &op, int maxn, bool st
     const Shape sh = chain_shape(b, nsite_of(J->p), op.nup, J->norb);
     I.wide = sh.wide;
     I.cb = sh.cb;
-    I.hvn = chain_hvn(b, nsite_of(J->p), op.nup, op.ndn, J->norb);
-    I.axn = chain_axn(D);
+    C.direct = D >= kDirectMin;
+    I.hvn = chain_hv_parts(b, nsite_of(J->p), op.nup, op.ndn, J->norb);
+    I.axn = chain_ax_parts(D);
+    C.ax_n = 0;
     const size_t lds = J->norb >= 2 ? hv2_lds(op.nup, I.cb, b->es) : (size_t)I.cb * op.nup * b->es;
     if (lds > kLdsMax) return fail(b, EDG_ERR_ARG, "tile exceeds LDS");
     char *st = (char *)J->ar.take(state_bytes(maxn, I.hvn, I.axn));
@@ -800,8 +820,13 @@ int progress(edg_batch *b)
                 if (C->pending) {
                     C->pending = false;
                     C->active = true;
-                    b->active.push_back(C);
-                    b->dirty = true;
+                    if (C->direct) {
+                        b->direct.push_back(C);
+                        b->moved = true;
+                    } else {
+                        b->active.push_back(C);
+                        b->dirty = true;
+                    }
                 }
         }
         if (J->st == ST_GS && !J->gsc.active) {
@@ -1014,12 +1039,77 @@ int enqueue_chunk(edg_batch *b, int k)
     return 0;
 }
 
-// Run k batched steps of the active chains (1 <= k <= kChunk, k <= every
+// Step j of a direct chain: the single-problem H.v and axpy launches of
+// edg_cluster_green (lanczos_enqueue in capi.cpp), with that path's default
+// schedule.
+template <typename T>
+int direct_step(edg_batch *b, Chain &C, int j)
+{
+    BInst &I = C.bi;
+    T *base = static_cast<T *>(I.base);
+    auto slot = [&](int k) { return base + (long long)(k % I.nslot) * I.stride; };
+    HvArgs ha{};
+    ha.x = slot(j);
+    ha.y = slot(j + 1);
+    ha.p = j > 0 ? slot(j - 1) : nullptr;
+    ha.partial = I.hvpart;
+    ha.step = j;
+    ha.prev_part = I.axpart;
+    ha.prev_n = C.ax_n;
+    ha.scale = I.scale;
+    ha.beta = I.beta;
+    ha.stop = I.flags;
+    ha.nstep = I.flags + 1;
+    const int cols = (I.op.norb == 1 || !b->fp64) ? 2 : 1;
+    const bool acc64 = !b->fp64 && I.op.norb >= 2;
+    const int hv_n = launch_hv<T>(b->stream, I.op, HV_LANCZOS, cols, acc64, ha);
+    if (hv_n < 0 || hv_n > I.hvn) return fail(b, EDG_ERR_HIP, "direct H.v launch failed");
+    if (j == I.maxn - 1) {
+        if (launch_alpha(b->stream, I.hvpart, hv_n, &I.scale[j], &I.alpha[j], I.flags))
+            return fail(b, EDG_ERR_HIP, "direct alpha launch failed");
+        return 0;
+    }
+    const int ax = launch_axpy_norm<T>(b->stream, slot(j + 1), slot(j), C.D, I.hvpart, hv_n, I.scale, I.alpha, j,
+                                       I.axpart, I.flags);
+    if (ax < 0 || ax > I.axn) return fail(b, EDG_ERR_HIP, "direct axpy launch failed");
+    C.ax_n = ax;
+    return 0;
+}
+
+// k steps of every direct chain, after the batched launches of the chunk
+// (outside any graph: their arguments change from chunk to chunk).
+int run_direct(edg_batch *b, int k)
+{
+    for (Chain *C : b->direct)
+        for (int sub = 0; sub < k; ++sub) {
+            const int rc = b->fp64 ? direct_step<double>(b, *C, C->launched + sub)
+                                   : direct_step<float>(b, *C, C->launched + sub);
+            if (rc) return rc;
+        }
+    b->st_steps += k;
+    b->st_chain_steps += (double)k * b->direct.size();
+    for (size_t i = 0; i < b->direct.size();) {
+        Chain *C = b->direct[i];
+        C->launched += k;
+        if (C->launched >= C->bi.maxn) {
+            C->active = false;
+            b->direct.erase(b->direct.begin() + i);
+            b->moved = true;
+            continue;
+        }
+        ++i;
+    }
+    return 0;
+}
+
+// Run k steps of the active chains (1 <= k <= kChunk, k <= every
 // chain's remainder) as one chunk: a captured graph of k steps, captured on
 // the first chunk of that length after the set of chains last changed.
+// The direct chains follow the chunk with their own launches.
 int run_steps(edg_batch *b, int k)
 {
     if (k < 1 || k > kChunk) return fail(b, EDG_ERR_ARG, "chunk length out of range");
+    if (b->active.empty()) return run_direct(b, k);
     if (b->dirty)
         if (int rc = rebuild(b)) return rc;
     // A graph pays only when it is replayed: chunks right after a change of
@@ -1049,7 +1139,7 @@ int run_steps(edg_batch *b, int k)
     } else if (int rc = enqueue_chunk(b, k)) {
         return rc;
     }
-    b->st_steps += k;
+    if (int rc = run_direct(b, k)) return rc;
     b->st_chain_steps += (double)k * b->active.size();
     for (size_t i = 0; i < b->active.size();) {
         Chain *C = b->active[i];
@@ -1244,11 +1334,13 @@ int edg_batch_step(edg_batch *b, int max_steps, int *pending)
         large_in_flight += J->large ? 1 : 0;
         it = b->queue.erase(it);
     }
-    if (!b->active.empty() && max_steps > 0) {
+    if ((!b->active.empty() || !b->direct.empty()) && max_steps > 0) {
         int left = max_steps;
-        while (left > 0 && !b->active.empty()) {
+        while (left > 0 && (!b->active.empty() || !b->direct.empty())) {
             int k = std::min(left, kChunk);
             for (const Chain *C : b->active) k = std::min(k, C->bi.maxn - C->launched);
+            for (const Chain *C : b->direct) k = std::min(k, C->bi.maxn - C->launched);
+            b->moved = false;
             if (int rc = run_steps(b, k)) return rc;
             left -= k;
             // bounded run-ahead: wait for the chunk kAhead chunks back
@@ -1260,7 +1352,7 @@ int edg_batch_step(edg_batch *b, int max_steps, int *pending)
                 --b->chunk_n;
             }
             if (int rc = progress(b)) return rc;
-            if (b->dirty) break;  // chains joined or left: let the caller see progress
+            if (b->dirty || b->moved) break;  // chains joined or left: let the caller see progress
         }
     } else if (!b->running.empty()) {
         // nothing to step: wait for the oldest outstanding event, then move on

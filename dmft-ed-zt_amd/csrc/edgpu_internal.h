@@ -177,6 +177,9 @@ struct BInst {
     double *axpart;    // [axn]
 };
 
+// Largest workgroup count of launch_axpy_norm (its partial sums).
+constexpr int kAxpyMaxGrid = 256;
+
 // ---- launchers (kernels.hip) ----
 int launch_spin_table(hipStream_t s, int nsite, int norb, int nbath, int ne, const double *em_s,
                       const double *ek_s, const double *v_s, double mu, double UpmJ,

@@ -970,7 +970,7 @@ template int launch_norm<double>(hipStream_t, const double *, long long, double 
 // the partial counts are kept small (1024 x 256 threads as many as 4096 x
 // 64 would be, a quarter of the partials of 256-thread workgroups).
 constexpr int kAxpyBlock = 1024;
-constexpr int kAxpyGrid = 256;
+constexpr int kAxpyGrid = kAxpyMaxGrid;
 
 template <typename T, int BS>
 __global__ __launch_bounds__(BS) void k_axpy_norm(T *r, const T *u, long long n,
