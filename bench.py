@@ -243,11 +243,29 @@ def bench_dmft_ns6(edgpu, W, cpu, threads):
 
 
 def profile_hv(edgpu, em, ek, vmk, U, mu):
-    """H.v launches plus one plain device copy of 2*4*D bytes (read 4D, write
-    4D) whose counter readings calibrate FETCH_SIZE / WRITE_SIZE."""
+    """H.v launches of the hot kernels plus one plain device copy of 2*4*D
+    bytes (read 4D, write 4D) whose counter readings calibrate FETCH_SIZE /
+    WRITE_SIZE: the single-orbital k_hv1 (FP32, Ns=14, sector (7,7)), the
+    two-orbital k_hv2 (FP32 vectors, Ns=14 Kanamori, sector (7,7)), and one
+    batched run of 64 single-orbital problems (Ns in {8,10,12}; k_bhv and
+    k_baxpy)."""
     import ctypes as C
+    from edgpu import workloads as W
     s32 = make_solver(edgpu, "fp32", U, mu)
     ms, by = s32.time_hv(em, ek, vmk, 7, 7, warmup=2, reps=10)
+    s32.close()
+    emk, ekk, vmkk, muk, J, Up = W.two_orbital_kanamori(6, 3.0, seed=1)
+    sk = edgpu.Solver(norb=2, nbath=12, U=3.0, J=J, Up=Up, Jp=J, mu=muk, maxnstep=200,
+                      maxnstep_green=150, precision="fp32", both_spins=True, seed=1)
+    ms_k, by_k = sk.time_hv(emk, ekk, vmkk, 7, 7, warmup=2, reps=10)
+    sk.close()
+    om = W.matsubara(2048, 100.0)
+    b = edgpu.Batch(om, precision="fp32", graphs=False)
+    for p in W.batched_problems(edgpu, n=64):
+        b.submit(p)
+    b.run()
+    st = b.stats()
+    b.close()
     nbytes = 4 * 3432 * 3432
     hip = C.CDLL("libamdhip64.so")
     a, b = C.c_void_p(), C.c_void_p()
@@ -261,8 +279,9 @@ def profile_hv(edgpu, em, ek, vmk, U, mu):
     hip.hipDeviceSynchronize()
     hip.hipFree(a)
     hip.hipFree(b)
-    print(json.dumps({"hv_us": ms * 1e3, "hv_bytes": by, "copy_bytes_read": nbytes,
-                      "copy_bytes_written": nbytes}))
+    print(json.dumps({"hv_us": ms * 1e3, "hv_bytes": by, "kanamori_hv_us": ms_k * 1e3,
+                      "kanamori_hv_bytes": by_k, "batched_chain_steps": st["chain_steps"],
+                      "copy_bytes_read": nbytes, "copy_bytes_written": nbytes}))
 
 
 def bench_batched(edgpu, W, S, graphs=True):
@@ -430,8 +449,9 @@ def main():
                     help="batched and served sections launch each step's kernels instead of "
                          "replaying a captured HIP graph (for kernel-trace profiles)")
     ap.add_argument("--profile-hv", action="store_true",
-                    help="only run H.v launches (FP32, Ns=14) and one device copy of the same "
-                         "byte count, for hardware-counter profiles")
+                    help="only run the hot H.v kernels (FP32 k_hv1 and k_hv2 at Ns=14, one "
+                         "batched run of 64 problems) and one device copy, for hardware-counter "
+                         "profiles")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} "
