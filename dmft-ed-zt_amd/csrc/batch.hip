@@ -120,13 +120,13 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, 
                 const uint32_t *dr = s_drec + c * KS;
                 T gn[KS];
 #pragma unroll
-                for (int k = 0; k < KS; ++k) gn[k] = gload(x + (long long)(dr[k] & 0xFFFFu) * nup + iup);
+                for (int k = 0; k < KS; ++k) gn[k] = gload(x + (long long)rec_rank(dr[k]) * nup + iup);
                 const ACC xv = (ACC)xc[iup];
                 ACC acc = (eu + s_ed[c] + (mu && s_md[c] ? U : ACC(0))) * xv;
 #pragma unroll
-                for (int k = 0; k < KS; ++k) acc += s_vu[ur.v[k] >> 16] * (ACC)xc[ur.v[k] & 0xFFFFu];
+                for (int k = 0; k < KS; ++k) acc += s_vu[rec_vidx(ur.v[k])] * (ACC)xc[rec_rank(ur.v[k])];
 #pragma unroll
-                for (int k = 0; k < KS; ++k) acc += s_vd[dr[k] >> 16] * (ACC)gn[k];
+                for (int k = 0; k < KS; ++k) acc += s_vd[rec_vidx(dr[k])] * (ACC)gn[k];
                 const long long gi = (long long)(c0 + c) * nup + iup;
                 double r = sx * (double)acc;
                 if (pv) r -= bsp * (double)gload(pv + gi);

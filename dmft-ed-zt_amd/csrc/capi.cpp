@@ -881,14 +881,7 @@ int edg_create(const edg_params *p, int device, edg_ctx **out)
     if (c->P.Jp < 0) c->P.Jp = c->P.J;                  // dmft_params.f90:94-96
     if (c->P.maxnstep_green <= 0) c->P.maxnstep_green = c->P.maxnstep;
     c->device = device;
-    // measured on MI355X at Ns=14 (sector (7,7)), single orbital: two columns
-    // per workgroup (47 us FP32, 81 us FP64) beat one (52 us, 94 us); two
-    // orbitals, FP32 vectors: two columns (H.v 79 us, solve 135 ms) beat one
-    // (100 us, 141 ms), FP64 vectors: one column (solve 262 against 280 ms)
-    c->cols = (p->norb == 1 || (p->norb == 2 && p->precision == EDG_FP32)) ? 2 : 1;
-    // Kanamori models (norb >= 2) accumulate FP32-vector row sums in FP64: the
-    // spin-flip / pair-hop terms make FP32 sums the leading error in G(i w)
-    c->acc64 = p->precision == EDG_FP32 && p->norb >= 2;
+    default_schedule(p->norb, p->precision == EDG_FP32, &c->cols, &c->acc64);
     auto bad = [&]() {
         (void)hipGetLastError();
         delete c;

@@ -166,6 +166,56 @@ __device__ __forceinline__ T gload(const T *p)
     }
 }
 
+// Raw buffer loads: the array's base lives in a 4-register scalar resource,
+// each lane gives a 32-bit byte offset and the wave a 32-bit scalar byte
+// offset.  The down-hop gathers of a tile read C x KH neighbour columns at
+// wave-uniform column offsets and one per-lane row offset: as buffer loads
+// they need no per-gather address arithmetic (a global load needs a 64-bit
+// vector address or a vector add of the column offset).  Word 3 is the
+// raw-buffer setting of CDNA (32-bit data format, no swizzle); accesses at
+// or beyond `bytes` read 0.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, long long bytes)
+{
+    const int n = bytes < 0x7FFFFFFFLL ? (int)bytes : 0x7FFFFFFF;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, n, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+template <>
+__device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+
+// Fields of a 32-bit hop record (k_compress): the target rank in bits 0-15
+// and the value-table index in bits 24-31, bits 16-23 zero, so that the LDS
+// address of the target row is one 24-bit multiply-add and the value index
+// one byte select.
+__device__ __forceinline__ uint32_t rec_rank(uint32_t w) { return w & 0xFFFFFFu; }
+__device__ __forceinline__ uint32_t rec_vidx(uint32_t w) { return w >> 24; }
+// The staged row a record points at, in a row-interleaved LDS tile of C
+// columns of T starting at `tile`: one 24-bit multiply-add of the record
+// (the compiler turns a plain multiply by the row size into a shift, a mask
+// and an add).  Arithmetic only; the load is an ordinary LDS read.
+template <typename T, int C>
+__device__ __forceinline__ Row<T, C> rec_row(const T *tile, uint32_t w)
+{
+    typedef const __attribute__((address_space(3))) T *LdsT;
+    const uint32_t base = (uint32_t)(size_t)(LdsT)tile;
+    uint32_t a;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(a) : "v"(w), "i"((int)(C * sizeof(T))), "v"(base));
+    const LdsT p = (LdsT)(size_t)a;
+    Row<T, C> r;
+#pragma unroll
+    for (int c = 0; c < C; ++c) r.v[c] = p[c];
+    return r;
+}
+
 // A value or pointer the whole wave holds, moved to scalar registers.
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 template <typename T>

@@ -43,7 +43,8 @@ struct SpinTab {
     // compressed hop tables of the single-orbital kernels, [n][KC]:
     // ccol = target rank, cidx = index into vsrc (bit 7: odd Jordan-Wigner
     // parity, the element is -vsrc[idx]), 64 = zero; hrec = target rank |
-    // (value-table index << 16), the value table being [idx] = +v,
+    // (value-table index << 24), bits 16-23 zero (rec_rank / rec_vidx in
+    // device_util.h), the value table being [idx] = +v,
     // [kMaxHops + 1 + idx] = -v (v the signed hybridization), idx 64 = 0
     int KC = 0;                    // 8 or 16 (0: not built)
     uint16_t *ccol = nullptr;
@@ -188,6 +189,19 @@ int launch_compress(hipStream_t s, const SpinTab &t);
 // Returns the number of workgroups (= partial sums written) or < 0 on error.
 template <typename T>
 int launch_hv(hipStream_t s, const HvOp &op, int mode, int cols, bool acc64, const HvArgs &a);
+// Default H.v schedule of a new context (and of edg_hx_noalloc), measured on
+// MI355X at Ns=14, sector (7,7): single orbital, two columns per workgroup
+// (FP32 43 us, FP64 81 us) beat one (51 us, 87 us); two orbitals, FP32
+// vectors: two columns with FP32 row sums (H.v 65.6 us, solve 122 ms) beat
+// FP64 sums (75.1 us, 131 ms) and one column (73.2 us, 130 ms), with errors
+// against the FP64 oracle's solve of |dE0| 1.7e-13 and max|dG| 2.5e-7 (FP64
+// sums: 1.3e-11 and 7.9e-7); FP64 vectors: one column (solve 262 against
+// 280 ms).  The general kernel (norb >= 3) sums FP32 rows in FP64.
+inline void default_schedule(int norb, bool fp32, int *cols, bool *acc64)
+{
+    *cols = (norb == 1 || (norb == 2 && fp32)) ? 2 : 1;
+    *acc64 = fp32 && norb >= 3;
+}
 // alpha[step] = scale[step] * sum(partial[0..nblk)) unless stopped.
 int launch_alpha(hipStream_t s, const double *partial, int nblk, const double *scale_j,
                  double *alpha_j, int *stop);

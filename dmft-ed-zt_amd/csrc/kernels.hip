@@ -266,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_spin_table(int nsite, int norb, int 
 // Compressed [n][KC] copies of the first KC slots: 16-bit target rank and
 // 8-bit element index (bit 7 = odd parity, the element being minus the
 // signed V at that index; 64 = zero element); and the same pair as one
-// 32-bit record, rank | (value-table index << 16).
+// 32-bit record, rank | (value-table index << 24).
 __global__ __launch_bounds__(kBlock) void k_compress(SpinTab t)
 {
     const int r = blockIdx.x * kBlock + threadIdx.x;
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_compress(SpinTab t)
         t.ccol[(long long)r * t.KC + k] = (uint16_t)h.col;
         t.cidx[(long long)r * t.KC + k] = (uint8_t)h.pad;
         const uint32_t vi = (h.pad & 0x7Fu) + ((h.pad >> 7) & 1u ? (uint32_t)kMaxHops + 1u : 0u);
-        t.hrec[(long long)r * t.KC + k] = (h.col & 0xFFFFu) | (vi << 16);
+        t.hrec[(long long)r * t.KC + k] = (h.col & 0xFFFFu) | (vi << 24);
     }
 }
 

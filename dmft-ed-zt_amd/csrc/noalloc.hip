@@ -209,8 +209,9 @@ extern "C" int edg_hx_noalloc(const edg_params *p, const double *em, const doubl
     a.y = y;
     a.accumulate = py;
     // the default schedule of a context (edg_create)
-    const int cols = (norb == 1 || (norb == 2 && p->precision == EDG_FP32)) ? 2 : 1;
-    const bool acc64 = p->precision == EDG_FP32 && norb >= 2;
+    int cols;
+    bool acc64;
+    default_schedule(norb, p->precision == EDG_FP32, &cols, &acc64);
     const int rc = es == 4 ? launch_hv<float>(st, op, HV_PLAIN, cols, acc64, a)
                            : launch_hv<double>(st, op, HV_PLAIN, cols, acc64, a);
     if (rc < 0) return nfail(EDG_ERR_HIP, "edg_hx_noalloc: H.v launch failed");

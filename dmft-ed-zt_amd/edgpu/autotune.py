@@ -25,8 +25,8 @@ def tune_single(solver, em, ek, vmk, ne_up, ne_dn, candidates=SINGLE_CANDIDATES,
     """Time each schedule of the H.v on sector (ne_up, ne_dn), install the
     fastest on `solver` and return (best, {schedule: microseconds})."""
     acc = fp64_accumulate
-    if acc is None:
-        acc = solver.precision == "fp32" and solver.norb >= 2
+    if acc is None:  # the context default: FP64 row sums for FP32 vectors with norb >= 3
+        acc = solver.precision == "fp32" and solver.norb >= 3
     times = {}
     for c in candidates:
         solver.set_tuning(c, acc)
@@ -119,7 +119,7 @@ def cached_single(solver, em, ek, vmk, ne_up, ne_dn, path):
         solver.set_tuning(cache[key]["best"], cache[key]["fp64_accumulate"])
         return cache[key]["best"], cache[key]["times_us"]
     best, times = tune_single(solver, em, ek, vmk, ne_up, ne_dn)
-    acc = solver.precision == "fp32" and solver.norb >= 2
+    acc = solver.precision == "fp32" and solver.norb >= 3
     cache[key] = dict(best=best, fp64_accumulate=acc, times_us={str(k): v for k, v in times.items()})
     save(path, cache)
     return best, times

@@ -84,7 +84,7 @@ int edg_device_count(int *n);
  * and two for 2, 4 or 8 (when their LDS fits); 16, 32 and 64 send two-orbital
  * models to the general kernel.  Defaults: 2 columns for one orbital and for
  * two orbitals with FP32 vectors, else 1; FP64 row sums for FP32 vectors
- * with two or more orbitals.  No
+ * with three or more orbitals.  No
  * counterpart in the reference (its hx is one scalar loop); set by the
  * autotuner or a caller that has measured. */
 int edg_set_tuning(edg_ctx *ctx, int cols_per_block, int fp64_accumulate);

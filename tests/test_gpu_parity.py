@@ -468,14 +468,15 @@ def test_kanamori_ns14_columns_vs_oracle(edgpu, oracle):
     s64.set_tuning(1, False)
     e64 = s64.ground_state(em, ek, vmk, [(7, 7)]).val
     G64, _ = s64.cluster_green_ftn(em, ek, vmk, om)
-    for cols in (1, 2):
-        print(f"kanamori ns14 fp32 solve cols={cols}", flush=True)
+    # FP32 row sums with two columns is the two-orbital FP32 default
+    for cols, acc64 in ((1, True), (2, True), (1, False), (2, False)):
+        print(f"kanamori ns14 fp32 solve cols={cols} fp64_sums={acc64}", flush=True)
         s = edgpu.Solver(precision="fp32", **kw)
-        s.set_tuning(cols, True)  # FP64 row sums, the two-orbital default
+        s.set_tuning(cols, acc64)
         e32 = s.ground_state(em, ek, vmk, [(7, 7)]).val
         G32, _ = s.cluster_green_ftn(em, ek, vmk, om)
-        assert abs(e32 - e64) <= 1e-6, cols
-        assert np.abs(G32 - G64).max() <= 1e-5, cols
+        assert abs(e32 - e64) <= 1e-6, (cols, acc64)
+        assert np.abs(G32 - G64).max() <= 1e-5, (cols, acc64)
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
