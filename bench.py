@@ -31,6 +31,11 @@ The same JSON line also carries:
             --serve-seconds (60): solves/s, p50/p99 latency (result time -
             arrival time) overall and by Ns, and the queue depth over time
             (backlog growth: whether the GPU keeps up with the offered rate);
+  streaming_ceiling  torch's device copy and three-stream add of
+            Ns=14-sized FP32 vectors: what plain streaming reaches here;
+  reorth    the Ns=14 FP32 solve with each reorthogonalization mode of
+            the Green's-function chains (none, partial, full): Green's-phase
+            time, reorthogonalized steps, G's error against the oracle;
   kanamori  the two-orbital Kanamori Ns=14 problem, FP32 vs FP64 vectors:
             solve times, errors against the oracle's solve, H.v time, and
             the oracle's ms per Lanczos step on the host cores;
@@ -343,6 +348,63 @@ def bench_served(edgpu, W, S, seconds, graphs=True):
                 depth_every_5s=[[round(d[0], 2), d[1], d[2], d[3]] for d in depth[::every]])
 
 
+def streaming_ceiling(n=3432 * 3432, reps=50):
+    """What plain streaming reaches on this GPU, for the roofline reading:
+    torch's device copy (read n FP32, write n) and add (read 2n, write n)
+    of Ns=14-sized vectors, timed with HIP events; TB/s.  None without torch."""
+    try:
+        import torch
+    except ImportError:
+        return None
+    x = torch.rand(n, device="cuda")
+    z = torch.rand(n, device="cuda")
+    y = torch.empty_like(x)
+
+    def rate(f, nbytes):
+        for _ in range(5):
+            f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) / reps * 1e3
+        return us, nbytes / us / 1e6
+
+    cu, ct = rate(lambda: y.copy_(x), 2 * 4 * n)
+    au, at = rate(lambda: torch.add(x, z, out=y), 3 * 4 * n)
+    del x, y, z
+    torch.cuda.empty_cache()
+    return dict(copy_us=cu, copy_tbs=ct, add_us=au, add_tbs=at, bytes_per_vector=4 * n)
+
+
+def bench_reorth(edgpu, W, ref, U, mu, em, ek, vmk):
+    """The Ns=14 FP32 solve with each reorthogonalization mode of the
+    Green's-function chains (0: none, the reference's plain recurrence; 1:
+    partial, Simon's omega recurrence; 2: every step against every stored
+    chain vector, FP64 Gram-Schmidt): Green's-phase time, reorthogonalized
+    steps and G's error against the FP64 oracle's plain solve."""
+    om = W.matsubara(2048, 100.0)
+    out = {}
+    for mode in (0, 1, 2):
+        s = make_solver(edgpu, "fp32", U, mu)
+        s.set_reorthogonalization(mode)
+        solve(s, em, ek, vmk, om)
+        res = solve(s, em, ek, vmk, om)
+        st = s.last_solve_stats()
+        rs = s.reorth_stats()
+        err = errors_vs_oracle(ref, f"u{U:g}", res)
+        out[str(mode)] = dict(green_phase_ms=st["green_ms"], gs_phase_ms=st["gs_ms"],
+                              reorth_events=rs["events"], tested_steps=rs["steps"],
+                              max_dG_vs_oracle=None if err is None else err["max_dG"])
+        s.close()
+    out["workload"] = ("Ns=14 single orbital, sector (7,7), FP32 vectors, 200 GS steps + 4 chains x 150 steps; "
+                       "modes 0 none, 1 partial (omega recurrence), 2 full")
+    return out
+
+
 def bench_kanamori(edgpu, W, ref):
     """Two-orbital Kanamori impurity, Ns=14 (6 bath sites per orbital),
     sector (7,7), U=3, J=U/4, U'=U-2J; FP32-vector vs FP64 solve with the
@@ -445,6 +507,7 @@ def main():
                          "(about 2 minutes each on 16 cores)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-kanamori", action="store_true")
+    ap.add_argument("--no-reorth", action="store_true")
     ap.add_argument("--no-graphs", action="store_true",
                     help="batched and served sections launch each step's kernels instead of "
                          "replaying a captured HIP graph (for kernel-trace profiles)")
@@ -542,6 +605,11 @@ def main():
         log("served stream")
         extra["served"] = bench_served(edgpu, W, S, args.serve_seconds, graphs=not args.no_graphs)
     threads = min(os.cpu_count() or 1, 16)
+    log("streaming ceiling")
+    extra["streaming_ceiling"] = streaming_ceiling()
+    if not args.no_reorth:
+        log("reorthogonalization modes")
+        extra["reorth"] = bench_reorth(edgpu, W, ref, U, mu, em, ek, vmk)
     if not args.no_kanamori:
         log("two-orbital Kanamori")
         extra["kanamori"] = bench_kanamori(edgpu, W, ref)
