@@ -43,7 +43,10 @@ namespace {
 
 constexpr long long kPadB = 64;       // element alignment of vectors
 constexpr int kPartialCap = 1 << 17;  // doubles of the shared partial-sum scratch
-constexpr size_t kLdsMax = 152 * 1024;
+// dynamic LDS of a batched tile: the 160 KiB of a gfx950 CU less the static
+// LDS of the batched kernels (k_bhv<double, double, 16> holds 14 KiB of
+// per-column neighbour offsets and elements, tables and reduction scratch)
+constexpr size_t kLdsMax = 144 * 1024;
 constexpr int kClasses = 9;            // H.v launch classes, chain_class()
 constexpr int kMaxChains = 8;          // c+/c chains per problem: 2 orbitals x 2 spins x 2
 constexpr int kChunk = 8;              // batched steps launched per chunk

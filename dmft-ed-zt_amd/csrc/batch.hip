@@ -29,18 +29,22 @@ __device__ __forceinline__ T *slot_of(const BInst &I, int k)
 // columns are contiguous in memory and staged in LDS; up hops gather from
 // the staged column, down hops read the neighbour column at the same row
 // from global memory (coalesced over the rows).  Hop records are the 32-bit
-// compressed tables (target rank | value index << 16, KC slots per row,
-// self-padded).
+// compressed tables (target rank | value index << 24, KC slots per row,
+// self-padded).  Each tile column's neighbour-column byte offsets and
+// elements are decoded once into LDS, and the vectors are read and written
+// through buffer resources with 32-bit offsets: a down hop then costs one
+// vector add (a 64-bit global address cost four instructions per hop).
 template <typename T, typename ACC, int KS>
 __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, const int *__restrict__ hv_off, const int *__restrict__ own,
                        int sub)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     T *xs = reinterpret_cast<T *>(smem);
-    __shared__ uint32_t s_drec[kBatchMaxCols * 16];
+    __shared__ uint32_t s_doff[kBatchMaxCols * KS];  // neighbour-column byte offsets
+    __shared__ ACC s_dv[kBatchMaxCols * KS];         // their elements
     __shared__ ACC s_ed[kBatchMaxCols];
     __shared__ uint8_t s_md[kBatchMaxCols];
-    __shared__ ACC s_vu[2 * (kMaxHops + 1)], s_vd[2 * (kMaxHops + 1)];
+    __shared__ ACC s_vu[2 * (kMaxHops + 1)];
 
     const int i = own[blockIdx.x];  // chain of this workgroup
     const BInst &I = inst[i];
@@ -60,11 +64,8 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, 
 
     for (int e = threadIdx.x; e <= kMaxHops; e += kBlock) {
         const ACC vu = e < op.nvup ? (ACC)gload(op.up.vsrc + e) : ACC(0);
-        const ACC vd = e < op.nvup ? (ACC)gload(op.dn.vsrc + e) : ACC(0);
         s_vu[e] = vu;
         s_vu[kMaxHops + 1 + e] = -vu;
-        s_vd[e] = vd;
-        s_vd[kMaxHops + 1 + e] = -vd;
     }
     const int nel = ncol * nup;
     const T *xt = x + (long long)c0 * nup;
@@ -82,7 +83,15 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, 
         for (int r = 0; r < 8; ++r)
             if (e0 + r * kBlock < nel) xs[e0 + r * kBlock] = v[r];
     }
-    for (int e = threadIdx.x; e < ncol * kcd; e += kBlock) s_drec[e] = gload(op.dn.hrec + (long long)c0 * kcd + e);
+    for (int e = threadIdx.x; e < ncol * kcd; e += kBlock) {
+        // value index vi: [vi] = +v, [kMaxHops + 1 + vi] = -v of the signed
+        // hybridization v = vsrc[vi] (vi >= nvup: a zero element)
+        const uint32_t w = gload(op.dn.hrec + (long long)c0 * kcd + e);
+        const uint32_t vi = rec_vidx(w), vb = vi > (uint32_t)kMaxHops ? vi - kMaxHops - 1 : vi;
+        const ACC v = vb < (uint32_t)op.nvup ? (ACC)gload(op.dn.vsrc + vb) : ACC(0);
+        s_doff[e] = rec_rank(w) * (uint32_t)nup * (uint32_t)sizeof(T);
+        s_dv[e] = vi > (uint32_t)kMaxHops ? -v : v;
+    }
     for (int c = threadIdx.x; c < ncol; c += kBlock) {
         s_ed[c] = energy_of<ACC>(op.dn, c0 + c);
         s_md[c] = gload(op.dn.mask + c0 + c);
@@ -110,28 +119,34 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, 
     const int G = kBlock / span;
     const int g = threadIdx.x / span;
     double dot = 0.0;
+    const long long vbytes = (long long)nup * ndn * (long long)sizeof(T);
+    const __amdgpu_buffer_rsrc_t xrs = buf_rsrc(x, vbytes), yrs = buf_rsrc(y, vbytes);
+    const __amdgpu_buffer_rsrc_t prs = buf_rsrc(pv ? pv : x, vbytes);
     if (g < G) {
         for (int iup = threadIdx.x - g * span; iup < nup; iup += span) {
             const Row<uint32_t, KS> ur = gload(reinterpret_cast<const Row<uint32_t, KS> *>(op.up.hrec + (long long)iup * KS));
             const ACC eu = energy_of<ACC>(op.up, iup);
             const bool mu = gload(op.up.mask + iup) != 0;
+            const uint32_t ib = (uint32_t)iup * (uint32_t)sizeof(T);
             for (int c = g; c < ncol; c += G) {
                 const T *xc = xs + c * nup;
-                const uint32_t *dr = s_drec + c * KS;
+                const uint32_t *doff = s_doff + c * KS;
+                const ACC *dv = s_dv + c * KS;
                 T gn[KS];
 #pragma unroll
-                for (int k = 0; k < KS; ++k) gn[k] = gload(x + (long long)rec_rank(dr[k]) * nup + iup);
+                for (int k = 0; k < KS; ++k) gn[k] = bload<T>(xrs, doff[k] + ib, 0);
+                const uint32_t eb = (uint32_t)(c0 + c) * (uint32_t)nup * (uint32_t)sizeof(T) + ib;
+                const T old = pv ? bload<T>(prs, eb, 0) : T(0);
                 const ACC xv = (ACC)xc[iup];
                 ACC acc = (eu + s_ed[c] + (mu && s_md[c] ? U : ACC(0))) * xv;
 #pragma unroll
                 for (int k = 0; k < KS; ++k) acc += s_vu[rec_vidx(ur.v[k])] * (ACC)xc[rec_rank(ur.v[k])];
 #pragma unroll
-                for (int k = 0; k < KS; ++k) acc += s_vd[rec_vidx(dr[k])] * (ACC)gn[k];
-                const long long gi = (long long)(c0 + c) * nup + iup;
+                for (int k = 0; k < KS; ++k) acc += dv[k] * (ACC)gn[k];
                 double r = sx * (double)acc;
-                if (pv) r -= bsp * (double)gload(pv + gi);
+                if (pv) r -= bsp * (double)old;
                 const T rt = (T)r;
-                gstore(y + gi, rt);
+                bstore<T>(yrs, eb, rt);
                 dot += (double)xv * (double)rt;
             }
         }

@@ -192,6 +192,20 @@ __device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, uint32
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
 }
 
+template <typename T>
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t voff, T v);
+template <>
+__device__ __forceinline__ void bstore<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, 0, 0);
+}
+template <>
+__device__ __forceinline__ void bstore<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff, double v)
+{
+    typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, (int)voff, 0, 0);
+}
+
 // Fields of a 32-bit hop record (k_compress): the target rank in bits 0-15
 // and the value-table index in bits 24-31, bits 16-23 zero, so that the LDS
 // address of the target row is one 24-bit multiply-add and the value index
