@@ -31,7 +31,7 @@ The same JSON line also carries:
             --serve-seconds (60): solves/s, p50/p99 latency (result time -
             arrival time) overall and by Ns, and the queue depth over time
             (backlog growth: whether the GPU keeps up with the offered rate);
-  streaming_ceiling  torch's device copy and three-stream add of
+  streaming_ceiling  a hand-written copy and three-stream add of
             Ns=14-sized FP32 vectors: what plain streaming reaches here;
   reorth    the Ns=14 FP32 solve with each reorthogonalization mode of
             the Green's-function chains (none, partial, full): Green's-phase
@@ -250,7 +250,8 @@ def bench_dmft_ns6(edgpu, W, cpu, threads):
 def profile_hv(edgpu, em, ek, vmk, U, mu):
     """H.v launches of the hot kernels plus one plain device copy of 2*4*D
     bytes (read 4D, write 4D) whose counter readings calibrate FETCH_SIZE /
-    WRITE_SIZE: the single-orbital k_hv1 (FP32, Ns=14, sector (7,7)), the
+    WRITE_SIZE: the single-orbital k_hv1 (FP32, Ns=14, sector (7,7)) and the
+    streaming reference kernels on vectors of the same size, the
     two-orbital k_hv2 (FP32 vectors, Ns=14 Kanamori, sector (7,7)), and one
     batched run of 64 single-orbital problems (Ns in {8,10,12}; k_bhv and
     k_baxpy)."""
@@ -258,6 +259,7 @@ def profile_hv(edgpu, em, ek, vmk, U, mu):
     from edgpu import workloads as W
     s32 = make_solver(edgpu, "fp32", U, mu)
     ms, by = s32.time_hv(em, ek, vmk, 7, 7, warmup=2, reps=10)
+    stream = streaming_ceiling(s32, reps=10)
     s32.close()
     emk, ekk, vmkk, muk, J, Up = W.two_orbital_kanamori(6, 3.0, seed=1)
     sk = edgpu.Solver(norb=2, nbath=12, U=3.0, J=J, Up=Up, Jp=J, mu=muk, maxnstep=200,
@@ -284,7 +286,7 @@ def profile_hv(edgpu, em, ek, vmk, U, mu):
     hip.hipDeviceSynchronize()
     hip.hipFree(a)
     hip.hipFree(b)
-    print(json.dumps({"hv_us": ms * 1e3, "hv_bytes": by, "kanamori_hv_us": ms_k * 1e3,
+    print(json.dumps({"hv_us": ms * 1e3, "hv_bytes": by, "streaming_ceiling": stream, "kanamori_hv_us": ms_k * 1e3,
                       "kanamori_hv_bytes": by_k, "batched_chain_steps": st["chain_steps"],
                       "copy_bytes_read": nbytes, "copy_bytes_written": nbytes}))
 
@@ -348,36 +350,14 @@ def bench_served(edgpu, W, S, seconds, graphs=True):
                 depth_every_5s=[[round(d[0], 2), d[1], d[2], d[3]] for d in depth[::every]])
 
 
-def streaming_ceiling(n=3432 * 3432, reps=50):
+def streaming_ceiling(solver, n=3432 * 3432, reps=100):
     """What plain streaming reaches on this GPU, for the roofline reading:
-    torch's device copy (read n FP32, write n) and add (read 2n, write n)
-    of Ns=14-sized vectors, timed with HIP events; TB/s.  None without torch."""
-    try:
-        import torch
-    except ImportError:
-        return None
-    x = torch.rand(n, device="cuda")
-    z = torch.rand(n, device="cuda")
-    y = torch.empty_like(x)
-
-    def rate(f, nbytes):
-        for _ in range(5):
-            f()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(reps):
-            f()
-        e1.record()
-        torch.cuda.synchronize()
-        us = e0.elapsed_time(e1) / reps * 1e3
-        return us, nbytes / us / 1e6
-
-    cu, ct = rate(lambda: y.copy_(x), 2 * 4 * n)
-    au, at = rate(lambda: torch.add(x, z, out=y), 3 * 4 * n)
-    del x, y, z
-    torch.cuda.empty_cache()
-    return dict(copy_us=cu, copy_tbs=ct, add_us=au, add_tbs=at, bytes_per_vector=4 * n)
+    the library's hand-written 16-byte-access copy (read 4n bytes, write 4n)
+    and add (read 8n, write 4n) of Ns=14-sized FP32 vectors, timed with HIP
+    events on the solver's stream (edg_time_stream); TB/s."""
+    cm, am = solver.time_stream(n, reps=reps)
+    return dict(copy_us=cm * 1e3, copy_tbs=8 * n / (cm * 1e-3) / 1e12,
+                add_us=am * 1e3, add_tbs=12 * n / (am * 1e-3) / 1e12, floats_per_vector=n)
 
 
 def bench_reorth(edgpu, W, ref, U, mu, em, ek, vmk):
@@ -606,7 +586,7 @@ def main():
         extra["served"] = bench_served(edgpu, W, S, args.serve_seconds, graphs=not args.no_graphs)
     threads = min(os.cpu_count() or 1, 16)
     log("streaming ceiling")
-    extra["streaming_ceiling"] = streaming_ceiling()
+    extra["streaming_ceiling"] = streaming_ceiling(s32)
     if not args.no_reorth:
         log("reorthogonalization modes")
         extra["reorth"] = bench_reorth(edgpu, W, ref, U, mu, em, ek, vmk)

@@ -1063,8 +1063,9 @@ int direct_step(edg_batch *b, Chain &C, int j)
     ha.beta = I.beta;
     ha.stop = I.flags;
     ha.nstep = I.flags + 1;
-    const int cols = (I.op.norb == 1 || !b->fp64) ? 2 : 1;
-    const bool acc64 = !b->fp64 && I.op.norb >= 2;
+    int cols;
+    bool acc64;
+    default_schedule(I.op.norb, !b->fp64, &cols, &acc64);  // the single-problem path's schedule
     const int hv_n = launch_hv<T>(b->stream, I.op, HV_LANCZOS, cols, acc64, ha);
     if (hv_n < 0 || hv_n > I.hvn) return fail(b, EDG_ERR_HIP, "direct H.v launch failed");
     if (j == I.maxn - 1) {

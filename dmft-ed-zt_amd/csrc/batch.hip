@@ -293,7 +293,7 @@ static int set_lds_ks(size_t max_lds)
 template <int KS>
 static int set_lds_kan(size_t max_lds)
 {
-    const void *fs[2] = {(const void *)k_bhv2<float, double, kBatchKanColsF32, KS>,
+    const void *fs[2] = {(const void *)k_bhv2<float, float, kBatchKanColsF32, KS>,
                          (const void *)k_bhv2<double, double, 1, KS>};
     for (const void *f : fs)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds) != hipSuccess)
@@ -338,9 +338,9 @@ int launch_batch_hv2(hipStream_t s, bool fp64, int kh, const BInst *inst, const 
 {
     if (ntiles <= 0) return 0;
     void (*f)(const BInst *, const int *, const int *, int);
-    if (kh == 8) f = fp64 ? k_bhv2<double, double, 1, 8> : k_bhv2<float, double, kBatchKanColsF32, 8>;
-    else if (kh == 12) f = fp64 ? k_bhv2<double, double, 1, 12> : k_bhv2<float, double, kBatchKanColsF32, 12>;
-    else if (kh == 16) f = fp64 ? k_bhv2<double, double, 1, 16> : k_bhv2<float, double, kBatchKanColsF32, 16>;
+    if (kh == 8) f = fp64 ? k_bhv2<double, double, 1, 8> : k_bhv2<float, float, kBatchKanColsF32, 8>;
+    else if (kh == 12) f = fp64 ? k_bhv2<double, double, 1, 12> : k_bhv2<float, float, kBatchKanColsF32, 12>;
+    else if (kh == 16) f = fp64 ? k_bhv2<double, double, 1, 16> : k_bhv2<float, float, kBatchKanColsF32, 16>;
     else return -1;
     hipLaunchKernelGGL(f, dim3(ntiles), dim3(kHvBlock), lds_bytes, s, inst, hv_off, own, sub);
     return hipGetLastError() == hipSuccess ? 0 : -1;

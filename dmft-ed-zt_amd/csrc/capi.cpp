@@ -1189,6 +1189,35 @@ int edg_time_hv(edg_ctx *c, const double *em, const double *ek, const double *vm
                                       : time_hv_t<double>(c, ne_up, ne_dn, warmup, reps, ms_per_launch, bytes_per_launch);
 }
 
+int edg_time_stream(edg_ctx *c, long long n, int reps, double *copy_ms, double *add_ms)
+{
+    if (!c || n < 4 || reps < 1 || !copy_ms || !add_ms) return EDG_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    n &= ~3LL;
+    const long long np = round_up(n, kPad);
+    if (int rc = ensure(c, c->tmp, 3 * np * sizeof(float))) return rc;
+    float *x = (float *)c->tmp.p, *z = x + np, *y = z + np;
+    if (launch_start<float>(c->stream, x, n, 1) || launch_start<float>(c->stream, z, n, 2))
+        return fail(c, EDG_ERR_HIP, "start launch failed");
+    c->have_gs_ev = false;
+    for (int pass = 0; pass < 2; ++pass) {
+        auto one = [&]() {
+            return pass == 0 ? launch_stream_copy(c->stream, x, y, n) : launch_stream_add(c->stream, x, z, y, n);
+        };
+        for (int i = 0; i < 5; ++i)
+            if (one()) return fail(c, EDG_ERR_HIP, "stream launch failed");
+        HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+        for (int i = 0; i < reps; ++i)
+            if (one()) return fail(c, EDG_ERR_HIP, "stream launch failed");
+        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev[1]));
+        float t = 0;
+        HIPCHK(c, hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
+        (pass == 0 ? *copy_ms : *add_ms) = (double)t / reps;
+    }
+    return EDG_OK;
+}
+
 int edg_last_solve_stats(edg_ctx *c, double *phase_ms, long long *n_hv, long long *n_steps)
 {
     if (!c || !phase_ms) return EDG_ERR_ARG;

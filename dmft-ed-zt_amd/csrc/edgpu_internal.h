@@ -225,6 +225,10 @@ int launch_axpy_norm(hipStream_t s, T *r, const T *u, long long n, const double 
                      int *stop);
 template <typename T>
 int launch_start(hipStream_t s, T *x, long long n, unsigned long long seed);
+// Streaming reference passes over n floats (n a multiple of 4): y = x
+// (copy) and y = x + z (add), 16-byte accesses, grid-stride.
+int launch_stream_copy(hipStream_t s, const float *x, float *y, long long n);
+int launch_stream_add(hipStream_t s, const float *x, const float *z, float *y, long long n);
 template <typename T>
 int launch_combine(hipStream_t s, T *out, const T *basis, long long stride, int nvec,
                    const double *coef, long long n, double *partial,

@@ -1038,6 +1038,41 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t z)
     return z ^ (z >> 31);
 }
 
+// Streaming reference passes (edg_time_stream): what plain 16-byte loads and
+// stores reach on this device, the yardstick of the H.v's fabric traffic.
+// One 16-byte element per thread, the grid covering the vector once.
+// Measured on MI355X at 47 MB per vector: a grid-stride loop over a capped
+// grid (part of the grid takes a second pass) and four elements per thread
+// with the loads first were both slower (15.4 and 23.7 against 14.2 us per
+// copy in a kernel trace).
+__global__ __launch_bounds__(kBlock) void k_stream_copy(const float4 *x, float4 *y, long long n4)
+{
+    const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n4) y[i] = x[i];
+}
+__global__ __launch_bounds__(kBlock) void k_stream_add(const float4 *x, const float4 *z, float4 *y, long long n4)
+{
+    const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n4) {
+        const float4 a = x[i], b = z[i];
+        y[i] = make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+    }
+}
+static int stream_grid(long long n4) { return (int)((n4 + kBlock - 1) / kBlock); }
+int launch_stream_copy(hipStream_t s, const float *x, float *y, long long n)
+{
+    const long long n4 = n / 4;
+    hipLaunchKernelGGL(k_stream_copy, dim3(stream_grid(n4)), dim3(kBlock), 0, s, (const float4 *)x, (float4 *)y, n4);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+int launch_stream_add(hipStream_t s, const float *x, const float *z, float *y, long long n)
+{
+    const long long n4 = n / 4;
+    hipLaunchKernelGGL(k_stream_add, dim3(stream_grid(n4)), dim3(kBlock), 0, s, (const float4 *)x,
+                       (const float4 *)z, (float4 *)y, n4);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_start(T *x, long long n, unsigned long long seed)
 {

@@ -52,7 +52,7 @@ EXPORTS = ["edg_default_params", "edg_create", "edg_destroy", "edg_last_error",
            "edg_set_memory_budget", "edg_device_count", "edg_set_tuning", "edg_spin_table",
            "edg_hx", "edg_lanczos_iteration", "edg_ground_state", "edg_gs_info",
            "edg_get_gs_vector", "edg_set_gs", "edg_cluster_green", "edg_observables",
-           "edg_apply_c", "edg_time_hv", "edg_last_solve_stats",
+           "edg_apply_c", "edg_time_hv", "edg_time_stream", "edg_last_solve_stats",
            "edg_batch_create", "edg_batch_destroy", "edg_batch_last_error",
            "edg_batch_set_graphs", "edg_batch_problem_bytes", "edg_batch_submit",
            "edg_batch_step", "edg_batch_run", "edg_batch_poll", "edg_batch_stats",
@@ -101,6 +101,7 @@ def load_library(path=LIB_PATH):
     L.edg_self_energy.argtypes = [ctx, dp, dp, dp, C.c_int, dp, dp, dp, dp]
     L.edg_apply_c.argtypes = [ctx, C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_int, dp]
     L.edg_time_hv.argtypes = [ctx, dp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
+    L.edg_time_stream.argtypes = [ctx, C.c_longlong, C.c_int, dp, dp]
     L.edg_last_solve_stats.argtypes = [ctx, dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     bh = C.c_void_p
     L.edg_batch_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_ulonglong,
@@ -360,6 +361,13 @@ class Solver:
         self._chk(self._L.edg_time_hv(self._h, _d(em), _d(ek), _d(vmk), ne_up, ne_dn, warmup,
                                       reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+    def time_stream(self, n, reps=50):
+        """(ms per copy of n floats, ms per add of n floats): the streaming
+        reference kernels, timed with HIP events on the solver's stream."""
+        cm, am = C.c_double(0), C.c_double(0)
+        self._chk(self._L.edg_time_stream(self._h, int(n), int(reps), C.byref(cm), C.byref(am)))
+        return cm.value, am.value
 
     def last_solve_stats(self):
         ph = (C.c_double * 2)()

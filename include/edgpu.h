@@ -226,6 +226,14 @@ int edg_apply_c(edg_ctx *ctx, int ne_up, int ne_dn, const double *vec, int pm, i
 int edg_time_hv(edg_ctx *ctx, const double *em, const double *ek, const double *vmk, int ne_up,
                 int ne_dn, int warmup, int reps, double *ms_per_launch, double *bytes_per_launch);
 
+/* Streaming reference (no counterpart in the reference): average
+ * milliseconds of a copy y = x (4n bytes read, 4n written) and of an add
+ * y = x + z (8n read, 4n written) over n floats on the context's stream,
+ * hand-written 16-byte-access kernels timed with HIP events over `reps`
+ * launches after 5 warm-up ones: the plain-streaming rate the H.v's
+ * fabric traffic is measured against. */
+int edg_time_stream(edg_ctx *ctx, long long n, int reps, double *copy_ms, double *add_ms);
+
 /* Full solve timing helper: ground state in one sector + all Green's
  * function chains, returning wall milliseconds of each phase measured with
  * HIP events on the context stream (phase_ms[0] ground state,

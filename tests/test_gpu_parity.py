@@ -517,3 +517,14 @@ def test_autotune_single_schedule(edgpu, oracle, tmp_path):
     y = s.hx(em, ek, vmk, 5, 5, x)
     ref = oracle.hx(m, em, ek, vmk, b, x)
     assert np.abs(y - ref).max() <= 2e-6 * np.abs(ref).max()
+
+
+def test_streaming_reference_timing(edgpu):
+    """edg_time_stream (the streaming yardstick of the H.v roofline) times
+    its copy and add on the device and refuses bad arguments."""
+    s = edgpu.Solver(precision="fp32")
+    cm, am = s.time_stream(1 << 22, reps=5)
+    assert 0.0 < cm < 10.0 and 0.0 < am < 10.0
+    with pytest.raises(edgpu.EdgpuError):
+        s.time_stream(2, reps=5)
+    s.close()
