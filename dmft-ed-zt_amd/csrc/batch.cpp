@@ -16,8 +16,8 @@
 // Every Lanczos chain in flight (ground state or c+/c, any problem, any
 // step) is advanced by the same launches per step (batch.hip: one batched
 // H.v per launch class and the axpy; one step advance per chunk of up to 8
-// steps); the stages between
-// chains are short per-problem launches on a second stream.
+// steps); the stages between chains are short per-problem launches on
+// side streams (problems dealt round-robin over kSides of them).
 // Host-side waits use events polled without blocking, so new problems are
 // admitted while others run (continuous batching).
 #include <hip/hip_runtime.h>
@@ -50,6 +50,12 @@ constexpr size_t kLdsMax = 144 * 1024;
 constexpr int kClasses = 9;            // H.v launch classes, chain_class()
 constexpr int kMaxChains = 8;          // c+/c chains per problem: 2 orbitals x 2 spins x 2
 constexpr int kChunk = 8;              // batched steps launched per chunk
+// Side streams of the stage work, problems dealt round-robin: the stage
+// kernels are small (a few workgroups each), and a batch whose problems all
+// reach a stage together queued them on one stream one after another while
+// the batched steps had no chains to run.  With the step stream this is the
+// device's 4 hardware queues.
+constexpr int kSides = 3;
 constexpr int kGraphAfter = 2;         // chunks of an unchanged chain set before one is captured
 constexpr size_t kDescMin = 1 << 20;   // initial bytes of the descriptor buffer
 constexpr int kAhead = 2;              // chunks the host may run ahead of the device
@@ -145,6 +151,7 @@ struct Arena {
 
 struct Job {
     long long id = 0;
+    int lane = 0;  // its side stream (stage work) and that stream's partial-sum scratch
     edg_problem p{};
     std::vector<double> ek, vmk;
     Stage st = ST_QUEUED;
@@ -180,15 +187,18 @@ struct Job {
 struct edg_batch {
     int device = 0;
     // stream: the batched Lanczos steps (and the reads of finished chains);
-    // side: per-problem stage work (tables, start vectors, combine, RQ,
-    // apply_c, continued fractions), which overlaps the steps.  The streams
+    // side[lane]: per-problem stage work (tables, start vectors, combine, RQ,
+    // apply_c, continued fractions) of the problems of that lane, which
+    // overlaps the steps and the other lanes' stages.  The streams
     // never wait on each other: the host moves a problem to its next stage,
     // and lets its new chains join the steps, only after an event shows the
     // previous stage's work is done.  The host stays at most kAhead chunks
     // of kChunk steps ahead of the device, so chains that become ready join
     // within a few steps.
     hipStream_t stream = nullptr;
-    hipStream_t side = nullptr;
+    hipStream_t side[kSides] = {};
+    double *side_part[kSides] = {};  // partial-sum scratch of each side stream's stage kernels
+    int next_lane = 0;
     hipEvent_t chunk_ev[4] = {};
     int chunk_head = 0, chunk_n = 0;
     bool fp64 = false;
@@ -204,7 +214,6 @@ struct edg_batch {
     std::vector<Job *> running;
     std::deque<Job *> finished;
     long long next_id = 1;
-    double *partial = nullptr;
     // chains in the batched launches, and the direct ones
     std::vector<Chain *> active, direct;
     bool dirty = true;
@@ -445,11 +454,11 @@ int build_table(edg_batch *b, Job *J, int spin, int ne, SpinTab &t, int kc, int 
     const double *ek_s = J->dparams + 2 * norb + (size_t)(spin - 1) * nbath;
     const double *v_s = J->dparams + 2 * norb + 2 * (size_t)nbath + (size_t)(spin - 1) * norb * nbath;
     t.vsrc = v_s;
-    BCHK(b, hipMemsetAsync(t.maxcnt, 0, sizeof(int), b->side));
+    BCHK(b, hipMemsetAsync(t.maxcnt, 0, sizeof(int), b->side[J->lane]));
     const double UpmJ = norb >= 2 ? kan_up(J->p) - J->p.J : 0.0;
-    if (launch_spin_table(b->side, ns, norb, nbath, ne, em_s, ek_s, v_s, J->p.mu, UpmJ, t))
+    if (launch_spin_table(b->side[J->lane], ns, norb, nbath, ne, em_s, ek_s, v_s, J->p.mu, UpmJ, t))
         return fail(b, EDG_ERR_HIP, "spin table launch failed");
-    if (launch_compress(b->side, t)) return fail(b, EDG_ERR_HIP, "compress launch failed");
+    if (launch_compress(b->side[J->lane], t)) return fail(b, EDG_ERR_HIP, "compress launch failed");
     return 0;
 }
 
@@ -501,7 +510,7 @@ int init_chain(edg_batch *b, Job *J, Chain &C, const HvOp &op, int maxn, bool st
     I.hvpart = I.scale + maxn + 1;
     I.axpart = I.hvpart + I.hvn;
     if (J->ar.overflow) return fail(b, EDG_ERR_NOMEM, "problem arena overflow (chain)");
-    BCHK(b, hipMemsetAsync(st, 0, state_bytes(maxn, I.hvn, I.axn), b->side));
+    BCHK(b, hipMemsetAsync(st, 0, state_bytes(maxn, I.hvn, I.axn), b->side[J->lane]));
     C.D = D;
     C.exists = true;
     C.launched = 0;
@@ -517,10 +526,10 @@ int start_chain(edg_batch *b, Job *J, Chain &C, double *norm_out, int slot)
     int *h = (int *)(J->host + J->host_n - 16) + 2 * slot;
     h[0] = 0;
     h[1] = I.maxn;
-    BCHK(b, hipMemcpyAsync(I.flags, h, 2 * sizeof(int), hipMemcpyHostToDevice, b->side));
-    int rc = b->fp64 ? launch_norm<double>(b->side, (const double *)I.base, C.D, b->partial,
+    BCHK(b, hipMemcpyAsync(I.flags, h, 2 * sizeof(int), hipMemcpyHostToDevice, b->side[J->lane]));
+    int rc = b->fp64 ? launch_norm<double>(b->side[J->lane], (const double *)I.base, C.D, b->side_part[J->lane],
                                            norm_out, I.scale, nullptr)
-                     : launch_norm<float>(b->side, (const float *)I.base, C.D, b->partial,
+                     : launch_norm<float>(b->side[J->lane], (const float *)I.base, C.D, b->side_part[J->lane],
                                           norm_out, I.scale, nullptr);
     if (rc) return fail(b, EDG_ERR_HIP, "norm launch failed");
     C.pending = true;  // joins the steps when the host sees J->start_ev complete
@@ -556,7 +565,7 @@ int admit(edg_batch *b, Job *J)
     }
     std::memcpy(h + 2 * no, J->ek.data(), sizeof(double) * 2 * nbath);
     std::memcpy(h + 2 * no + 2 * nbath, J->vmk.data(), sizeof(double) * 2 * no * nbath);
-    BCHK(b, hipMemcpyAsync(J->dparams, h, 8 * npar, hipMemcpyHostToDevice, b->side));
+    BCHK(b, hipMemcpyAsync(J->dparams, h, 8 * npar, hipMemcpyHostToDevice, b->side[J->lane]));
     // exact hop counts (zero hybridizations skipped, as the tables do)
     int km[2][3] = {}, kc = 8;
     for (int s = 0; s < 2; ++s) {
@@ -585,11 +594,11 @@ int admit(edg_batch *b, Job *J)
     if (J->ar.overflow) return fail(b, EDG_ERR_NOMEM, "problem arena overflow");
     if (int rc = init_chain(b, J, J->gsc, op, p.maxnstep, true)) return rc;
     const long long D = J->gsc.D;
-    const int rc0 = b->fp64 ? launch_start<double>(b->side, (double *)J->gsc.bi.base, D, p.seed)
-                            : launch_start<float>(b->side, (float *)J->gsc.bi.base, D, p.seed);
+    const int rc0 = b->fp64 ? launch_start<double>(b->side[J->lane], (double *)J->gsc.bi.base, D, p.seed)
+                            : launch_start<float>(b->side[J->lane], (float *)J->gsc.bi.base, D, p.seed);
     if (rc0) return fail(b, EDG_ERR_HIP, "start launch failed");
     if (int rc = start_chain(b, J, J->gsc, &J->misc[0], 0)) return rc;
-    BCHK(b, hipEventRecord(J->start_ev, b->side));
+    BCHK(b, hipEventRecord(J->start_ev, b->side[J->lane]));
     J->start_pending = true;
     J->st = ST_GS;
     b->running.push_back(J);
@@ -641,26 +650,26 @@ int after_gs(edg_batch *b, Job *J)
         return fail(b, EDG_ERR_SOLVER, "tridiagonal eigensolve failed");
     double *hc = J->host + 4 * (size_t)m + 8;
     for (int j = 0; j < nst; ++j) hc[j] = z[j] * s[j];
-    BCHK(b, hipMemcpyAsync(J->coef, hc, sizeof(double) * nst, hipMemcpyHostToDevice, b->side));
+    BCHK(b, hipMemcpyAsync(J->coef, hc, sizeof(double) * nst, hipMemcpyHostToDevice, b->side[J->lane]));
     const HvOp op = J->gsc.bi.op;
     const long long D = J->gsc.D;
     int rc;
     if (b->fp64) {
-        rc = launch_combine<double>(b->side, (double *)J->gs, (const double *)J->gsc.bi.base, J->gsc.bi.stride,
-                                    nst, J->coef, D, b->partial, &J->misc[4], &J->misc[5]);
-        rc |= launch_scale<double>(b->side, (double *)J->gs, D, &J->misc[5]);
+        rc = launch_combine<double>(b->side[J->lane], (double *)J->gs, (const double *)J->gsc.bi.base, J->gsc.bi.stride,
+                                    nst, J->coef, D, b->side_part[J->lane], &J->misc[4], &J->misc[5]);
+        rc |= launch_scale<double>(b->side[J->lane], (double *)J->gs, D, &J->misc[5]);
     } else {
-        rc = launch_combine<float>(b->side, (float *)J->gs, (const float *)J->gsc.bi.base, J->gsc.bi.stride,
-                                   nst, J->coef, D, b->partial, &J->misc[4], &J->misc[5]);
-        rc |= launch_scale<float>(b->side, (float *)J->gs, D, &J->misc[5]);
+        rc = launch_combine<float>(b->side[J->lane], (float *)J->gs, (const float *)J->gsc.bi.base, J->gsc.bi.stride,
+                                   nst, J->coef, D, b->side_part[J->lane], &J->misc[4], &J->misc[5]);
+        rc |= launch_scale<float>(b->side[J->lane], (float *)J->gs, D, &J->misc[5]);
     }
     if (rc) return fail(b, EDG_ERR_HIP, "combine launch failed");
     HvArgs ha{};
     ha.x = J->gs;
-    ha.partial = b->partial;
+    ha.partial = b->side_part[J->lane];
     ha.result = &J->misc[1];
-    const int hrc = b->fp64 ? launch_hv<double>(b->side, op, HV_RQ, 2, true, ha)
-                            : launch_hv<float>(b->side, op, HV_RQ, 2, true, ha);
+    const int hrc = b->fp64 ? launch_hv<double>(b->side[J->lane], op, HV_RQ, 2, true, ha)
+                            : launch_hv<float>(b->side[J->lane], op, HV_RQ, 2, true, ha);
     if (hrc < 0) return fail(b, EDG_ERR_HIP, "Rayleigh-quotient launch failed");
     // c+/c start vectors of every orbital (dmft_green.f90:71-118, 202-257),
     // one ring per chain
@@ -683,18 +692,18 @@ int after_gs(edg_batch *b, Job *J)
         J->map[k] = (uint32_t *)J->ar.take(4 * (size_t)tout.n);
         if (J->ar.overflow) return fail(b, EDG_ERR_NOMEM, "problem arena overflow (map)");
         const int ne_in = C.ispin == 1 ? p.nup : p.ndn;
-        if (launch_cmap(b->side, tout.pattern, tout.n, ns_site, ne_in, C.iorb - 1, C.pm, J->map[k]))
+        if (launch_cmap(b->side[J->lane], tout.pattern, tout.n, ns_site, ne_in, C.iorb - 1, C.pm, J->map[k]))
             return fail(b, EDG_ERR_HIP, "cmap launch failed");
         const int godd = C.ispin == 2 ? (p.nup & 1) : 0;
-        const int arc = b->fp64 ? launch_apply_c<double>(b->side, (const double *)J->gs, (double *)C.bi.base,
+        const int arc = b->fp64 ? launch_apply_c<double>(b->side[J->lane], (const double *)J->gs, (double *)C.bi.base,
                                                          cop.nup, cop.ndn, op.nup, C.ispin, J->map[k], godd)
-                                : launch_apply_c<float>(b->side, (const float *)J->gs, (float *)C.bi.base,
+                                : launch_apply_c<float>(b->side[J->lane], (const float *)J->gs, (float *)C.bi.base,
                                                         cop.nup, cop.ndn, op.nup, C.ispin, J->map[k], godd);
         if (arc) return fail(b, EDG_ERR_HIP, "apply_c launch failed");
         if (int r = start_chain(b, J, C, &J->misc[8 + k], 1 + k)) return r;
     }
     // also covers the RQ sums a ground-state-only problem reads next
-    BCHK(b, hipEventRecord(J->start_ev, b->side));
+    BCHK(b, hipEventRecord(J->start_ev, b->side[J->lane]));
     J->start_pending = true;
     J->st = ST_CHAINS;
     return 0;
@@ -748,14 +757,14 @@ int after_chains(edg_batch *b, Job *J)
             const size_t q = ((size_t)o * 2 + s) * 4 * mg;
             std::memcpy(hc + q, co[o][s].data(), sizeof(double) * 4 * mg);
             const double *dc = J->coef + q;
-            BCHK(b, hipMemcpyAsync((void *)dc, hc + q, sizeof(double) * 4 * mg, hipMemcpyHostToDevice, b->side));
-            if (launch_cf(b->side, b->nw, b->omega, J->e0, nsv[o][s][0], dc, dc + mg, nsv[o][s][1], dc + 2 * mg,
+            BCHK(b, hipMemcpyAsync((void *)dc, hc + q, sizeof(double) * 4 * mg, hipMemcpyHostToDevice, b->side[J->lane]));
+            if (launch_cf(b->side[J->lane], b->nw, b->omega, J->e0, nsv[o][s][0], dc, dc + mg, nsv[o][s][1], dc + 2 * mg,
                           dc + 3 * mg, J->gdev + ((size_t)o * 2 + s) * 2 * b->nw, 0))
                 return fail(b, EDG_ERR_HIP, "continued fraction launch failed");
         }
     double *hg = J->host + host_g(mg);
-    BCHK(b, hipMemcpyAsync(hg, J->gdev, sizeof(double) * 4 * b->nw * no, hipMemcpyDeviceToHost, b->side));
-    BCHK(b, hipEventRecord(J->ev, b->side));
+    BCHK(b, hipMemcpyAsync(hg, J->gdev, sizeof(double) * 4 * b->nw * no, hipMemcpyDeviceToHost, b->side[J->lane]));
+    BCHK(b, hipEventRecord(J->ev, b->side[J->lane]));
     J->st = ST_CF_READ;
     return 0;
 }
@@ -1183,11 +1192,11 @@ int edg_batch_create(int device, int precision, int nspin, int nw, const double 
         if (e != hipSuccess && !rc) rc = EDG_ERR_HIP;
     };
     chk(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-    chk(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+    for (hipStream_t &s : b->side) chk(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     for (auto &e : b->chunk_ev) chk(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     chk(hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, device));
     chk(hipMalloc((void **)&b->omega, sizeof(double) * nw));
-    chk(hipMalloc((void **)&b->partial, sizeof(double) * kPartialCap));
+    for (double *&p : b->side_part) chk(hipMalloc((void **)&p, sizeof(double) * kPartialCap));
     chk(hipEventCreateWithFlags(&b->hdesc_ev[0], hipEventDisableTiming));
     chk(hipEventCreateWithFlags(&b->hdesc_ev[1], hipEventDisableTiming));
     if (!rc) chk(hipMemcpy(b->omega, omega, sizeof(double) * nw, hipMemcpyHostToDevice));
@@ -1215,7 +1224,8 @@ int edg_batch_destroy(edg_batch *b)
     if (!b) return EDG_OK;
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    if (b->side) (void)hipStreamSynchronize(b->side);
+    for (hipStream_t s : b->side)
+        if (s) (void)hipStreamSynchronize(s);
     auto drop = [&](Job *J) {
         if (J->ar.base && b->heap) b->heap->free(J->ar.base);
         if (J->host) (void)hipHostFree(J->host);
@@ -1235,10 +1245,12 @@ int edg_batch_destroy(edg_batch *b)
         if (b->hdesc_ev[f]) (void)hipEventDestroy(b->hdesc_ev[f]);
     }
     if (b->omega) (void)hipFree(b->omega);
-    if (b->partial) (void)hipFree(b->partial);
+    for (double *p : b->side_part)
+        if (p) (void)hipFree(p);
     for (auto &e : b->chunk_ev)
         if (e) (void)hipEventDestroy(e);
-    if (b->side) (void)hipStreamDestroy(b->side);
+    for (hipStream_t s : b->side)
+        if (s) (void)hipStreamDestroy(s);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return EDG_OK;
@@ -1306,6 +1318,8 @@ int edg_batch_submit(edg_batch *b, const edg_problem *p, long long *id)
                                               " device bytes, budget is " + std::to_string(b->budget));
     }
     J->id = b->next_id++;
+    J->lane = b->next_lane;
+    b->next_lane = (b->next_lane + 1) % kSides;
     *id = J->id;
     b->queue.push_back(J);
     return EDG_OK;
