@@ -1325,13 +1325,14 @@ struct cplx {
 };
 __device__ __forceinline__ cplx cdiv(double num, cplx d)
 {
-    // num / d for real num, Smith's scaling
-    if (fabs(d.re) >= fabs(d.im)) {
-        const double r = d.im / d.re, den = d.re + d.im * r;
-        return {num / den, -num * r / den};
-    }
-    const double r = d.re / d.im, den = d.re * r + d.im;
-    return {num * r / den, -num / den};
+    // num / d for real num, Smith's scaling, without a branch: the lanes of
+    // a wave take either ordering (a branch ran both sides) and the level's
+    // two divisions are the only ones on the recursion's critical path
+    const bool big_re = fabs(d.re) >= fabs(d.im);
+    const double p = big_re ? d.re : d.im, q = big_re ? d.im : d.re;
+    const double r = q / p, inv = 1.0 / (p + q * r);
+    const double a = num * inv, b = num * r * inv;
+    return big_re ? cplx{a, -b} : cplx{b, -a};
 }
 
 // numeric_utils.f90:11-44: f = b1^2/(z -+ a1 - b2^2/(z -+ a2 - ...)),
@@ -1344,15 +1345,19 @@ __device__ cplx cfrac(cplx z, int n, const double *a, const double *b2, double s
     return cdiv(b2[0], {z.re + sgn * a[0] - f.re, z.im - f.im});
 }
 
-// g[iw] (+)= CF_p(E0 + i w; ap, bp) + CF_m(-E0 + i w; am, bm).  One thread
-// per frequency; the recursion is a serial chain of divisions per thread, so
-// the coefficients (and b^2) are staged in LDS first: read from global memory
-// level by level, every level would wait on an L2 round trip.
-__global__ __launch_bounds__(128) void k_cf(int nw, const double *omega, double e0, int np, const double *ap,
-                                            const double *bp, int nm, const double *am, const double *bm,
-                                            double *g, int accumulate)
+// g[iw] (+)= CF_p(E0 + i w; ap, bp) + CF_m(-E0 + i w; am, bm).  A workgroup
+// of 128 threads takes 64 frequencies: threads 0-63 evaluate CF_p and 64-127
+// CF_m of the same frequencies, and the pair adds in LDS.  The recursion is
+// a serial chain of divisions per thread, so the coefficients (and b^2) are
+// staged in LDS first: read from global memory level by level, every level
+// would wait on an L2 round trip.
+constexpr int kCfPer = 64;  // frequencies per workgroup
+__global__ __launch_bounds__(2 * kCfPer) void k_cf(int nw, const double *omega, double e0, int np, const double *ap,
+                                                   const double *bp, int nm, const double *am, const double *bm,
+                                                   double *g, int accumulate)
 {
     extern __shared__ double cfs[];  // ap[np] | bp^2[np] | am[nm] | bm^2[nm]
+    __shared__ cplx s_m[kCfPer];
     double *sap = cfs, *sbp = cfs + np, *sam = cfs + 2 * np, *sbm = cfs + 2 * np + nm;
     for (int i = threadIdx.x; i < np; i += blockDim.x) {
         sap[i] = ap[i];
@@ -1363,19 +1368,17 @@ __global__ __launch_bounds__(128) void k_cf(int nw, const double *omega, double 
         sbm[i] = bm[i] * bm[i];
     }
     __syncthreads();
-    const int iw = blockIdx.x * blockDim.x + threadIdx.x;
-    if (iw >= nw) return;
-    double re = 0.0, im = 0.0;
-    if (np > 0) {
-        const cplx f = cfrac({e0, omega[iw]}, np, sap, sbp, -1.0);
-        re += f.re;
-        im += f.im;
+    const int half = threadIdx.x / kCfPer, k = threadIdx.x % kCfPer;
+    const int iw = blockIdx.x * kCfPer + k;
+    cplx f{0.0, 0.0};
+    if (iw < nw) {
+        if (half == 0 && np > 0) f = cfrac({e0, omega[iw]}, np, sap, sbp, -1.0);
+        if (half == 1 && nm > 0) f = cfrac({-e0, omega[iw]}, nm, sam, sbm, 1.0);
     }
-    if (nm > 0) {
-        const cplx f = cfrac({-e0, omega[iw]}, nm, sam, sbm, 1.0);
-        re += f.re;
-        im += f.im;
-    }
+    if (half == 1) s_m[k] = f;
+    __syncthreads();
+    if (half == 1 || iw >= nw) return;
+    const double re = f.re + s_m[k].re, im = f.im + s_m[k].im;
     if (accumulate) {
         g[2 * iw] += re;
         g[2 * iw + 1] += im;
@@ -1433,7 +1436,7 @@ int launch_cf(hipStream_t s, int nw, const double *omega, double e0, int np, con
 {
     if (np < 0 || nm < 0 || np > 1000 || nm > 1000) return -1;  // chains are at most 1000 levels
     const size_t lds = sizeof(double) * 2 * ((size_t)np + nm);
-    hipLaunchKernelGGL(k_cf, dim3((nw + 127) / 128), dim3(128), lds, s, nw, omega, e0, np, ap, bp,
+    hipLaunchKernelGGL(k_cf, dim3((nw + kCfPer - 1) / kCfPer), dim3(2 * kCfPer), lds, s, nw, omega, e0, np, ap, bp,
                        nm, am, bm, g, accumulate);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
