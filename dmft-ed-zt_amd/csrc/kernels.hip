@@ -1091,7 +1091,10 @@ int launch_start(hipStream_t s, T *x, long long n, unsigned long long seed)
 template int launch_start<float>(hipStream_t, float *, long long, unsigned long long);
 template int launch_start<double>(hipStream_t, double *, long long, unsigned long long);
 
-// out = sum_j coef_j * basis_j (FP64 accumulation), plus |out|^2.
+// out = sum_j coef_j * basis_j (FP64 accumulation, in j order), plus
+// |out|^2.  Each thread issues the loads of eight vectors before their
+// FMAs: with one load per step (nvec = 200) a small sector's combine waited
+// on 200 memory round trips in a row.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_combine(T *out, const T *basis, long long stride,
                                                     int nvec, const double *coef, long long n,
@@ -1104,7 +1107,15 @@ __global__ __launch_bounds__(kBlock) void k_combine(T *out, const T *basis, long
     for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
          i += (long long)gridDim.x * kBlock) {
         double acc = 0.0;
-        for (int j = 0; j < nvec; ++j) acc += c[j] * (double)basis[(long long)j * stride + i];
+        int j = 0;
+        for (; j + 8 <= nvec; j += 8) {
+            T b[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) b[r] = basis[(long long)(j + r) * stride + i];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) acc += c[j + r] * (double)b[r];
+        }
+        for (; j < nvec; ++j) acc += c[j] * (double)basis[(long long)j * stride + i];
         const T t = (T)acc;
         out[i] = t;
         v[0] += (double)t * (double)t;
@@ -1118,7 +1129,10 @@ int launch_combine(hipStream_t s, T *out, const T *basis, long long stride, int 
                    double *norm_out, double *scale_out)
 {
     if (nvec > 1024) return -2;
-    const int grid = grid_for(n);
+    // one element per thread up to 4096 workgroups: every element is a sum
+    // over nvec vectors, so small sectors need the threads
+    const long long g = (n + kBlock - 1) / kBlock;
+    const int grid = (int)(g < 1 ? 1 : (g > 4096 ? 4096 : g));
     hipLaunchKernelGGL(k_combine<T>, dim3(grid), dim3(kBlock), 0, s, out, basis, stride,
                        nvec, coef, n, partial);
     if (hipGetLastError() != hipSuccess) return -1;
