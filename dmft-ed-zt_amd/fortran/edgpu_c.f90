@@ -125,6 +125,24 @@ module edgpu_c
             real(c_double), intent(out) :: occ(*), docc(*)
         end function
 
+        integer(c_int) function edg_self_energy(ctx, em, ek, vmk, nw, omega, G_cl, Sigma, Z) &
+                bind(C, name="edg_self_energy")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: em(*), ek(*), vmk(*), omega(*)
+            integer(c_int), value :: nw
+            complex(c_double_complex), intent(in) :: G_cl(*)
+            complex(c_double_complex), intent(out) :: Sigma(*)
+            real(c_double), intent(out) :: Z(*)
+        end function
+
+        integer(c_int) function edg_set_reorthogonalization(ctx, mode) &
+                bind(C, name="edg_set_reorthogonalization")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: mode
+        end function
+
         integer(c_int) function edg_apply_c(ctx, ne_up, ne_dn, vec, pm, iorb, ispin, vec_out) &
                 bind(C, name="edg_apply_c")
             import :: c_int, c_ptr, c_double

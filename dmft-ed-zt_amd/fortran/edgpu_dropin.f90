@@ -19,9 +19,22 @@ module edgpu_dropin
     implicit none
     private
     public :: edgpu_ground_state, edgpu_lanczos_iteration, edgpu_cluster_green_ftn, edgpu_hx, &
-              edgpu_finalize
+              edgpu_finalize, edgpu_observables, edgpu_self_energy
 
     type(c_ptr), save :: ctx = c_null_ptr
+
+    ! Options a driver may set before its first call; neither has a
+    ! counterpart in the reference.
+    ! edgpu_precision: storage of the Lanczos vectors (EDG_FP64, the
+    !   reference's, or EDG_FP32; dot products, alpha/beta, the tridiagonal
+    !   solve and the continued fraction stay FP64).
+    ! edgpu_gs_to_host: .true. (the reference's interface) fills gs%vec with
+    !   this rank's chunk of the ground state; .false. leaves the vector on
+    !   the device and gs%vec empty (size 0), for a driver that takes the
+    !   occupancies and the self-energy from edgpu_observables and
+    !   edgpu_self_energy instead of post_process.f90's loops over gs%vec.
+    integer(c_int), public, save :: edgpu_precision = EDG_FP64
+    logical, public, save :: edgpu_gs_to_host = .true.
 
 contains
 
@@ -48,7 +61,7 @@ contains
         p%mu = mu
         p%maxnstep = maxnstep
         p%diag_solver = diag_solver
-        p%precision = EDG_FP64
+        p%precision = edgpu_precision
         if (edg_create(p, 0_c_int, ctx) /= EDG_OK) call die("edgpu", "edg_create failed")
     end subroutine ensure_ctx
 
@@ -80,11 +93,15 @@ contains
         call check(edg_ground_state(ctx, em, ek, vmk, int(nsector, c_int), sec, isector, e0), &
                    "ground_state")
         call check(edg_gs_info(ctx, nu, nd, dim, er, e0, res, ns), "ground_state")
+        gs%isector = isector
+        gs%val = e0
+        if (.not. edgpu_gs_to_host) then
+            allocate(gs%vec(0))
+            return
+        endif
         allocate(full(dim))
         call check(edg_get_gs_vector(ctx, full, dim), "ground_state")
         call generate_basis(int(nu), int(nd), basis)
-        gs%isector = isector
-        gs%val = e0
         allocate(gs%vec(basis%nloc))
         gs%vec = full(basis%offsets(taskid)+1 : basis%offsets(taskid)+basis%nloc)
         call dealloc_basis(basis)
@@ -124,6 +141,7 @@ contains
     ! `even` shortcut skips its chains, :76-80,102-109).
     subroutine edgpu_cluster_green_ftn(em, ek, vmk, gs, G_cl)
         use mpi
+        use utils, only: die
         use io_units, only: IO_G_COEFFS, FN_G_COEFFS
         use dmft_params, only: norb, nbath, nspin, maxnstep, sectors
         use dmft_grid, only: nwloc, omega
@@ -142,6 +160,8 @@ contains
         rc = edg_gs_info(ctx, nu, nd, dim, er, e0, res, ns)
         if (rc /= EDG_OK .or. nu /= sectors(gs%isector,1) .or. nd /= sectors(gs%isector,2) .or. &
             e0 /= gs%val) then
+            if (size(gs%vec) == 0) call die("cluster_green_ftn", &
+                "gs has no vector (edgpu_gs_to_host = .false.) and is not the ground state held on the device")
             call generate_basis(sectors(gs%isector,1), sectors(gs%isector,2), basis)
             allocate(full(basis%ntot))
             call mpi_allgatherv(gs%vec, basis%nloc, mpi_double_precision, full, basis%nlocals, &
@@ -170,6 +190,31 @@ contains
         endif
         deallocate(coeffs)
     end subroutine edgpu_cluster_green_ftn
+
+    ! Occupancies occ(iorb,ispin) and double occupancies docc(iorb) of the
+    ! ground state held on the device: the loops of post_process.f90:63-112
+    ! over gs%vec, with the vector left on the device.
+    subroutine edgpu_observables(occ, docc)
+        use dmft_params, only: norb
+        double precision, intent(out) :: occ(norb,2), docc(norb)
+        call ensure_ctx()
+        call check(edg_observables(ctx, occ, docc), "observables")
+    end subroutine edgpu_observables
+
+    ! Self-energy Sigma(iw,iorb,ispin) = i w + mu - em - Delta - 1/G_cl and the
+    ! quasiparticle weight Z(iorb,ispin) of post_process.f90:40-58,121-185 on
+    ! the device (every orbital's Sigma kept).
+    subroutine edgpu_self_energy(em, ek, vmk, G_cl, Sigma, Z)
+        use dmft_params, only: norb, nbath, nspin
+        use dmft_grid, only: nwloc, omega
+        double precision, intent(in) :: em(norb,2), ek(nbath,2), vmk(norb,nbath,2)
+        double complex, intent(in) :: G_cl(nwloc,norb,nspin)
+        double complex, intent(out) :: Sigma(nwloc,norb,nspin)
+        double precision, intent(out) :: Z(norb,nspin)
+        call ensure_ctx()
+        call check(edg_self_energy(ctx, em, ek, vmk, int(nwloc, c_int), omega, G_cl, Sigma, Z), &
+                   "self_energy")
+    end subroutine edgpu_self_energy
 
     ! hx(em, ek, vmk, basis, x, y, x_all, py) of impurity_hamiltonian.f90:77-247
     subroutine edgpu_hx(em, ek, vmk, basis, x, y, x_all, py)

@@ -17,8 +17,9 @@ program dropin_driver
     implicit none
     double precision :: em(1,2), ek(5,2), vmk(1,5,2), ek2(5,2), vmk2(1,5,2), b_(200), a_(200)
     double precision, allocatable :: x(:), y(:), x_all(:)
-    double complex, allocatable :: G(:,:,:), G2(:,:,:)
-    type(eigpair_t) :: gs, gs2
+    double complex, allocatable :: G(:,:,:), G2(:,:,:), G3(:,:,:), Sig(:,:,:)
+    double precision :: occ(1,2), dd(1), Z(1,1)
+    type(eigpair_t) :: gs, gs2, gs3
     type(basis_t) :: basis
     double precision :: n_up, docc, w
     integer :: i, iup, idn, r, nstep
@@ -97,6 +98,22 @@ program dropin_driver
     write(*,'(a,es25.16)') 'E0B ', gs2%val
     call edgpu_cluster_green_ftn(em, ek, vmk, gs, G2)
     write(*,'(a,es25.16)') 'GDIFF ', maxval(abs(G2 - G))
+
+    ! the ground state left on the device (edgpu_gs_to_host = .false.):
+    ! gs%vec empty, occupancies and self-energy from the device
+    edgpu_gs_to_host = .false.
+    call edgpu_ground_state(em, ek, vmk, gs3)
+    write(*,'(a,i6,es25.16)') 'E0C ', size(gs3%vec), gs3%val
+    call edgpu_observables(occ, dd)
+    write(*,'(a,3es25.16)') 'OBS ', occ(1,1), occ(1,2), dd(1)
+    allocate(G3(nwloc,1,1), Sig(nwloc,1,1))
+    call edgpu_cluster_green_ftn(em, ek, vmk, gs3, G3)
+    write(*,'(a,es25.16)') 'GDIFFC ', maxval(abs(G3 - G))
+    call edgpu_self_energy(em, ek, vmk, G3, Sig, Z)
+    do i = 1, nwloc, 97
+        write(*,'(a,i6,2es25.16)') 'SIG ', i, dble(Sig(i,1,1)), aimag(Sig(i,1,1))
+    enddo
+    write(*,'(a,es25.16)') 'Z ', Z(1,1)
     call dealloc_basis(basis)
     call edgpu_finalize()
 end program dropin_driver

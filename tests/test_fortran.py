@@ -94,7 +94,10 @@ def test_fortran_dropin_vs_golden_and_oracle(tmp_path, oracle):
     Lanczos coefficients within 1e-10 of the oracle; g_coeffs.dat in the
     reference layout with the oracle's leading coefficients (1e-10); and a
     G computed from a gs argument that is not the ground state held on the
-    device (another bath solved in between) equal to the first (1e-12)."""
+    device (another bath solved in between) equal to the first (1e-12); with
+    edgpu_gs_to_host = .false. an empty gs%vec, the device's occupancies
+    (1e-10 of the fixture), G from the held vector equal to the first
+    (1e-12), and Sigma and Z against the host formula (1e-10)."""
     exe = os.path.join(DROPIN, "dropin_driver")
     if not os.path.exists(exe):
         pytest.fail("dropin_driver not built: run __graft_entry__.build()")
@@ -124,6 +127,26 @@ def test_fortran_dropin_vs_golden_and_oracle(tmp_path, oracle):
     for i, ai, bi in val("AB"):
         assert abs(ai - a[int(i) - 1]) < 1e-10 and abs(bi - bb[int(i) - 1]) < 1e-10
     assert val("GDIFF")[0][0] < 1e-12
+    # edgpu_gs_to_host = .false.: gs%vec empty, the vector stays on the
+    # device; occupancies and Sigma/Z from the device (post_process.f90:
+    # 40-58,63-112,121-185) against the fixture and the host formula on the
+    # driver's own G
+    size, e0c = val("E0C")[0]
+    assert int(size) == 0 and abs(e0c - float(g["E0"])) < 1e-10
+    n_up, n_dn, dd = val("OBS")[0]
+    assert abs(n_up - float(g["n_up"])) < 1e-10 and abs(n_dn - float(g["n_up"])) < 1e-10
+    assert abs(dd - float(g["docc"])) < 1e-10
+    assert val("GDIFFC")[0][0] < 1e-12
+    om = np.pi * (2 * np.arange(2048) + 1) / 100.0
+    Gd = {int(i): complex(re, im) for i, re, im in val("G")}
+    ek1, v1 = g["ek"][:, 0], g["vmk"][0, :, 0]
+    for i, re, im in val("SIG"):
+        w = 1j * om[int(i) - 1]
+        hyb = np.sum(v1 ** 2 / (w - ek1))
+        want = w + float(g["mu"]) - float(g["em"][0, 0]) - hyb - 1.0 / Gd[int(i)]
+        assert abs(complex(re, im) - want) < 1e-10 * max(1.0, abs(want)), i
+    s1 = complex(*val("SIG")[0][1:])
+    assert abs(val("Z")[0][0] - 1.0 / (1.0 - s1.imag / om[0])) < 1e-10
     # g_coeffs.dat (dmft_green.f90:66-69,89-109)
     lines = (tmp_path / "g_coeffs.dat").read_text().split("\n")
     assert [int(t) for t in lines[0].split()] == [200, 1]
