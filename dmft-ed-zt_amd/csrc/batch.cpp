@@ -23,9 +23,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -149,6 +151,20 @@ struct Arena {
     }
 };
 
+// Pinned host memory is taken in slabs and cut into the problems' areas;
+// a destroyed queue leaves its slabs to later queues of the process
+// (pinning pages costs far more than the few transfers of a small problem).
+struct PinSlab {
+    int device;
+    void *p;
+    size_t bytes;
+};
+constexpr size_t kPinSlab = 16u << 20;
+constexpr size_t kPinCacheMax = 1ull << 30;
+std::mutex g_pin_mu;
+std::vector<PinSlab> g_pin_cache;
+size_t g_pin_cached = 0;
+
 struct Job {
     long long id = 0;
     int lane = 0;  // its side stream (stage work) and that stream's partial-sum scratch
@@ -170,7 +186,6 @@ struct Job {
     Chain gsc;
     Chain ch[kMaxChains];       // chain 4 (iorb - 1) + k: (up,+), (up,-), (dn,+), (dn,-)
     double *host = nullptr;     // pinned readback / upload area
-    size_t host_n = 0;
     hipEvent_t ev = nullptr;
     // recorded on the side stream after the stage work that started chains;
     // the chains join the batched steps once the host sees it complete
@@ -239,10 +254,16 @@ struct edg_batch {
     // stream has passed the event recorded after their last launch
     std::vector<std::pair<hipGraphExec_t, hipEvent_t>> retired;
     int since_rebuild = 0;  // chunks launched since the chain set last changed
-    std::vector<double *> pinned_free;
+    std::vector<double *> pinned_free;  // problem areas of pinned_n doubles, cut from slabs
+    std::vector<PinSlab> slabs;
     size_t pinned_n = 0;
+    std::vector<hipEvent_t> ev_free;  // events of finished problems, for the next ones
     double st_steps = 0, st_chain_steps = 0, st_done = 0, st_graph = 0, st_rebuild = 0;
     long long st_inst = 0, st_exec_destroyed = 0;  // graph instantiations / execs destroyed
+    // host seconds inside edg_batch_step: admission (tables, start vectors),
+    // stage transitions (progress: read-backs, eigenvectors, combine, apply_c,
+    // continued fractions), queueing the steps, and blocked on events
+    double st_t_admit = 0, st_t_progress = 0, st_t_steps = 0, st_t_wait = 0;
     // H.v tile schedule by size class: the first entry with nsite <= nsite_max
     // applies to every chain of a problem
     struct Sched {
@@ -285,17 +306,23 @@ size_t tab_bytes(int n, int norb, int nbath)
            n + 16 * (size_t)n * (2 + 1 + 4) + orb + 16 * 256;
 }
 
-// Pinned host area of a problem (doubles): the misc sums, the raw arrays of
-// each chain read back, the coefficient staging and G; the last 16 doubles
-// hold the flag pairs uploaded to the chains.
-size_t host_chain(int k, int mg) { return 16 + (size_t)k * (3 * mg + 4); }
+// A chain's device state block: step index (int) at byte 0, flags (two
+// ints: stop, number of steps) at byte 8, then alpha[maxn], beta[maxn + 1]
+// and scale[maxn + 1] from byte 256 (kHead doubles).  A read-back copies the
+// block's first chain_host_n(maxn) doubles in one transfer.
+constexpr int kHead = 32;
+size_t chain_host_n(int m) { return kHead + 3 * (size_t)m + 2; }
+
+// Pinned host area of a problem (doubles): the misc sums, the state blocks
+// of the chains read back, the coefficient staging and G.
+size_t host_chain(int k, int mg) { return 16 + (size_t)k * chain_host_n(mg); }
 size_t host_coef(int mg) { return host_chain(kMaxChains, mg); }
 size_t host_g(int mg) { return host_coef(mg) + 16 * (size_t)mg; }  // (ap bp am bm) x 2 spins x 2 orbitals
 constexpr int kMaxLen = 1000;  // longest Lanczos run of a problem
 
 size_t state_bytes(int maxn, int hvn, int axn)
 {
-    return 256 * 8 + ((size_t)maxn + 1) * 3 * 8 + 16 + (size_t)(hvn + axn) * 8;
+    return 8 * kHead + ((size_t)maxn + 1) * 3 * 8 + 16 + (size_t)(hvn + axn) * 8;
 }
 
 // Tile shape of a chain with columns of nup rows under the engine's
@@ -413,11 +440,30 @@ double *pinned_get(edg_batch *b)
         b->pinned_free.pop_back();
         return p;
     }
-    double *p = nullptr;
-    if (hipHostMalloc((void **)&p, b->pinned_n * sizeof(double), hipHostMallocDefault) != hipSuccess) {
-        (void)hipGetLastError();
-        return nullptr;
+    const size_t per = (b->pinned_n * sizeof(double) + 255) & ~size_t(255);
+    PinSlab sl{b->device, nullptr, 0};
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        for (size_t i = 0; i < g_pin_cache.size(); ++i)
+            if (g_pin_cache[i].device == b->device && g_pin_cache[i].bytes >= per) {
+                sl = g_pin_cache[i];
+                g_pin_cache.erase(g_pin_cache.begin() + i);
+                g_pin_cached -= sl.bytes;
+                break;
+            }
     }
+    if (!sl.p) {
+        sl.bytes = std::max(kPinSlab, per);
+        if (hipHostMalloc(&sl.p, sl.bytes, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    b->slabs.push_back(sl);
+    for (size_t off = sl.bytes / per * per; off >= per; off -= per)
+        b->pinned_free.push_back((double *)((char *)sl.p + off - per));
+    double *p = b->pinned_free.back();
+    b->pinned_free.pop_back();
     return p;
 }
 
@@ -439,7 +485,6 @@ int build_table(edg_batch *b, Job *J, int spin, int ne, SpinTab &t, int kc, int 
     t.hf = (HopF *)J->ar.take(K * n * sizeof(HopF));
     t.hd = (HopD *)J->ar.take(K * n * sizeof(HopD));
     t.cnt = (uint8_t *)J->ar.take(n);
-    t.maxcnt = (int *)J->ar.take(4);
     t.energy = (double *)J->ar.take(8 * n);
     t.energyf = (float *)J->ar.take(4 * n);
     t.mask = (uint8_t *)J->ar.take(n);
@@ -454,11 +499,11 @@ int build_table(edg_batch *b, Job *J, int spin, int ne, SpinTab &t, int kc, int 
     const double *ek_s = J->dparams + 2 * norb + (size_t)(spin - 1) * nbath;
     const double *v_s = J->dparams + 2 * norb + 2 * (size_t)nbath + (size_t)(spin - 1) * norb * nbath;
     t.vsrc = v_s;
-    BCHK(b, hipMemsetAsync(t.maxcnt, 0, sizeof(int), b->side[J->lane]));
+    // the hop counts are known on the host (km), so no device maximum; the
+    // compressed records are written by the same launch
     const double UpmJ = norb >= 2 ? kan_up(J->p) - J->p.J : 0.0;
-    if (launch_spin_table(b->side[J->lane], ns, norb, nbath, ne, em_s, ek_s, v_s, J->p.mu, UpmJ, t))
+    if (launch_spin_table(b->side[J->lane], ns, norb, nbath, ne, em_s, ek_s, v_s, J->p.mu, UpmJ, t, true))
         return fail(b, EDG_ERR_HIP, "spin table launch failed");
-    if (launch_compress(b->side[J->lane], t)) return fail(b, EDG_ERR_HIP, "compress launch failed");
     return 0;
 }
 
@@ -504,29 +549,25 @@ int init_chain(edg_batch *b, Job *J, Chain &C, const HvOp &op, int maxn, bool st
     char *st = (char *)J->ar.take(state_bytes(maxn, I.hvn, I.axn));
     I.step = (int *)st;
     I.flags = (int *)(st + 8);
-    I.alpha = (double *)(st + 256);
+    I.alpha = (double *)st + kHead;
     I.beta = I.alpha + maxn;
     I.scale = I.beta + maxn + 1;
     I.hvpart = I.scale + maxn + 1;
     I.axpart = I.hvpart + I.hvn;
     if (J->ar.overflow) return fail(b, EDG_ERR_NOMEM, "problem arena overflow (chain)");
-    BCHK(b, hipMemsetAsync(st, 0, state_bytes(maxn, I.hvn, I.axn), b->side[J->lane]));
+    // zeroed, with flags = {0, maxn}
+    if (launch_batch_state_init(b->side[J->lane], st, state_bytes(maxn, I.hvn, I.axn), maxn))
+        return fail(b, EDG_ERR_HIP, "state init launch failed");
     C.D = D;
     C.exists = true;
     C.launched = 0;
     return 0;
 }
 
-// flags = {0, maxn}, scale[0] = 1/|slot 0| (norm into norm_out).  Chain
-// number slot (0 = ground state, 1..8 = c+/c) owns its own pinned flag pair,
-// so no upload waits for an earlier one.
-int start_chain(edg_batch *b, Job *J, Chain &C, double *norm_out, int slot)
+// scale[0] = 1/|slot 0| (norm into norm_out); init_chain set the flags.
+int start_chain(edg_batch *b, Job *J, Chain &C, double *norm_out)
 {
     BInst &I = C.bi;
-    int *h = (int *)(J->host + J->host_n - 16) + 2 * slot;
-    h[0] = 0;
-    h[1] = I.maxn;
-    BCHK(b, hipMemcpyAsync(I.flags, h, 2 * sizeof(int), hipMemcpyHostToDevice, b->side[J->lane]));
     int rc = b->fp64 ? launch_norm<double>(b->side[J->lane], (const double *)I.base, C.D, b->side_part[J->lane],
                                            norm_out, I.scale, nullptr)
                      : launch_norm<float>(b->side[J->lane], (const float *)I.base, C.D, b->side_part[J->lane],
@@ -550,9 +591,14 @@ int admit(edg_batch *b, Job *J)
     b->in_use += J->bytes;
     J->host = pinned_get(b);
     if (!J->host) return fail(b, EDG_ERR_NOMEM, "pinned host allocation failed");
-    J->host_n = b->pinned_n;
-    BCHK(b, hipEventCreateWithFlags(&J->ev, hipEventDisableTiming));
-    BCHK(b, hipEventCreateWithFlags(&J->start_ev, hipEventDisableTiming));
+    for (hipEvent_t *e : {&J->ev, &J->start_ev}) {
+        if (!b->ev_free.empty()) {
+            *e = b->ev_free.back();
+            b->ev_free.pop_back();
+        } else {
+            BCHK(b, hipEventCreateWithFlags(e, hipEventDisableTiming));
+        }
+    }
     const int nbath = p.nbath, ns = nsite_of(p), no = J->norb;
     const size_t npar = 2 * (size_t)no + 2 * (size_t)nbath + 2 * (size_t)no * nbath;
     J->dparams = (double *)J->ar.take(8 * npar);
@@ -597,7 +643,7 @@ int admit(edg_batch *b, Job *J)
     const int rc0 = b->fp64 ? launch_start<double>(b->side[J->lane], (double *)J->gsc.bi.base, D, p.seed)
                             : launch_start<float>(b->side[J->lane], (float *)J->gsc.bi.base, D, p.seed);
     if (rc0) return fail(b, EDG_ERR_HIP, "start launch failed");
-    if (int rc = start_chain(b, J, J->gsc, &J->misc[0], 0)) return rc;
+    if (int rc = start_chain(b, J, J->gsc, &J->misc[0])) return rc;
     BCHK(b, hipEventRecord(J->start_ev, b->side[J->lane]));
     J->start_pending = true;
     J->st = ST_GS;
@@ -605,26 +651,21 @@ int admit(edg_batch *b, Job *J)
     return 0;
 }
 
-// Queue the device->host copy of a finished chain's alpha / beta / scale /
-// flags into the pinned area at off; returns the doubles used.
-size_t read_chain(edg_batch *b, Job *J, const Chain &C, size_t off)
+// Queue the device->host copy of a finished chain's state block (flags,
+// alpha, beta, scale) into the pinned area at off.
+int read_chain(edg_batch *b, Job *J, const Chain &C, size_t off)
 {
-    const int m = C.bi.maxn;
-    double *h = J->host + off;
-    (void)hipMemcpyAsync(h, C.bi.alpha, sizeof(double) * m, hipMemcpyDeviceToHost, b->stream);
-    (void)hipMemcpyAsync(h + m, C.bi.beta, sizeof(double) * (m + 1), hipMemcpyDeviceToHost, b->stream);
-    (void)hipMemcpyAsync(h + 2 * m + 1, C.bi.scale, sizeof(double) * (m + 1), hipMemcpyDeviceToHost,
-                         b->stream);
-    (void)hipMemcpyAsync(h + 3 * m + 2, C.bi.flags, sizeof(int) * 2, hipMemcpyDeviceToHost, b->stream);
-    return 3 * (size_t)m + 4;
+    BCHK(b, hipMemcpyAsync(J->host + off, C.bi.step, sizeof(double) * chain_host_n(C.bi.maxn),
+                           hipMemcpyDeviceToHost, b->stream));
+    return 0;
 }
 
-// a, b of run_lanczos from the raw arrays at h (maxn m).
+// a, b of run_lanczos from a state block read back to h (maxn m).
 int chain_ab(const double *h, int m, double *a, double *bb)
 {
-    const int *fl = (const int *)(h + 3 * m + 2);
+    const int *fl = (const int *)(h + 1);
     const int ns = fl[1];
-    const double *ha = h, *hb = h + m;
+    const double *ha = h + kHead, *hb = ha + m;
     for (int k = 0; k < m; ++k) {
         a[k] = k < ns ? ha[k] : 0.0;
         bb[k] = (k >= 1 && k <= ns) ? hb[k] : 0.0;
@@ -641,14 +682,14 @@ int after_gs(edg_batch *b, Job *J)
     const int m = p.maxnstep;
     std::vector<double> a(m), bb(m);
     const int nst = chain_ab(J->host, m, a.data(), bb.data());
-    const double *s = J->host + 2 * m + 1;
+    const double *s = J->host + kHead + 2 * m + 1;
     J->gs_nstep = nst;
     if (nst < 1) return fail(b, EDG_ERR_SOLVER, "ground-state chain broke down at its start");
     std::vector<double> z(nst);
     double lam;
     if (tridiag_lowest(nst, a.data(), bb.data() + 1, &lam, z.data()))
         return fail(b, EDG_ERR_SOLVER, "tridiagonal eigensolve failed");
-    double *hc = J->host + 4 * (size_t)m + 8;
+    double *hc = J->host + chain_host_n(m) + 8;
     for (int j = 0; j < nst; ++j) hc[j] = z[j] * s[j];
     BCHK(b, hipMemcpyAsync(J->coef, hc, sizeof(double) * nst, hipMemcpyHostToDevice, b->side[J->lane]));
     const HvOp op = J->gsc.bi.op;
@@ -700,7 +741,7 @@ int after_gs(edg_batch *b, Job *J)
                                 : launch_apply_c<float>(b->side[J->lane], (const float *)J->gs, (float *)C.bi.base,
                                                         cop.nup, cop.ndn, op.nup, C.ispin, J->map[k], godd);
         if (arc) return fail(b, EDG_ERR_HIP, "apply_c launch failed");
-        if (int r = start_chain(b, J, C, &J->misc[8 + k], 1 + k)) return r;
+        if (int r = start_chain(b, J, C, &J->misc[8 + k])) return r;
     }
     // also covers the RQ sums a ground-state-only problem reads next
     BCHK(b, hipEventRecord(J->start_ev, b->side[J->lane]));
@@ -751,13 +792,14 @@ int after_chains(edg_batch *b, Job *J)
         for (int s = 0; s < 2; ++s)
             std::memcpy(J->coeffs.data() + ((size_t)o * 2 + s) * 4 * mg, co[o][s].data(), sizeof(double) * 4 * mg);
     double *hc = J->host + host_coef(mg);
+    std::memcpy(hc, J->coeffs.data(), sizeof(double) * J->coeffs.size());
+    BCHK(b, hipMemcpyAsync(J->coef, hc, sizeof(double) * J->coeffs.size(), hipMemcpyHostToDevice,
+                           b->side[J->lane]));
     const int nspin_run = spin2 ? 2 : 1;
     for (int o = 0; o < no; ++o)
         for (int s = 0; s < nspin_run; ++s) {
             const size_t q = ((size_t)o * 2 + s) * 4 * mg;
-            std::memcpy(hc + q, co[o][s].data(), sizeof(double) * 4 * mg);
             const double *dc = J->coef + q;
-            BCHK(b, hipMemcpyAsync((void *)dc, hc + q, sizeof(double) * 4 * mg, hipMemcpyHostToDevice, b->side[J->lane]));
             if (launch_cf(b->side[J->lane], b->nw, b->omega, J->e0, nsv[o][s][0], dc, dc + mg, nsv[o][s][1], dc + 2 * mg,
                           dc + 3 * mg, J->gdev + ((size_t)o * 2 + s) * 2 * b->nw, 0))
                 return fail(b, EDG_ERR_HIP, "continued fraction launch failed");
@@ -806,9 +848,9 @@ int finish(edg_batch *b, Job *J)
     b->in_use -= J->bytes;
     b->pinned_free.push_back(J->host);
     J->host = nullptr;
-    (void)hipEventDestroy(J->ev);
+    b->ev_free.push_back(J->ev);
     J->ev = nullptr;
-    (void)hipEventDestroy(J->start_ev);
+    b->ev_free.push_back(J->start_ev);
     J->start_ev = nullptr;
     J->st = ST_DONE;
     b->st_done += 1;
@@ -842,7 +884,7 @@ int progress(edg_batch *b)
                 }
         }
         if (J->st == ST_GS && !J->gsc.active) {
-            read_chain(b, J, J->gsc, 0);
+            if (int rc = read_chain(b, J, J->gsc, 0)) return rc;
             BCHK(b, hipEventRecord(J->ev, b->stream));
             J->st = ST_GS_READ;
         } else if (J->st == ST_GS_READ && hipEventQuery(J->ev) == hipSuccess) {
@@ -852,7 +894,8 @@ int progress(edg_batch *b)
             BCHK(b, hipMemcpyAsync(J->host, J->misc, 16 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
             const int mg = green_len(J->p);
             for (int k = 0; k < kMaxChains; ++k)
-                if (J->ch[k].exists) read_chain(b, J, J->ch[k], host_chain(k, mg));
+                if (J->ch[k].exists)
+                    if (int rc = read_chain(b, J, J->ch[k], host_chain(k, mg))) return rc;
             BCHK(b, hipEventRecord(J->ev, b->stream));
             J->st = ST_CH_READ;
         } else if (J->st == ST_CH_READ && hipEventQuery(J->ev) == hipSuccess) {
@@ -1186,7 +1229,7 @@ int edg_batch_create(int device, int precision, int nspin, int nw, const double 
     b->es = b->fp64 ? 8 : 4;
     b->nspin = nspin;
     b->nw = nw;
-    b->pinned_n = host_g(kMaxLen) + 8 * (size_t)nw + 16;
+    b->pinned_n = host_g(kMaxLen) + 8 * (size_t)nw;
     int rc = 0;
     auto chk = [&](hipError_t e) {
         if (e != hipSuccess && !rc) rc = EDG_ERR_HIP;
@@ -1228,7 +1271,6 @@ int edg_batch_destroy(edg_batch *b)
         if (s) (void)hipStreamSynchronize(s);
     auto drop = [&](Job *J) {
         if (J->ar.base && b->heap) b->heap->free(J->ar.base);
-        if (J->host) (void)hipHostFree(J->host);
         if (J->ev) (void)hipEventDestroy(J->ev);
         if (J->start_ev) (void)hipEventDestroy(J->start_ev);
         delete J;
@@ -1236,7 +1278,16 @@ int edg_batch_destroy(edg_batch *b)
     for (Job *J : b->queue) drop(J);
     for (Job *J : b->running) drop(J);
     for (Job *J : b->finished) drop(J);
-    for (double *p : b->pinned_free) (void)hipHostFree(p);
+    for (const PinSlab &sl : b->slabs) {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        if (g_pin_cached + sl.bytes <= kPinCacheMax) {
+            g_pin_cache.push_back(sl);
+            g_pin_cached += sl.bytes;
+        } else {
+            (void)hipHostFree(sl.p);
+        }
+    }
+    for (hipEvent_t e : b->ev_free) (void)hipEventDestroy(e);
     (void)drop_graphs(b);
     if (b->ddesc) (void)hipFree(b->ddesc);
     delete b->heap;  // its chunks go to the device cache for later queues
@@ -1260,6 +1311,17 @@ int edg_batch_release_memory(int device)
 {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return EDG_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        for (size_t i = 0; i < g_pin_cache.size();)
+            if (g_pin_cache[i].device == device) {
+                (void)hipHostFree(g_pin_cache[i].p);
+                g_pin_cached -= g_pin_cache[i].bytes;
+                g_pin_cache.erase(g_pin_cache.begin() + i);
+            } else {
+                ++i;
+            }
+    }
     return devmem_release_cache(device) ? EDG_ERR_HIP : EDG_OK;
 }
 
@@ -1329,7 +1391,12 @@ int edg_batch_step(edg_batch *b, int max_steps, int *pending)
 {
     if (!b) return EDG_ERR_ARG;
     (void)hipSetDevice(b->device);
+    using clk = std::chrono::steady_clock;
+    auto secs = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
+    auto t = clk::now();
     if (int rc = progress(b)) return rc;
+    b->st_t_progress += secs(t);
+    t = clk::now();
     // admission in submission order by device bytes; a large problem waits
     // while kMaxLarge others run, and the problems behind it go ahead when
     // they fit
@@ -1352,6 +1419,7 @@ int edg_batch_step(edg_batch *b, int max_steps, int *pending)
         large_in_flight += J->large ? 1 : 0;
         it = b->queue.erase(it);
     }
+    b->st_t_admit += secs(t);
     if ((!b->active.empty() || !b->direct.empty()) && max_steps > 0) {
         int left = max_steps;
         while (left > 0 && (!b->active.empty() || !b->direct.empty())) {
@@ -1359,21 +1427,28 @@ int edg_batch_step(edg_batch *b, int max_steps, int *pending)
             for (const Chain *C : b->active) k = std::min(k, C->bi.maxn - C->launched);
             for (const Chain *C : b->direct) k = std::min(k, C->bi.maxn - C->launched);
             b->moved = false;
+            t = clk::now();
             if (int rc = run_steps(b, k)) return rc;
             left -= k;
             // bounded run-ahead: wait for the chunk kAhead chunks back
             BCHK(b, hipEventRecord(b->chunk_ev[b->chunk_head], b->stream));
             b->chunk_head = (b->chunk_head + 1) % kChunkEvents;
+            b->st_t_steps += secs(t);
             if (++b->chunk_n > kAhead) {
                 const int old = (b->chunk_head + kChunkEvents - b->chunk_n) % kChunkEvents;
+                t = clk::now();
                 BCHK(b, hipEventSynchronize(b->chunk_ev[old]));
+                b->st_t_wait += secs(t);
                 --b->chunk_n;
             }
+            t = clk::now();
             if (int rc = progress(b)) return rc;
+            b->st_t_progress += secs(t);
             if (b->dirty || b->moved) break;  // chains joined or left: let the caller see progress
         }
     } else if (!b->running.empty()) {
         // nothing to step: wait for the oldest outstanding event, then move on
+        t = clk::now();
         for (Job *J : b->running) {
             if (J->start_pending) {
                 BCHK(b, hipEventSynchronize(J->start_ev));
@@ -1384,7 +1459,10 @@ int edg_batch_step(edg_batch *b, int max_steps, int *pending)
                 break;
             }
         }
+        b->st_t_wait += secs(t);
+        t = clk::now();
         if (int rc = progress(b)) return rc;
+        b->st_t_progress += secs(t);
     }
     if (pending) *pending = (int)(b->queue.size() + b->running.size());
     return EDG_OK;
@@ -1425,6 +1503,10 @@ int edg_batch_stats(const edg_batch *b, double *stats)
     stats[5] = (double)b->running.size();
     stats[6] = b->st_graph;
     stats[7] = b->st_rebuild;
+    stats[8] = b->st_t_admit;
+    stats[9] = b->st_t_progress;
+    stats[10] = b->st_t_steps;
+    stats[11] = b->st_t_wait;
     return EDG_OK;
 }
 

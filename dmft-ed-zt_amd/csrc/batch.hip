@@ -355,6 +355,22 @@ int launch_batch_axpy(hipStream_t s, bool fp64, const BInst *inst, const int *ax
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// A chain's state block zeroed, with its flags (the two ints at byte 8)
+// set to {0, maxn}.
+__global__ void k_state_init(unsigned long long *st, long long n8, int maxn)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n8; i += (long long)gridDim.x * blockDim.x)
+        st[i] = i == 1 ? (unsigned long long)(unsigned)maxn << 32 : 0ull;
+}
+
+int launch_batch_state_init(hipStream_t s, void *st, size_t bytes, int maxn)
+{
+    const long long n8 = (long long)(bytes / 8);
+    const int grid = (int)std::min<long long>((n8 + 255) / 256, 64);
+    hipLaunchKernelGGL(k_state_init, dim3(grid), dim3(256), 0, s, (unsigned long long *)st, n8, maxn);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_batch_advance(hipStream_t s, const BInst *inst, int ninst, int k)
 {
     if (ninst <= 0) return 0;

@@ -39,7 +39,7 @@ struct SpinTab {
     float *energyf = nullptr;      // [n] FP32 copy for FP32 row sums
     uint8_t *mask = nullptr;       // [n] occupied impurity orbitals
     uint32_t *orb = nullptr;       // [norb*norb][n] orbital moves dst<-src
-    int *maxcnt = nullptr;         // device scalar: max over cnt
+    int *maxcnt = nullptr;         // device scalar: max over cnt (null: not computed)
     // compressed hop tables of the single-orbital kernels, [n][KC]:
     // ccol = target rank, cidx = index into vsrc (bit 7: odd Jordan-Wigner
     // parity, the element is -vsrc[idx]), 64 = zero; hrec = target rank |
@@ -184,7 +184,7 @@ constexpr int kAxpyMaxGrid = 256;
 // ---- launchers (kernels.hip) ----
 int launch_spin_table(hipStream_t s, int nsite, int norb, int nbath, int ne, const double *em_s,
                       const double *ek_s, const double *v_s, double mu, double UpmJ,
-                      const SpinTab &t);
+                      const SpinTab &t, bool compress = false);
 int launch_compress(hipStream_t s, const SpinTab &t);
 // Returns the number of workgroups (= partial sums written) or < 0 on error.
 template <typename T>
@@ -276,6 +276,7 @@ constexpr int kBatchKanColsF32 = 2;
 int launch_batch_axpy(hipStream_t s, bool fp64, const BInst *inst, const int *ax_off, const int *own,
                       int nblocks, int sub);
 int launch_batch_advance(hipStream_t s, const BInst *inst, int ninst, int k);
+int launch_batch_state_init(hipStream_t s, void *st, size_t bytes, int maxn);
 // Allow max_lds bytes of dynamic LDS for the batched H.v (call once, outside
 // any graph capture).
 int batch_set_lds(size_t max_lds);

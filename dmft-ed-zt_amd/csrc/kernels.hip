@@ -171,13 +171,29 @@ __device__ __forceinline__ uint32_t binom_tab(const uint32_t (*B)[16], int n, in
 
 // ------------------------------------------------------ per-spin tables
 
+// Compressed [n][KC] copies of row r's first KC slots: 16-bit target rank
+// and 8-bit element index (bit 7 = odd parity, the element being minus the
+// signed V at that index; 64 = zero element); and the same pair as one
+// 32-bit record, rank | (value-table index << 24).  Reads the row's full
+// records, so it runs after them in the same thread or a later launch.
+__device__ inline void compress_row(const SpinTab &t, int r)
+{
+    for (int k = 0; k < t.KC; ++k) {
+        const HopD h = t.hd[(long long)r * t.K + k];
+        t.ccol[(long long)r * t.KC + k] = (uint16_t)h.col;
+        t.cidx[(long long)r * t.KC + k] = (uint8_t)h.pad;
+        const uint32_t vi = (h.pad & 0x7Fu) + ((h.pad >> 7) & 1u ? (uint32_t)kMaxHops + 1u : 0u);
+        t.hrec[(long long)r * t.KC + k] = (h.col & 0xFFFFu) | (vi << 24);
+    }
+}
+
 // One thread per configuration rank r of N electrons on nsite sites.
 // Rank order = increasing bit-pattern order (ed_basis.f90:94-112), i.e. the
 // combinatorial number system r = sum_k C(p_k, k) over set bits p_1<..<p_N.
 __global__ __launch_bounds__(kBlock) void k_spin_table(int nsite, int norb, int nbath, int ne,
                                                        const double *em_s, const double *ek_s,
                                                        const double *v_s, double mu, double UpmJ,
-                                                       SpinTab t)
+                                                       SpinTab t, int compress)
 {
     __shared__ uint32_t B[16][16];
     for (int e = threadIdx.x; e < 256; e += kBlock) {
@@ -234,7 +250,8 @@ __global__ __launch_bounds__(kBlock) void k_spin_table(int nsite, int norb, int 
         t.hd[(long long)r * t.K + k] = HopD{(uint32_t)r, 64u, 0.0};
         t.hf[(long long)r * t.K + k] = HopF{(uint32_t)r, 0.0f};
     }
-    atomicMax(t.maxcnt, slot);
+    if (t.maxcnt) atomicMax(t.maxcnt, slot);
+    if (compress) compress_row(t, r);
     // one-spin diagonal energy
     double e = 0.0;
     for (int m = 0; m < norb; ++m)
@@ -263,21 +280,13 @@ __global__ __launch_bounds__(kBlock) void k_spin_table(int nsite, int norb, int 
     }
 }
 
-// Compressed [n][KC] copies of the first KC slots: 16-bit target rank and
-// 8-bit element index (bit 7 = odd parity, the element being minus the
-// signed V at that index; 64 = zero element); and the same pair as one
-// 32-bit record, rank | (value-table index << 24).
+// Compressed tables of the sectors whose hop targets fit 16 bits (k_compress,
+// or k_spin_table with compress set).
 __global__ __launch_bounds__(kBlock) void k_compress(SpinTab t)
 {
     const int r = blockIdx.x * kBlock + threadIdx.x;
     if (r >= t.n) return;
-    for (int k = 0; k < t.KC; ++k) {
-        const HopD h = t.hd[(long long)r * t.K + k];
-        t.ccol[(long long)r * t.KC + k] = (uint16_t)h.col;
-        t.cidx[(long long)r * t.KC + k] = (uint8_t)h.pad;
-        const uint32_t vi = (h.pad & 0x7Fu) + ((h.pad >> 7) & 1u ? (uint32_t)kMaxHops + 1u : 0u);
-        t.hrec[(long long)r * t.KC + k] = (h.col & 0xFFFFu) | (vi << 24);
-    }
+    compress_row(t, r);
 }
 
 int launch_compress(hipStream_t s, const SpinTab &t)
@@ -288,11 +297,11 @@ int launch_compress(hipStream_t s, const SpinTab &t)
 
 int launch_spin_table(hipStream_t s, int nsite, int norb, int nbath, int ne, const double *em_s,
                       const double *ek_s, const double *v_s, double mu, double UpmJ,
-                      const SpinTab &t)
+                      const SpinTab &t, bool compress)
 {
     const int grid = (t.n + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_spin_table, dim3(grid), dim3(kBlock), 0, s, nsite, norb, nbath, ne, em_s,
-                       ek_s, v_s, mu, UpmJ, t);
+                       ek_s, v_s, mu, UpmJ, t, compress ? 1 : 0);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -2,7 +2,7 @@
 // MKL dstevx('V','A') call of lanczos_solver.f90:175-211 (the reference then
 // keeps only ev(1) and z(:,1), lanczos_solver.f90:67-68).
 //
-// Method: Sturm-sequence bisection for the smallest eigenvalue to full
+// Method: Sturm-sequence multisection for the smallest eigenvalue to full
 // precision, then inverse iteration with a partially pivoted tridiagonal LU
 // for its eigenvector.  Copies of the same Ritz value that plain Lanczos can
 // produce (loss of orthogonality) form a tight cluster; inverse iteration
@@ -14,19 +14,27 @@
 
 namespace edg {
 
-// Number of eigenvalues strictly below x.
-static int sturm_count(int n, const double *d, const double *e2, double x)
+// Number of eigenvalues strictly below each of the kPts points x[k].  The
+// pivot recurrence carries one division per element; the points' recurrences
+// are independent, so they run interleaved (and in vector registers), at
+// about the cost of one.
+constexpr int kPts = 4;
+static void sturm_counts(int n, const double *d, const double *e2, const double *x, int *cnt)
 {
     const double tiny = 1e-300;
-    int cnt = 0;
-    double q = d[0] - x;
-    if (q < 0) ++cnt;
-    for (int i = 1; i < n; ++i) {
-        if (std::fabs(q) < tiny) q = -tiny;
-        q = d[i] - x - e2[i - 1] / q;
-        if (q < 0) ++cnt;
+    double q[kPts];
+    int c[kPts];
+    for (int k = 0; k < kPts; ++k) {
+        q[k] = d[0] - x[k];
+        c[k] = q[k] < 0;
     }
-    return cnt;
+    for (int i = 1; i < n; ++i)
+        for (int k = 0; k < kPts; ++k) {
+            const double qk = std::fabs(q[k]) < tiny ? -tiny : q[k];
+            q[k] = d[i] - x[k] - e2[i - 1] / qk;
+            c[k] += q[k] < 0;
+        }
+    for (int k = 0; k < kPts; ++k) cnt[k] = c[k];
 }
 
 // d[n], e[n-1].  Returns 0 on success, fills *lambda and z[n] (unit norm,
@@ -48,11 +56,18 @@ int tridiag_lowest(int n, const double *d, const double *e, double *lambda, doub
     }
     for (int i = 0; i < n - 1; ++i) e2[i] = e[i] * e[i];
     const double scale = std::max(std::fabs(lo), std::fabs(hi));
+    // multisection: kPts interior points cut [lo, hi] into kPts + 1 parts
     for (int it = 0; it < 200; ++it) {
-        const double mid = 0.5 * (lo + hi);
-        if (mid <= lo || mid >= hi || hi - lo <= 4.0 * 2.220446049250313e-16 * scale) break;
-        if (sturm_count(n, d, e2.data(), mid) >= 1) hi = mid;
-        else lo = mid;
+        if (hi - lo <= 4.0 * 2.220446049250313e-16 * scale) break;
+        double x[kPts];
+        int cnt[kPts];
+        for (int k = 0; k < kPts; ++k) x[k] = lo + (hi - lo) * (double)(k + 1) / (double)(kPts + 1);
+        if (x[0] <= lo || x[kPts - 1] >= hi) break;
+        sturm_counts(n, d, e2.data(), x, cnt);
+        int k = 0;
+        while (k < kPts && cnt[k] < 1) ++k;  // first point with an eigenvalue below it
+        if (k < kPts) hi = x[k];
+        if (k > 0) lo = x[k - 1];
     }
     const double lam = 0.5 * (lo + hi);
     *lambda = lam;

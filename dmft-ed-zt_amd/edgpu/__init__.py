@@ -510,10 +510,11 @@ class Batch:
             out.append(dict(id=i.value, e0=e0.value, G=Gc, nstep=ns.value, coeffs=coeffs))
 
     def stats(self):
-        st = (C.c_double * 8)()
+        st = (C.c_double * 12)()
         self._chk(self._L.edg_batch_stats(self._h, st))
         keys = ("steps", "chain_steps", "finished", "bytes_in_use", "budget", "in_flight",
-                "graph_launches", "rebuilds")
+                "graph_launches", "rebuilds", "host_admit_s", "host_progress_s", "host_steps_s",
+                "host_wait_s")
         out = dict(zip(keys, list(st)))
         ni, nd, nl = C.c_longlong(0), C.c_longlong(0), C.c_int(0)
         self._chk(self._L.edg_batch_graph_stats(self._h, C.byref(ni), C.byref(nd), C.byref(nl)))

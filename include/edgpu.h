@@ -285,14 +285,17 @@ typedef struct {
  * problems in flight may hold (0 = 85% of the free device memory, counting
  * as free the idle chunks of earlier queues); a problem is also held back
  * while the device cannot allocate it.  Problem memory is sub-allocated
- * from large device chunks that are not unmapped while the process runs;
- * edg_batch_destroy leaves a queue's chunks to later queues, and
- * edg_batch_release_memory returns the idle ones to the device. */
+ * from large device chunks that are not unmapped while the process runs,
+ * and the pinned host areas of the problems from 16 MiB slabs;
+ * edg_batch_destroy leaves a queue's chunks and slabs (up to 1 GiB of
+ * them) to later queues, and edg_batch_release_memory returns the idle
+ * ones. */
 int edg_batch_create(int device, int precision, int nspin, int nw, const double *omega,
                      unsigned long long budget_bytes, edg_batch **out);
 int edg_batch_destroy(edg_batch *b);
 /* Return to the device the idle memory chunks that destroyed queues left
- * on it (no counterpart in the reference). */
+ * on it, and free their idle pinned host slabs (no counterpart in the
+ * reference). */
 int edg_batch_release_memory(int device);
 const char *edg_batch_last_error(const edg_batch *b);
 /* Replay the batched steps as captured HIP graphs (1, default) or launch
@@ -329,9 +332,13 @@ int edg_batch_run(edg_batch *b);
  * edg_cluster_green), or returns 0 when none is ready.  Pointers may be
  * NULL. */
 int edg_batch_poll(edg_batch *b, long long *id, double *e0, double *G, int *nstep, double *coeffs);
-/* stats[0] batched steps, [1] chain-steps, [2] problems finished,
- * [3] device bytes in use, [4] budget, [5] problems in flight, [6] graph
- * launches, [7] descriptor rebuilds. */
+/* stats[0..11] (the caller passes 12 doubles): [0] batched steps,
+ * [1] chain-steps, [2] problems finished, [3] device bytes in use,
+ * [4] budget, [5] problems in flight, [6] graph launches, [7] descriptor
+ * rebuilds; host seconds inside edg_batch_step spent on [8] admission
+ * (tables, start vectors), [9] stage transitions (read-backs, tridiagonal
+ * eigenvectors, combine, apply_c, continued fractions), [10] queueing the
+ * steps, [11] blocked on device events. */
 int edg_batch_stats(const edg_batch *b, double *stats);
 /* Graph lifecycle counters (no counterpart in the reference): graph execs
  * instantiated and destroyed so far, and execs alive now (current plus
