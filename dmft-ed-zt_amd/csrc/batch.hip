@@ -122,6 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, 
     const long long vbytes = (long long)nup * ndn * (long long)sizeof(T);
     const __amdgpu_buffer_rsrc_t xrs = buf_rsrc(x, vbytes), yrs = buf_rsrc(y, vbytes);
     const __amdgpu_buffer_rsrc_t prs = buf_rsrc(pv ? pv : x, vbytes);
+    const double bq = pv ? bsp : 0.0;
     if (g < G) {
         for (int iup = threadIdx.x - g * span; iup < nup; iup += span) {
             const Row<uint32_t, KS> ur = gload(reinterpret_cast<const Row<uint32_t, KS> *>(op.up.hrec + (long long)iup * KS));
@@ -136,16 +137,16 @@ __global__ __launch_bounds__(kBlock) void k_bhv(const BInst *__restrict__ inst, 
 #pragma unroll
                 for (int k = 0; k < KS; ++k) gn[k] = bload<T>(xrs, doff[k] + ib, 0);
                 const uint32_t eb = (uint32_t)(c0 + c) * (uint32_t)nup * (uint32_t)sizeof(T) + ib;
-                const T old = pv ? bload<T>(prs, eb, 0) : T(0);
+                // step 0 (no u_{j-1}) reads x here and weights it by 0: no
+                // branch, so nothing waits on the loads before their use
+                const T old = bload<T>(prs, eb, 0);
                 const ACC xv = (ACC)xc[iup];
                 ACC acc = (eu + s_ed[c] + (mu && s_md[c] ? U : ACC(0))) * xv;
 #pragma unroll
                 for (int k = 0; k < KS; ++k) acc += s_vu[rec_vidx(ur.v[k])] * (ACC)xc[rec_rank(ur.v[k])];
 #pragma unroll
                 for (int k = 0; k < KS; ++k) acc += dv[k] * (ACC)gn[k];
-                double r = sx * (double)acc;
-                if (pv) r -= bsp * (double)old;
-                const T rt = (T)r;
+                const T rt = (T)(sx * (double)acc - bq * (double)old);
                 bstore<T>(yrs, eb, rt);
                 dot += (double)xv * (double)rt;
             }
