@@ -28,6 +28,7 @@
 #include <cstring>
 #include <deque>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -58,7 +59,11 @@ constexpr int kChunk = 8;              // batched steps launched per chunk
 // the batched steps had no chains to run.  With the step stream this is the
 // device's 4 hardware queues.
 constexpr int kSides = 3;
-constexpr int kGraphAfter = 2;         // chunks of an unchanged chain set before one is captured
+constexpr int kGraphAfter = 2;
+// host threads of the ground-state eigensolves (tridiagonal of up to 1000
+// steps, ~30 us at 200), at least kEigenPerThread problems each
+constexpr int kEigenThreads = 8;
+constexpr int kEigenPerThread = 8;         // chunks of an unchanged chain set before one is captured
 constexpr size_t kDescMin = 1 << 20;   // initial bytes of the descriptor buffer
 constexpr int kAhead = 2;              // chunks the host may run ahead of the device
 constexpr int kChunkEvents = 4;        // > kAhead
@@ -178,11 +183,12 @@ struct Job {
     double *dparams = nullptr;  // em(norb,2) | ek(nbath,2) | vmk(norb,nbath,2)
     SpinTab tab[2][3];          // [spin][ne - N_spin + 1]
     bool has_tab[2][3] = {};
+    // c+ maps of the N + 1 tables, c maps of the N - 1 tables ([orbital][rank])
+    uint32_t *tmap[2][3] = {};
     void *gs = nullptr;
     double *misc = nullptr;     // [1..3] RQ sums, [4] |combine|, [5] 1/|combine|, [8+k] |v_init| of chain k
     double *coef = nullptr;     // device coefficient staging
     double *gdev = nullptr;     // norb x 2 spins x 2 nw
-    uint32_t *map[kMaxChains] = {};
     Chain gsc;
     Chain ch[kMaxChains];       // chain 4 (iorb - 1) + k: (up,+), (up,-), (dn,+), (dn,-)
     double *host = nullptr;     // pinned readback / upload area
@@ -303,7 +309,7 @@ size_t tab_bytes(int n, int norb, int nbath)
     const size_t K = std::max(norb * nbath, kMinSlots);
     const size_t orb = norb >= 2 ? 4 * (size_t)norb * norb * n + 256 : 0;
     return 4 * (size_t)n + K * n * (sizeof(HopF) + sizeof(HopD)) + n + 4 + 8 * (size_t)n + 4 * (size_t)n +
-           n + 16 * (size_t)n * (2 + 1 + 4) + orb + 16 * 256;
+           n + 16 * (size_t)n * (2 + 1 + 4) + orb + 4 * (size_t)norb * n + 17 * 256;
 }
 
 // A chain's device state block: step index (int) at byte 0, flags (two
@@ -399,7 +405,7 @@ size_t problem_bytes(const edg_batch *b, const edg_problem &p)
         const int cd = p.ndn + (ispin == 2 ? (pm == 1 ? 1 : -1) : 0);
         if (cu < 0 || cd < 0 || cu > ns || cd > ns) continue;
         const long long cnu = binom_ll(ns, cu), cnd = binom_ll(ns, cd), Dc = cnu * cnd;
-        bytes += 3 * rup(Dc, kPadB) * b->es + 4 * (size_t)(ispin == 1 ? cnu : cnd) + 256;
+        bytes += 3 * rup(Dc, kPadB) * b->es;
         bytes += state_bytes(mg, chain_hv_parts(b, ns, cnu, cnd, no), chain_ax_parts(Dc));
     }
     return bytes + 64 * 256;
@@ -473,8 +479,9 @@ double kan_jp(const edg_problem &p) { return p.Jp < 0 ? p.J : p.Jp; }
 
 // Per-spin tables of one (spin, N) half; kc = hop slots per row, the same
 // for every table of the problem so that its chains share one launch class;
-// km = the table's largest hop count.
-int build_table(edg_batch *b, Job *J, int spin, int ne, SpinTab &t, int kc, int km)
+// km = the table's largest hop count; pm = 1 / 2: also the c+ / c maps of
+// the sector (into *map).
+int build_table(edg_batch *b, Job *J, int spin, int ne, SpinTab &t, int kc, int km, int pm, uint32_t **map)
 {
     const int ns = nsite_of(J->p), nbath = J->p.nbath, norb = J->norb;
     t = SpinTab{};
@@ -494,6 +501,7 @@ int build_table(edg_batch *b, Job *J, int spin, int ne, SpinTab &t, int kc, int 
     t.ccol = (uint16_t *)J->ar.take(16 * n * 2);
     t.cidx = (uint8_t *)J->ar.take(16 * n);
     t.hrec = (uint32_t *)J->ar.take(16 * n * 4);
+    *map = pm ? (uint32_t *)J->ar.take(4 * (size_t)norb * n) : nullptr;
     if (J->ar.overflow) return fail(b, EDG_ERR_NOMEM, "problem arena overflow (tables)");
     const double *em_s = J->dparams + (size_t)(spin - 1) * norb;
     const double *ek_s = J->dparams + 2 * norb + (size_t)(spin - 1) * nbath;
@@ -502,7 +510,7 @@ int build_table(edg_batch *b, Job *J, int spin, int ne, SpinTab &t, int kc, int 
     // the hop counts are known on the host (km), so no device maximum; the
     // compressed records are written by the same launch
     const double UpmJ = norb >= 2 ? kan_up(J->p) - J->p.J : 0.0;
-    if (launch_spin_table(b->side[J->lane], ns, norb, nbath, ne, em_s, ek_s, v_s, J->p.mu, UpmJ, t, true))
+    if (launch_spin_table(b->side[J->lane], ns, norb, nbath, ne, em_s, ek_s, v_s, J->p.mu, UpmJ, t, true, *map, pm))
         return fail(b, EDG_ERR_HIP, "spin table launch failed");
     return 0;
 }
@@ -564,18 +572,6 @@ int init_chain(edg_batch *b, Job *J, Chain &C, const HvOp &op, int maxn, bool st
     return 0;
 }
 
-// scale[0] = 1/|slot 0| (norm into norm_out); init_chain set the flags.
-int start_chain(edg_batch *b, Job *J, Chain &C, double *norm_out)
-{
-    BInst &I = C.bi;
-    int rc = b->fp64 ? launch_norm<double>(b->side[J->lane], (const double *)I.base, C.D, b->side_part[J->lane],
-                                           norm_out, I.scale, nullptr)
-                     : launch_norm<float>(b->side[J->lane], (const float *)I.base, C.D, b->side_part[J->lane],
-                                          norm_out, I.scale, nullptr);
-    if (rc) return fail(b, EDG_ERR_HIP, "norm launch failed");
-    C.pending = true;  // joins the steps when the host sees J->start_ev complete
-    return 0;
-}
 
 // Returns 1 (nothing changed) when the device cannot hold the problem now.
 int admit(edg_batch *b, Job *J)
@@ -628,7 +624,9 @@ int admit(edg_batch *b, Job *J)
             const int ne = n0 + d;
             J->has_tab[s][d + 1] = ne >= 0 && ne <= ns;
             if (J->has_tab[s][d + 1])
-                if (int rc = build_table(b, J, s + 1, ne, J->tab[s][d + 1], kc, km[s][d + 1])) return rc;
+                if (int rc = build_table(b, J, s + 1, ne, J->tab[s][d + 1], kc, km[s][d + 1],
+                                         d == 1 ? 1 : (d == -1 ? 2 : 0), &J->tmap[s][d + 1]))
+                    return rc;
         }
     }
     J->misc = (double *)J->ar.take(64 * 8);
@@ -640,10 +638,14 @@ int admit(edg_batch *b, Job *J)
     if (J->ar.overflow) return fail(b, EDG_ERR_NOMEM, "problem arena overflow");
     if (int rc = init_chain(b, J, J->gsc, op, p.maxnstep, true)) return rc;
     const long long D = J->gsc.D;
-    const int rc0 = b->fp64 ? launch_start<double>(b->side[J->lane], (double *)J->gsc.bi.base, D, p.seed)
-                            : launch_start<float>(b->side[J->lane], (float *)J->gsc.bi.base, D, p.seed);
+    // the start vector with its norm (misc[0]) and scale[0] = 1/|v|
+    double *pt = b->side_part[J->lane];
+    const int rc0 = b->fp64 ? launch_start<double>(b->side[J->lane], (double *)J->gsc.bi.base, D, p.seed, pt,
+                                                   &J->misc[0], J->gsc.bi.scale)
+                            : launch_start<float>(b->side[J->lane], (float *)J->gsc.bi.base, D, p.seed, pt,
+                                                  &J->misc[0], J->gsc.bi.scale);
     if (rc0) return fail(b, EDG_ERR_HIP, "start launch failed");
-    if (int rc = start_chain(b, J, J->gsc, &J->misc[0])) return rc;
+    J->gsc.pending = true;  // joins the steps when the host sees start_ev complete
     BCHK(b, hipEventRecord(J->start_ev, b->side[J->lane]));
     J->start_pending = true;
     J->st = ST_GS;
@@ -676,21 +678,34 @@ int chain_ab(const double *h, int m, double *a, double *bb)
 }
 
 // Ground-state chain done: eigenvector, E0 and the c+/c start vectors.
-int after_gs(edg_batch *b, Job *J)
+// Host half of the ground-state stage: the lowest eigenvector of the chain's
+// tridiagonal matrix, staged as combine coefficients z_j s_j.  Touches only
+// the problem's own host data, so several problems run it on threads at
+// once.  Returns 0, 1 (breakdown at the start) or 2 (eigensolve failed).
+int gs_eigen(Job *J)
 {
-    const edg_problem &p = J->p;
-    const int m = p.maxnstep;
+    const int m = J->p.maxnstep;
     std::vector<double> a(m), bb(m);
     const int nst = chain_ab(J->host, m, a.data(), bb.data());
     const double *s = J->host + kHead + 2 * m + 1;
     J->gs_nstep = nst;
-    if (nst < 1) return fail(b, EDG_ERR_SOLVER, "ground-state chain broke down at its start");
+    if (nst < 1) return 1;
     std::vector<double> z(nst);
     double lam;
-    if (tridiag_lowest(nst, a.data(), bb.data() + 1, &lam, z.data()))
-        return fail(b, EDG_ERR_SOLVER, "tridiagonal eigensolve failed");
+    if (tridiag_lowest(nst, a.data(), bb.data() + 1, &lam, z.data())) return 2;
     double *hc = J->host + chain_host_n(m) + 8;
     for (int j = 0; j < nst; ++j) hc[j] = z[j] * s[j];
+    return 0;
+}
+
+// Device half: combine, Rayleigh quotient and the c+/c start vectors.
+int after_gs(edg_batch *b, Job *J, int eig_rc)
+{
+    if (eig_rc == 1) return fail(b, EDG_ERR_SOLVER, "ground-state chain broke down at its start");
+    if (eig_rc) return fail(b, EDG_ERR_SOLVER, "tridiagonal eigensolve failed");
+    const edg_problem &p = J->p;
+    const int m = p.maxnstep, nst = J->gs_nstep;
+    double *hc = J->host + chain_host_n(m) + 8;
     BCHK(b, hipMemcpyAsync(J->coef, hc, sizeof(double) * nst, hipMemcpyHostToDevice, b->side[J->lane]));
     const HvOp op = J->gsc.bi.op;
     const long long D = J->gsc.D;
@@ -729,19 +744,21 @@ int after_gs(edg_batch *b, Job *J)
         if (nu < 0 || nd < 0 || nu > ns_site || nd > ns_site) continue;
         const HvOp cop = make_op(b, J, nu, nd);
         if (int r = init_chain(b, J, C, cop, mg, false)) return r;
+        // the map built with the output sector's table (N + 1 for c+, N - 1
+        // for c), at this orbital
         const SpinTab &tout = C.ispin == 1 ? cop.up : cop.dn;
-        J->map[k] = (uint32_t *)J->ar.take(4 * (size_t)tout.n);
-        if (J->ar.overflow) return fail(b, EDG_ERR_NOMEM, "problem arena overflow (map)");
-        const int ne_in = C.ispin == 1 ? p.nup : p.ndn;
-        if (launch_cmap(b->side[J->lane], tout.pattern, tout.n, ns_site, ne_in, C.iorb - 1, C.pm, J->map[k]))
-            return fail(b, EDG_ERR_HIP, "cmap launch failed");
+        const uint32_t *map = J->tmap[C.ispin - 1][C.pm == 1 ? 2 : 0] + (size_t)(C.iorb - 1) * tout.n;
+        // the start vector c(+) |gs> with |v_init| (misc[8 + k]) and scale[0]
         const int godd = C.ispin == 2 ? (p.nup & 1) : 0;
+        double *pt = b->side_part[J->lane];
         const int arc = b->fp64 ? launch_apply_c<double>(b->side[J->lane], (const double *)J->gs, (double *)C.bi.base,
-                                                         cop.nup, cop.ndn, op.nup, C.ispin, J->map[k], godd)
+                                                         cop.nup, cop.ndn, op.nup, C.ispin, map, godd, pt,
+                                                         &J->misc[8 + k], C.bi.scale)
                                 : launch_apply_c<float>(b->side[J->lane], (const float *)J->gs, (float *)C.bi.base,
-                                                        cop.nup, cop.ndn, op.nup, C.ispin, J->map[k], godd);
+                                                        cop.nup, cop.ndn, op.nup, C.ispin, map, godd, pt,
+                                                        &J->misc[8 + k], C.bi.scale);
         if (arc) return fail(b, EDG_ERR_HIP, "apply_c launch failed");
-        if (int r = start_chain(b, J, C, &J->misc[8 + k])) return r;
+        C.pending = true;
     }
     // also covers the RQ sums a ground-state-only problem reads next
     BCHK(b, hipEventRecord(J->start_ev, b->side[J->lane]));
@@ -858,8 +875,12 @@ int finish(edg_batch *b, Job *J)
 }
 
 // Event-driven stage transitions of every running problem (no blocking).
+// Problems whose ground-state chains were read back together (a batch's
+// problems finish those chains at the same step) solve their tridiagonal
+// matrices on host threads at once, then queue their device work in order.
 int progress(edg_batch *b)
 {
+    std::vector<Job *> gs_ready;
     for (size_t i = 0; i < b->running.size();) {
         Job *J = b->running[i];
         if (J->start_pending) {
@@ -888,7 +909,7 @@ int progress(edg_batch *b)
             BCHK(b, hipEventRecord(J->ev, b->stream));
             J->st = ST_GS_READ;
         } else if (J->st == ST_GS_READ && hipEventQuery(J->ev) == hipSuccess) {
-            if (int rc = after_gs(b, J)) return rc;
+            gs_ready.push_back(J);
         } else if (J->st == ST_CHAINS && !chains_running(J)) {
             // misc (RQ sums, start norms) was written before start_ev
             BCHK(b, hipMemcpyAsync(J->host, J->misc, 16 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
@@ -908,6 +929,22 @@ int progress(edg_batch *b)
         }
         ++i;
     }
+    const int nr = (int)gs_ready.size();
+    if (nr == 0) return 0;
+    std::vector<int> erc(nr);
+    const int nt = nr < kEigenPerThread ? 1 : std::min(kEigenThreads, (nr + kEigenPerThread - 1) / kEigenPerThread);
+    if (nt == 1) {
+        for (int k = 0; k < nr; ++k) erc[k] = gs_eigen(gs_ready[k]);
+    } else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back([&, t]() {
+                for (int k = t; k < nr; k += nt) erc[k] = gs_eigen(gs_ready[k]);
+            });
+        for (auto &x : th) x.join();
+    }
+    for (int k = 0; k < nr; ++k)
+        if (int rc = after_gs(b, gs_ready[k], erc[k])) return rc;
     return 0;
 }
 

@@ -184,7 +184,8 @@ constexpr int kAxpyMaxGrid = 256;
 // ---- launchers (kernels.hip) ----
 int launch_spin_table(hipStream_t s, int nsite, int norb, int nbath, int ne, const double *em_s,
                       const double *ek_s, const double *v_s, double mu, double UpmJ,
-                      const SpinTab &t, bool compress = false);
+                      const SpinTab &t, bool compress = false, uint32_t *cmap = nullptr,
+                      int cmap_pm = 0);
 int launch_compress(hipStream_t s, const SpinTab &t);
 // Returns the number of workgroups (= partial sums written) or < 0 on error.
 template <typename T>
@@ -223,8 +224,11 @@ template <typename T>
 int launch_axpy_norm(hipStream_t s, T *r, const T *u, long long n, const double *hv_part,
                      int hv_n, const double *scale, double *alpha, int step, double *partial,
                      int *stop);
+// partial non-null: also |x| into norm_out and 1/|x| into scale_out (the
+// sums of launch_norm, in its order).
 template <typename T>
-int launch_start(hipStream_t s, T *x, long long n, unsigned long long seed);
+int launch_start(hipStream_t s, T *x, long long n, unsigned long long seed, double *partial = nullptr,
+                 double *norm_out = nullptr, double *scale_out = nullptr);
 // Streaming reference passes over n floats (n a multiple of 4): y = x
 // (copy) and y = x + z (add), 16-byte accesses, grid-stride.
 int launch_stream_copy(hipStream_t s, const float *x, float *y, long long n);
@@ -241,9 +245,11 @@ template <typename T>
 int launch_to_f64(hipStream_t s, double *dst, const T *src, long long n);
 int launch_cmap(hipStream_t s, const uint32_t *pattern_out, int n_out, int nsite, int ne_in,
                 int bit, int pm, uint32_t *map);
+// partial non-null: also |out| and 1/|out|, as launch_start.
 template <typename T>
 int launch_apply_c(hipStream_t s, const T *in, T *out, int nup_out, int ndn_out, int nup_in,
-                   int spin, const uint32_t *map, int global_odd);
+                   int spin, const uint32_t *map, int global_odd, double *partial = nullptr,
+                   double *norm_out = nullptr, double *scale_out = nullptr);
 template <typename T>
 int launch_observables(hipStream_t s, const T *gs, const HvOp &op, const double *scale,
                        double *partial, double *out);

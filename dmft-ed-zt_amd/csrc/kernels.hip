@@ -193,7 +193,8 @@ __device__ inline void compress_row(const SpinTab &t, int r)
 __global__ __launch_bounds__(kBlock) void k_spin_table(int nsite, int norb, int nbath, int ne,
                                                        const double *em_s, const double *ek_s,
                                                        const double *v_s, double mu, double UpmJ,
-                                                       SpinTab t, int compress)
+                                                       SpinTab t, int compress, uint32_t *cmap,
+                                                       int cmap_pm)
 {
     __shared__ uint32_t B[16][16];
     for (int e = threadIdx.x; e < 256; e += kBlock) {
@@ -224,6 +225,16 @@ __global__ __launch_bounds__(kBlock) void k_spin_table(int nsite, int norb, int 
             if ((q >> b) & 1u) { ++k; s += binom_tab(B, b, k); }
         return s;
     };
+    // c+ (cmap_pm 1) / c (2) source map of the N - 1 / N + 1 sector for
+    // every impurity orbital (the maps of k_cmap, cmap[m * n + r])
+    if (cmap)
+        for (int m = 0; m < norb; ++m) {
+            const uint32_t set = (p >> m) & 1u;
+            uint32_t code = kInvalid;
+            if ((cmap_pm == 1 && set) || (cmap_pm == 2 && !set))
+                code = rank_of(p ^ (1u << m)) | ((uint32_t)(__popc(p & ((1u << m) - 1u)) & 1) << 31);
+            cmap[(long long)m * t.n + r] = code;
+        }
     // hybridization hops in reference loop order (iorb outer, bath inner)
     int slot = 0;
     for (int m = 0; m < norb; ++m) {
@@ -297,11 +308,11 @@ int launch_compress(hipStream_t s, const SpinTab &t)
 
 int launch_spin_table(hipStream_t s, int nsite, int norb, int nbath, int ne, const double *em_s,
                       const double *ek_s, const double *v_s, double mu, double UpmJ,
-                      const SpinTab &t, bool compress)
+                      const SpinTab &t, bool compress, uint32_t *cmap, int cmap_pm)
 {
     const int grid = (t.n + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_spin_table, dim3(grid), dim3(kBlock), 0, s, nsite, norb, nbath, ne, em_s,
-                       ek_s, v_s, mu, UpmJ, t, compress ? 1 : 0);
+                       ek_s, v_s, mu, UpmJ, t, compress ? 1 : 0, cmap, cmap_pm);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1082,23 +1093,37 @@ int launch_stream_add(hipStream_t s, const float *x, const float *z, float *y, l
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// partial non-null: also the partial sums of |x|^2, in k_norm's element
+// order (same grid), so the norm is the one a separate k_norm would give.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_start(T *x, long long n, unsigned long long seed)
+__global__ __launch_bounds__(kBlock) void k_start(T *x, long long n, unsigned long long seed, double *partial)
 {
+    double v[1] = {0.0};
     for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
-         i += (long long)gridDim.x * kBlock)
-        x[i] = (T)((double)(splitmix64(seed * 0xD1B54A32D192ED03ull + (uint64_t)i) >> 11) *
-                   0x1.0p-53);
+         i += (long long)gridDim.x * kBlock) {
+        const T t = (T)((double)(splitmix64(seed * 0xD1B54A32D192ED03ull + (uint64_t)i) >> 11) *
+                        0x1.0p-53);
+        x[i] = t;
+        v[0] += (double)t * (double)t;
+    }
+    if (partial) block_partial<1>(v, partial);
 }
 
 template <typename T>
-int launch_start(hipStream_t s, T *x, long long n, unsigned long long seed)
+int launch_start(hipStream_t s, T *x, long long n, unsigned long long seed, double *partial,
+                 double *norm_out, double *scale_out)
 {
-    hipLaunchKernelGGL(k_start<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, x, n, seed);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    const int grid = grid_for(n);
+    hipLaunchKernelGGL(k_start<T>, dim3(grid), dim3(kBlock), 0, s, x, n, seed, partial);
+    if (hipGetLastError() != hipSuccess) return -1;
+    if (!partial) return 0;
+    return launch_finish<1>(s, partial, FinArgs{FIN_NORM, grid, nullptr, norm_out, scale_out,
+                                                nullptr, nullptr, 0, 0});
 }
-template int launch_start<float>(hipStream_t, float *, long long, unsigned long long);
-template int launch_start<double>(hipStream_t, double *, long long, unsigned long long);
+template int launch_start<float>(hipStream_t, float *, long long, unsigned long long, double *, double *,
+                                 double *);
+template int launch_start<double>(hipStream_t, double *, long long, unsigned long long, double *, double *,
+                                  double *);
 
 // out = sum_j coef_j * basis_j (FP64 accumulation, in j order), plus
 // |out|^2.  Each thread issues the loads of eight vectors before their
@@ -1249,12 +1274,16 @@ int launch_cmap(hipStream_t s, const uint32_t *pattern_out, int n_out, int nsite
 
 // spin 1: out(iup', idn) = sgn * in(map(iup'), idn)
 // spin 2: out(iup, idn') = sgn * (-1)^{N_up} * in(iup, map(idn'))
+// partial non-null: also the partial sums of |out|^2 (k_norm's order, as
+// k_start).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_apply_c(const T *in, T *out, int nup_out,
                                                     int ndn_out, int nup_in, int spin,
-                                                    const uint32_t *map, int global_odd)
+                                                    const uint32_t *map, int global_odd,
+                                                    double *partial)
 {
     const long long n = (long long)nup_out * ndn_out;
+    double acc[1] = {0.0};
     for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < n;
          e += (long long)gridDim.x * kBlock) {
         const int idn = (int)(e / nup_out);
@@ -1274,21 +1303,28 @@ __global__ __launch_bounds__(kBlock) void k_apply_c(const T *in, T *out, int nup
             }
         }
         out[e] = v;
+        acc[0] += (double)v * (double)v;
     }
+    if (partial) block_partial<1>(acc, partial);
 }
 template <typename T>
 int launch_apply_c(hipStream_t s, const T *in, T *out, int nup_out, int ndn_out, int nup_in,
-                   int spin, const uint32_t *map, int global_odd)
+                   int spin, const uint32_t *map, int global_odd, double *partial, double *norm_out,
+                   double *scale_out)
 {
     const long long n = (long long)nup_out * ndn_out;
-    hipLaunchKernelGGL(k_apply_c<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, in, out, nup_out,
-                       ndn_out, nup_in, spin, map, global_odd);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    const int grid = grid_for(n);
+    hipLaunchKernelGGL(k_apply_c<T>, dim3(grid), dim3(kBlock), 0, s, in, out, nup_out,
+                       ndn_out, nup_in, spin, map, global_odd, partial);
+    if (hipGetLastError() != hipSuccess) return -1;
+    if (!partial) return 0;
+    return launch_finish<1>(s, partial, FinArgs{FIN_NORM, grid, nullptr, norm_out, scale_out,
+                                                nullptr, nullptr, 0, 0});
 }
 template int launch_apply_c<float>(hipStream_t, const float *, float *, int, int, int, int,
-                                   const uint32_t *, int);
+                                   const uint32_t *, int, double *, double *, double *);
 template int launch_apply_c<double>(hipStream_t, const double *, double *, int, int, int, int,
-                                    const uint32_t *, int);
+                                    const uint32_t *, int, double *, double *, double *);
 
 // -------------------------------------------------------- observables
 
