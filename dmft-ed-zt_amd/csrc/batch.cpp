@@ -1248,6 +1248,43 @@ int run_steps(edg_batch *b, int k)
     return 0;
 }
 
+// Admission in submission order by device bytes: a large problem waits
+// while kMaxLarge others run, and the problems behind it go ahead when they
+// fit.  The device memory of the queue's problems that fit the budget is
+// reserved first, in one piece: growing the heap chunk by chunk while the
+// device runs cost up to a second per batch.
+int admit_queued(edg_batch *b)
+{
+    const auto t = std::chrono::steady_clock::now();
+    size_t want = 0;
+    for (const Job *J : b->queue) {
+        if (b->in_use + want + J->bytes > b->budget) break;
+        want += J->bytes;
+    }
+    if (want > 0 && b->heap->reserve(want) < 0) return fail(b, EDG_ERR_HIP, "device memory reservation failed");
+    int large_in_flight = 0;
+    for (const Job *R : b->running) large_in_flight += R->large ? 1 : 0;
+    for (auto it = b->queue.begin(); it != b->queue.end();) {
+        Job *J = *it;
+        if (J->large && large_in_flight >= kMaxLarge) {
+            ++it;
+            continue;
+        }
+        if (b->in_use + J->bytes > b->budget) break;
+        const int rc = admit(b, J);
+        if (rc == 1) {  // device full: wait for running problems to release memory
+            if (b->running.empty())
+                return fail(b, EDG_ERR_NOMEM, "device memory exhausted with no problem in flight");
+            break;
+        }
+        if (rc) return rc;
+        large_in_flight += J->large ? 1 : 0;
+        it = b->queue.erase(it);
+    }
+    b->st_t_admit += std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1433,30 +1470,7 @@ int edg_batch_step(edg_batch *b, int max_steps, int *pending)
     auto t = clk::now();
     if (int rc = progress(b)) return rc;
     b->st_t_progress += secs(t);
-    t = clk::now();
-    // admission in submission order by device bytes; a large problem waits
-    // while kMaxLarge others run, and the problems behind it go ahead when
-    // they fit
-    int large_in_flight = 0;
-    for (const Job *R : b->running) large_in_flight += R->large ? 1 : 0;
-    for (auto it = b->queue.begin(); it != b->queue.end();) {
-        Job *J = *it;
-        if (J->large && large_in_flight >= kMaxLarge) {
-            ++it;
-            continue;
-        }
-        if (b->in_use + J->bytes > b->budget) break;
-        const int rc = admit(b, J);
-        if (rc == 1) {  // device full: wait for running problems to release memory
-            if (b->running.empty())
-                return fail(b, EDG_ERR_NOMEM, "device memory exhausted with no problem in flight");
-            break;
-        }
-        if (rc) return rc;
-        large_in_flight += J->large ? 1 : 0;
-        it = b->queue.erase(it);
-    }
-    b->st_t_admit += secs(t);
+    if (int rc = admit_queued(b)) return rc;
     if ((!b->active.empty() || !b->direct.empty()) && max_steps > 0) {
         int left = max_steps;
         while (left > 0 && (!b->active.empty() || !b->direct.empty())) {

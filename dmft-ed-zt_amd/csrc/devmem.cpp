@@ -163,6 +163,37 @@ int DevHeap::alloc(size_t bytes, void **p)
     return 0;
 }
 
+int DevHeap::reserve(size_t bytes)
+{
+    if (device_ < 0 || device_ >= kMaxDevices) return -1;
+    size_t have = 0;
+    for (const DevChunk *c : chunks_)
+        for (const auto &f : c->free) have += f.second;
+    if (have >= bytes) return 0;
+    {
+        // idle cached chunks first: the smallest that covers the rest, else
+        // the largest, until the rest is covered or the cache is empty
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto &cache = g_cache[device_];
+        while (have < bytes && !cache.empty()) {
+            const size_t rest = bytes - have;
+            auto pick = cache.end();
+            for (auto it = cache.begin(); it != cache.end(); ++it)
+                if ((*it)->size >= rest && (pick == cache.end() || (*it)->size < (*pick)->size)) pick = it;
+            if (pick == cache.end())
+                pick = std::max_element(cache.begin(), cache.end(),
+                                        [](const DevChunk *a, const DevChunk *b) { return a->size < b->size; });
+            have += (*pick)->size;
+            chunks_.push_back(*pick);
+            cache.erase(pick);
+        }
+    }
+    if (have >= bytes) return 0;
+    DevChunk *c = new_chunk(std::max(round_up(bytes - have), kMinChunk));
+    if (c) chunks_.push_back(c);
+    return 0;
+}
+
 void DevHeap::free(void *p)
 {
     if (!p) return;

@@ -32,6 +32,10 @@ public:
     // must be ordered after the last user, e.g. on the same stream).
     void free(void *p);
     size_t owned_bytes() const;
+    // Make the free bytes of the owned chunks at least `bytes`, with one
+    // chunk (from the device cache, else the driver) when they fall short.
+    // 0: done or not possible now (alloc then grows as it can); -1: HIP error.
+    int reserve(size_t bytes);
 
 private:
     int device_;
