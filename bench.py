@@ -293,13 +293,20 @@ def profile_hv(edgpu, em, ek, vmk, U, mu):
 
 def bench_batched(edgpu, W, S, graphs=True):
     """512 independent single-orbital problems (Ns in {8,10,12}) through the
-    batched solver: whole-batch solves/s, after a warm-up batch of 32."""
+    batched solver: whole-batch solves/s, the median of three timed batches
+    after one untimed batch of the same problems (which maps the device
+    memory the batch holds; a first batch in a process also pays for that)."""
     from edgpu import autotune
     om = W.matsubara(2048, 100.0)
     probs = W.batched_problems(edgpu, n=512)
     sched, timings = autotune.tune_batch(edgpu, probs, om, graphs=graphs)
-    S.run_batched(edgpu, probs[:32], om, schedule=sched, graphs=graphs)
-    res, wall, st = S.run_batched(edgpu, probs, om, schedule=sched, graphs=graphs)
+    S.run_batched(edgpu, probs, om, schedule=sched, graphs=graphs)
+    runs = []
+    for _ in range(3):
+        res, wall, st = S.run_batched(edgpu, probs, om, schedule=sched, graphs=graphs)
+        runs.append((wall, res, st))
+    runs.sort(key=lambda r: r[0])
+    wall, res, st = runs[1]
     ns = [p.nbath + 1 for p in probs]
     return dict(workload="512 single-orbital problems, Ns uniform in {8,10,12}, U~U[0,6], "
                          "n~U[0.8,1.2] (N=round(n Ns), N_up=ceil(N/2)), eps~U[-1.5,1.5], "
@@ -307,7 +314,8 @@ def bench_batched(edgpu, W, S, graphs=True):
                          "2048 Matsubara points",
                 problems=len(probs), ns_counts={str(k): ns.count(k) for k in (8, 10, 12)},
                 schedule=[list(e) for e in sched],
-                solves_per_s=len(probs) / wall, wall_s=wall, batched_steps=st["steps"],
+                solves_per_s=len(probs) / wall, wall_s=wall, runs_solves_per_s=[len(probs) / r[0] for r in runs],
+                batched_steps=st["steps"],
                 chain_steps_per_s=st["chain_steps"] / wall,
                 e0_finite=bool(np.all(np.isfinite([r["e0"] for r in res]))))
 

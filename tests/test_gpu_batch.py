@@ -60,6 +60,9 @@ def run_batch(edgpu, probs, omega, precision="fp64", graphs=True, budget=0, nspi
     res = {r["id"]: r for r in b.poll()}
     st = b.stats()
     b.close()
+    # host-time counters: seconds, and together no more than the run took
+    hs = [st[k] for k in ("host_admit_s", "host_progress_s", "host_steps_s", "host_wait_s")]
+    assert all(np.isfinite(h) and h >= 0.0 for h in hs)
     return [res[i] for i in ids], st, peak
 
 
