@@ -13,7 +13,13 @@ Tolerances (written per test):
                                   single-problem FP32 golden test)
   alone vs in a mix               bit-identical (a chain's reductions depend
                                   only on its own tiles)
+  bench workload, Ns 10 and 12    FP64: E0 1e-9, G 1e-8; FP32: E0 1e-6, G 1e-4
+  vs the oracle fixture           (the Ns <= 8 bound: the fixture's Ns=12
+                                  sector (7,6) at U = 5.4 has a pole near
+                                  zero, |G(i w_0)| = 1.87, where FP32 pole
+                                  positions give 2e-5)
 """
+import os
 import time
 
 import numpy as np
@@ -387,3 +393,26 @@ def test_large_problems_admitted_one_at_a_time(edgpu, oracle):
         assert res[i]["e0"] == alone[0]["e0"]
         assert np.array_equal(res[i]["G"], alone[0]["G"])
 
+
+
+@pytest.mark.parametrize("precision,etol,gtol", [("fp64", 1e-9, 1e-8), ("fp32", 1e-6, 1e-4)])
+def test_bench_workload_vs_oracle_fixture(edgpu, precision, etol, gtol):
+    """The first 24 problems of the bench's 512-problem workload (Ns 8, 10,
+    12; 200 ground-state steps, 150-step chains, 2048 Matsubara points) in
+    one batch; those of the Ns = 10 and 12 classes that
+    tests/golden/batched_oracle.npz holds (make_batched.py: the FP64
+    oracle's solves, one sector with N_up != N_dn per class) against it."""
+    from edgpu import workloads as W
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "batched_oracle.npz"))
+    probs = W.batched_problems(edgpu, n=512)[:24]
+    idx = [int(i) for i in fx["index"]]
+    assert max(idx) < len(probs)
+    print(f"bench workload vs oracle fixture {precision}: problems {idx}", flush=True)
+    res, _, _ = run_batch(edgpu, probs, fx["omega"], precision=precision)
+    for k, i in enumerate(idx):
+        r, pr = res[i], probs[i]
+        de = abs(r["e0"] - fx["E0"][k])
+        dg = np.abs(r["G"][:, 0] - fx["G"][k]).max()
+        print(f"problem {i} Ns={pr.nbath + 1} ({pr.nup},{pr.ndn}): dE0 {de:.2e} dG {dg:.2e}", flush=True)
+        assert de < etol
+        assert dg < gtol
