@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <mutex>
 #include <thread>
 #include <string>
@@ -682,7 +683,7 @@ int chain_ab(const double *h, int m, double *a, double *bb)
 // tridiagonal matrix, staged as combine coefficients z_j s_j.  Touches only
 // the problem's own host data, so several problems run it on threads at
 // once.  Returns 0, 1 (breakdown at the start) or 2 (eigensolve failed).
-int gs_eigen(Job *J)
+int gs_eigen_body(Job *J)
 {
     const int m = J->p.maxnstep;
     std::vector<double> a(m), bb(m);
@@ -696,6 +697,14 @@ int gs_eigen(Job *J)
     double *hc = J->host + chain_host_n(m) + 8;
     for (int j = 0; j < nst; ++j) hc[j] = z[j] * s[j];
     return 0;
+}
+int gs_eigen(Job *J)
+{
+    try {
+        return gs_eigen_body(J);
+    } catch (const std::exception &) {  // host memory: reported as a failed eigensolve
+        return 2;
+    }
 }
 
 // Device half: combine, Rayleigh quotient and the c+/c start vectors.
@@ -936,11 +945,19 @@ int progress(edg_batch *b)
     if (nt == 1) {
         for (int k = 0; k < nr; ++k) erc[k] = gs_eigen(gs_ready[k]);
     } else {
+        // threads t < started take problems t, t + nt, ...; the host thread
+        // takes the rest when the system refuses a thread
         std::vector<std::thread> th;
-        for (int t = 0; t < nt; ++t)
-            th.emplace_back([&, t]() {
-                for (int k = t; k < nr; k += nt) erc[k] = gs_eigen(gs_ready[k]);
-            });
+        int started = 0;
+        try {
+            for (; started < nt; ++started)
+                th.emplace_back([&, t = started]() {
+                    for (int k = t; k < nr; k += nt) erc[k] = gs_eigen(gs_ready[k]);
+                });
+        } catch (const std::exception &) {
+        }
+        for (int t = started; t < nt; ++t)
+            for (int k = t; k < nr; k += nt) erc[k] = gs_eigen(gs_ready[k]);
         for (auto &x : th) x.join();
     }
     for (int k = 0; k < nr; ++k)
