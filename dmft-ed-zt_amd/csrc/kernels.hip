@@ -131,7 +131,7 @@ static int finish_hv(hipStream_t s, int mode, int nblk, const HvArgs &a)
 {
     if (mode == HV_RQ)
         if (launch_finish<3>(s, a.partial, FinArgs{FIN_COPY, nblk, nullptr, a.result, nullptr,
-                                                   nullptr, nullptr, 0, 0, nullptr}))
+                                                   nullptr, nullptr, 0, 0, nullptr, nullptr}))
             return -1;
     return nblk;
 }
@@ -140,7 +140,7 @@ int launch_alpha(hipStream_t s, const double *partial, int nblk, const double *s
                  double *alpha_j, int *stop)
 {
     return launch_finish<1>(s, partial, FinArgs{FIN_ALPHA, nblk, scale_j, alpha_j, nullptr, stop,
-                                                nullptr, 0, 0, nullptr});
+                                                nullptr, 0, 0, nullptr, nullptr});
 }
 
 int launch_fused_finish(hipStream_t s, const double *partial, int nblk, int step, double *alpha,
@@ -975,7 +975,7 @@ int launch_norm(hipStream_t s, const T *x, long long n, double *partial,
     hipLaunchKernelGGL(k_norm<T>, dim3(grid), dim3(kBlock), 0, s, x, n, partial, stop);
     if (hipGetLastError() != hipSuccess) return -1;
     return launch_finish<1>(s, partial, FinArgs{FIN_NORM, grid, nullptr, norm_out, scale_out,
-                                                nullptr, nullptr, 0, 0});
+                                                nullptr, nullptr, 0, 0, nullptr, nullptr});
 }
 template int launch_norm<float>(hipStream_t, const float *, long long, double *,
                                 double *, double *, const int *);
@@ -1118,7 +1118,7 @@ int launch_start(hipStream_t s, T *x, long long n, unsigned long long seed, doub
     if (hipGetLastError() != hipSuccess) return -1;
     if (!partial) return 0;
     return launch_finish<1>(s, partial, FinArgs{FIN_NORM, grid, nullptr, norm_out, scale_out,
-                                                nullptr, nullptr, 0, 0});
+                                                nullptr, nullptr, 0, 0, nullptr, nullptr});
 }
 template int launch_start<float>(hipStream_t, float *, long long, unsigned long long, double *, double *,
                                  double *);
@@ -1171,7 +1171,7 @@ int launch_combine(hipStream_t s, T *out, const T *basis, long long stride, int 
                        nvec, coef, n, partial);
     if (hipGetLastError() != hipSuccess) return -1;
     return launch_finish<1>(s, partial, FinArgs{FIN_NORM, grid, nullptr, norm_out, scale_out,
-                                                nullptr, nullptr, 0, 0});
+                                                nullptr, nullptr, 0, 0, nullptr, nullptr});
 }
 template int launch_combine<float>(hipStream_t, float *, const float *, long long, int,
                                    const double *, long long, double *, double *,
@@ -1319,7 +1319,7 @@ int launch_apply_c(hipStream_t s, const T *in, T *out, int nup_out, int ndn_out,
     if (hipGetLastError() != hipSuccess) return -1;
     if (!partial) return 0;
     return launch_finish<1>(s, partial, FinArgs{FIN_NORM, grid, nullptr, norm_out, scale_out,
-                                                nullptr, nullptr, 0, 0});
+                                                nullptr, nullptr, 0, 0, nullptr, nullptr});
 }
 template int launch_apply_c<float>(hipStream_t, const float *, float *, int, int, int, int,
                                    const uint32_t *, int, double *, double *, double *);
@@ -1370,7 +1370,7 @@ int launch_observables(hipStream_t s, const T *gs, const HvOp &op, const double 
                        partial);
     if (hipGetLastError() != hipSuccess) return -1;
     return launch_finish<3 * kMaxNorb>(s, partial, FinArgs{FIN_OBS, grid, nullptr, out, nullptr,
-                                                           nullptr, nullptr, 0, op.norb});
+                                                           nullptr, nullptr, 0, op.norb, nullptr, nullptr});
 }
 template int launch_observables<float>(hipStream_t, const float *, const HvOp &, const double *,
                                        double *, double *);
