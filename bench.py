@@ -315,6 +315,7 @@ def bench_batched(edgpu, W, S, graphs=True):
                 problems=len(probs), ns_counts={str(k): ns.count(k) for k in (8, 10, 12)},
                 schedule=[list(e) for e in sched],
                 solves_per_s=len(probs) / wall, wall_s=wall, runs_solves_per_s=[len(probs) / r[0] for r in runs],
+                host_s={k[5:-2]: st[k] for k in ("host_admit_s", "host_progress_s", "host_steps_s", "host_wait_s")},
                 batched_steps=st["steps"],
                 chain_steps_per_s=st["chain_steps"] / wall,
                 e0_finite=bool(np.all(np.isfinite([r["e0"] for r in res]))))
