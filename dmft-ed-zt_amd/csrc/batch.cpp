@@ -222,6 +222,9 @@ struct edg_batch {
     double *side_part[kSides] = {};  // partial-sum scratch of each side stream's stage kernels
     int next_lane = 0;
     hipEvent_t chunk_ev[4] = {};
+    // direct chains split over the step stream and the last side stream
+    // within a chunk: fork (step stream -> side) and join (side -> step)
+    hipEvent_t dfork = nullptr, djoin = nullptr;
     int chunk_head = 0, chunk_n = 0;
     bool fp64 = false;
     size_t es = 4;
@@ -1152,7 +1155,7 @@ int enqueue_chunk(edg_batch *b, int k)
 // edg_cluster_green (lanczos_enqueue in capi.cpp), with that path's default
 // schedule.
 template <typename T>
-int direct_step(edg_batch *b, Chain &C, int j)
+int direct_step(edg_batch *b, hipStream_t st, Chain &C, int j)
 {
     BInst &I = C.bi;
     T *base = static_cast<T *>(I.base);
@@ -1172,14 +1175,14 @@ int direct_step(edg_batch *b, Chain &C, int j)
     int cols;
     bool acc64;
     default_schedule(I.op.norb, !b->fp64, &cols, &acc64);  // the single-problem path's schedule
-    const int hv_n = launch_hv<T>(b->stream, I.op, HV_LANCZOS, cols, acc64, ha);
+    const int hv_n = launch_hv<T>(st, I.op, HV_LANCZOS, cols, acc64, ha);
     if (hv_n < 0 || hv_n > I.hvn) return fail(b, EDG_ERR_HIP, "direct H.v launch failed");
     if (j == I.maxn - 1) {
-        if (launch_alpha(b->stream, I.hvpart, hv_n, &I.scale[j], &I.alpha[j], I.flags))
+        if (launch_alpha(st, I.hvpart, hv_n, &I.scale[j], &I.alpha[j], I.flags))
             return fail(b, EDG_ERR_HIP, "direct alpha launch failed");
         return 0;
     }
-    const int ax = launch_axpy_norm<T>(b->stream, slot(j + 1), slot(j), C.D, I.hvpart, hv_n, I.scale, I.alpha, j,
+    const int ax = launch_axpy_norm<T>(st, slot(j + 1), slot(j), C.D, I.hvpart, hv_n, I.scale, I.alpha, j,
                                        I.axpart, I.flags);
     if (ax < 0 || ax > I.axn) return fail(b, EDG_ERR_HIP, "direct axpy launch failed");
     C.ax_n = ax;
@@ -1187,15 +1190,31 @@ int direct_step(edg_batch *b, Chain &C, int j)
 }
 
 // k steps of every direct chain, after the batched launches of the chunk
-// (outside any graph: their arguments change from chunk to chunk).
+// (outside any graph: their arguments change from chunk to chunk).  With two
+// or more, every other chain steps on the last side stream, between a fork
+// and a join event, so two chains' launches overlap (each fills the other's
+// kernel tails, as the single-problem path's c+ and c chains do).
 int run_direct(edg_batch *b, int k)
 {
-    for (Chain *C : b->direct)
+    const bool two = b->direct.size() >= 2;
+    hipStream_t s2 = b->side[kSides - 1];
+    if (two) {
+        BCHK(b, hipEventRecord(b->dfork, b->stream));
+        BCHK(b, hipStreamWaitEvent(s2, b->dfork, 0));
+    }
+    for (size_t i = 0; i < b->direct.size(); ++i) {
+        Chain *C = b->direct[i];
+        hipStream_t st = two && (i & 1) ? s2 : b->stream;
         for (int sub = 0; sub < k; ++sub) {
-            const int rc = b->fp64 ? direct_step<double>(b, *C, C->launched + sub)
-                                   : direct_step<float>(b, *C, C->launched + sub);
+            const int rc = b->fp64 ? direct_step<double>(b, st, *C, C->launched + sub)
+                                   : direct_step<float>(b, st, *C, C->launched + sub);
             if (rc) return rc;
         }
+    }
+    if (two) {
+        BCHK(b, hipEventRecord(b->djoin, s2));
+        BCHK(b, hipStreamWaitEvent(b->stream, b->djoin, 0));
+    }
     b->st_steps += k;
     b->st_chain_steps += (double)k * b->direct.size();
     for (size_t i = 0; i < b->direct.size();) {
@@ -1328,6 +1347,8 @@ int edg_batch_create(int device, int precision, int nspin, int nw, const double 
     chk(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     for (hipStream_t &s : b->side) chk(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     for (auto &e : b->chunk_ev) chk(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    chk(hipEventCreateWithFlags(&b->dfork, hipEventDisableTiming));
+    chk(hipEventCreateWithFlags(&b->djoin, hipEventDisableTiming));
     chk(hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, device));
     chk(hipMalloc((void **)&b->omega, sizeof(double) * nw));
     for (double *&p : b->side_part) chk(hipMalloc((void **)&p, sizeof(double) * kPartialCap));
@@ -1391,6 +1412,8 @@ int edg_batch_destroy(edg_batch *b)
         if (p) (void)hipFree(p);
     for (auto &e : b->chunk_ev)
         if (e) (void)hipEventDestroy(e);
+    if (b->dfork) (void)hipEventDestroy(b->dfork);
+    if (b->djoin) (void)hipEventDestroy(b->djoin);
     for (hipStream_t s : b->side)
         if (s) (void)hipStreamDestroy(s);
     if (b->stream) (void)hipStreamDestroy(b->stream);
