@@ -69,11 +69,13 @@ constexpr size_t kDescMin = 1 << 20;   // initial bytes of the descriptor buffer
 constexpr int kAhead = 2;              // chunks the host may run ahead of the device
 constexpr int kChunkEvents = 4;        // > kAhead
 // Problems whose state vector takes kLargeVec bytes or more are admitted at
-// most kMaxLarge at a time: several such vectors in flight compete for the
-// MALL (a batch of 8 Ns=14 problems, 47 MB FP32 vectors, measured 41.5 ms per
-// problem admitted one at a time against 44.8 all at once)
+// most kMaxLarge at a time: many such vectors in flight compete for the MALL
+// (a batch of 8 Ns=14 problems, 47 MB FP32 vectors, measured 41.5 ms per
+// problem admitted one at a time against 44.8 all at once), while two let
+// one problem's ground-state chain run beside the other's c+/c chains
+// (four Ns=14 problems 18.1-18.2 -> 18.6-18.9 solves/s against one at a time)
 constexpr size_t kLargeVec = size_t(32) << 20;
-constexpr int kMaxLarge = 1;
+constexpr int kMaxLarge = 2;
 // Chains of kDirectMin elements or more run their steps as launches of the
 // single-problem kernels (edg_cluster_green's, arguments from the host)
 // instead of the batched ones: one chain's H.v then fills the GPU, and the

@@ -319,8 +319,8 @@ int edg_batch_submit(edg_batch *b, const edg_problem *p, long long *id);
  * that are ready and up to max_steps batched Lanczos steps.  *pending
  * receives the number of submitted problems not yet finished.  Admission
  * follows submission order by device bytes, except that a problem whose
- * state vector takes 32 MiB or more waits while another such problem runs,
- * and smaller problems behind it are admitted meanwhile. */
+ * state vector takes 32 MiB or more waits while two other such problems
+ * run, and smaller problems behind it are admitted meanwhile. */
 int edg_batch_step(edg_batch *b, int max_steps, int *pending);
 /* Run until every submitted problem is finished. */
 int edg_batch_run(edg_batch *b);

@@ -370,20 +370,22 @@ def test_ground_state_sweep_matches_single_path(edgpu, oracle):
     assert ogs.isector == isec and abs(ogs.val - e0) < 1e-9
 
 
-def test_large_problems_admitted_one_at_a_time(edgpu, oracle):
-    """Two Ns=14 problems (FP32 state vectors of 47 MB, above the 32 MiB
+def test_large_problems_admitted_two_at_a_time(edgpu, oracle):
+    """Three Ns=14 problems (FP32 state vectors of 47 MB, above the 32 MiB
     large-problem line) and two Ns=10 problems submitted in that order: the
-    first step admits the first large problem and both small ones, holding
-    the second large one; every result equals the same problem run alone."""
+    first step admits the first two large problems and both small ones,
+    holding the third large one; every result equals the same problem run
+    alone."""
     omega = oracle.matsubara(64, 50.0)
     probs = [problem(edgpu, 13, 2.0, 71, maxnstep=30, maxnstep_green=20),
              problem(edgpu, 13, 3.0, 72, maxnstep=30, maxnstep_green=20),
+             problem(edgpu, 13, 4.0, 75, maxnstep=30, maxnstep_green=20),
              problem(edgpu, 9, 2.0, 73, maxnstep=30, maxnstep_green=20),
              problem(edgpu, 9, 1.0, 74, maxnstep=30, maxnstep_green=20)]
     b = edgpu.Batch(omega, precision="fp32")
     ids = [b.submit(p) for p in probs]
     pending = b.step(1)
-    assert b.stats()["in_flight"] == 3
+    assert b.stats()["in_flight"] == 4
     while pending:
         pending = b.step(16)
     res = {r["id"]: r for r in b.poll()}
