@@ -213,12 +213,14 @@ struct edg_batch {
     // stream: the batched Lanczos steps (and the reads of finished chains);
     // side[lane]: per-problem stage work (tables, start vectors, combine, RQ,
     // apply_c, continued fractions) of the problems of that lane, which
-    // overlaps the steps and the other lanes' stages.  The streams
-    // never wait on each other: the host moves a problem to its next stage,
-    // and lets its new chains join the steps, only after an event shows the
-    // previous stage's work is done.  The host stays at most kAhead chunks
-    // of kChunk steps ahead of the device, so chains that become ready join
-    // within a few steps.
+    // overlaps the steps and the other lanes' stages.  Stage streams never
+    // wait on each other: the host moves a problem to its next stage, and
+    // lets its new chains join the steps, only after an event shows the
+    // previous stage's work is done.  The one device-side wait is the fork
+    // and join around the direct chains' steps that run on the last side
+    // stream (run_direct).  The host stays at most kAhead chunks of kChunk
+    // steps ahead of the device, so chains that become ready join within a
+    // few steps.
     hipStream_t stream = nullptr;
     hipStream_t side[kSides] = {};
     double *side_part[kSides] = {};  // partial-sum scratch of each side stream's stage kernels
