@@ -41,7 +41,8 @@ The same JSON line also carries:
             the oracle's ms per Lanczos step on the host cores;
   batched_kanamori  two-orbital problems through the batched solver (all 8
             chains of a problem in shared launches) against the same
-            problems solved one by one: 4 at Ns=14, 48 at Ns=10;
+            problems solved one by one: 4 at Ns=14, 48 at Ns=10 (skipped,
+            like `batched`, with --no-batched);
   dmft_ns6  the Ns=6 problem through the DMFT loop (bath fit on the
             host) with the FP64 GPU solver and with the oracle: iterations,
             time per solve and per iteration.
@@ -602,8 +603,9 @@ def main():
     if not args.no_kanamori:
         log("two-orbital Kanamori")
         extra["kanamori"] = bench_kanamori(edgpu, W, ref)
-        log("batched two-orbital Kanamori")
-        extra["batched_kanamori"] = bench_batched_kanamori(edgpu, W, ref)
+        if not args.no_batched:
+            log("batched two-orbital Kanamori")
+            extra["batched_kanamori"] = bench_batched_kanamori(edgpu, W, ref)
         if not args.no_cpu_baseline:
             ms = cpu_kanamori_step_ms(W, 3, threads)
             extra["kanamori"].update(cpu_ms_per_lanczos_step=ms, cpu_cores=threads,
